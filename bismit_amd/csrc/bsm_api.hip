@@ -1,0 +1,656 @@
+// bsm_api.hip -- context, area construction, buffer I/O, rnd schedule and
+// synapse layout generation for the MI355X cortical-cycle engine.
+//
+// Replaces the reference's per-component ocl Buffer construction and host I/O
+// (src/cortex/{synapses,dendrites,tufts,pyramidals,spiny_stellates,axon_space}.rs)
+// with one device arena per area, and its entropy-seeded SmallRng draws with a
+// documented deterministic xorshift128 stream.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bsm_internal.h"
+
+namespace bsm {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(const std::string& msg) {
+    g_err = msg;
+    return 1;
+}
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return 2;
+}
+
+// splitmix64 (Steele, Lea, Flood 2014) and xorshift128 (Marsaglia 2003).
+static inline uint64_t splitmix64_next(uint64_t* x) {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void rng_seed(uint32_t st[4], uint64_t seed) {
+    uint64_t x = seed;
+    uint64_t a = splitmix64_next(&x), b = splitmix64_next(&x);
+    st[0] = (uint32_t)a;
+    st[1] = (uint32_t)(a >> 32);
+    st[2] = (uint32_t)b;
+    st[3] = (uint32_t)(b >> 32);
+    if (!(st[0] | st[1] | st[2] | st[3])) st[0] = 1;
+}
+
+uint32_t rng_next(uint32_t st[4]) {
+    uint32_t t = st[0] ^ (st[0] << 11);
+    st[0] = st[1];
+    st[1] = st[2];
+    st[2] = st[3];
+    st[3] = st[3] ^ (st[3] >> 19) ^ (t ^ (t >> 8));
+    return st[3];
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------- validation
+static int validate(const bsm_area_desc* d) {
+    if (!d) return fail("null area descriptor");
+    if (d->v_size == 0 || d->u_size == 0 || d->v_size > 65535 || d->u_size > 65535)
+        return fail("area v_size/u_size must be in [1, 65535]");
+    if (d->n_slices == 0 || d->n_slices > BSM_MAX_SLICES || !d->slices)
+        return fail("n_slices must be in [1, 255]");
+    if (d->n_layers > BSM_MAX_LAYERS || (d->n_layers && !d->layers))
+        return fail("n_layers must be <= 8");
+    uint64_t axn = 0;
+    for (uint32_t s = 0; s < d->n_slices; s++) {
+        const bsm_slice_desc& sl = d->slices[s];
+        if (sl.v_size == 0 || sl.u_size == 0 || sl.v_size > 65535 || sl.u_size > 65535)
+            return fail("slice " + std::to_string(s) + ": sizes must be in [1, 65535]");
+        axn += (uint64_t)sl.v_size * sl.u_size;
+    }
+    if (axn >= (1ull << 31)) return fail("axon space too large");
+    for (uint32_t l = 0; l < d->n_layers; l++) {
+        const bsm_layer_desc& L = d->layers[l];
+        std::string p = "layer " + std::to_string(l) + ": ";
+        if (L.kind != BSM_LAYER_SSC && L.kind != BSM_LAYER_PYR) return fail(p + "bad kind");
+        if (L.depth == 0) return fail(p + "depth must be >= 1");
+        if ((uint32_t)L.base_axn_slc + L.depth > d->n_slices) return fail(p + "slices out of range");
+        for (uint32_t s = L.base_axn_slc; s < (uint32_t)L.base_axn_slc + L.depth; s++) {
+            const bsm_slice_desc& sl = d->slices[s];
+            if (sl.v_size != d->v_size || sl.u_size != d->u_size || sl.v_scale != 16 ||
+                sl.u_scale != 16 || sl.v_mid != 0 || sl.u_mid != 0)
+                return fail(p + "own axon slices must be local spatial slices (area dims, "
+                                "scale 16, mid 0)");
+        }
+        if (L.kind == BSM_LAYER_SSC) {
+            // src/cortex/spiny_stellates.rs:54-56: exactly one tuft.
+            if (L.n_tufts != 1) return fail(p + "spiny stellate layers take exactly 1 tuft");
+            if (!L.spatial) return fail(p + "spiny stellate layers must carry the SPATIAL role");
+            if (L.control != BSM_CTRL_INHIB_SIMPLE && L.control != BSM_CTRL_NONE)
+                return fail(p + "spiny stellate control must be INHIB_SIMPLE or NONE");
+        } else {
+            if (L.n_tufts < 1 || L.n_tufts > 3) return fail(p + "pyramidal layers take 1..3 tufts");
+            if (L.spatial) return fail(p + "pyramidal layers must not carry the SPATIAL role");
+            if (L.control != BSM_CTRL_INTRA_COLUMN && L.control != BSM_CTRL_PYR_OUTPUT &&
+                L.control != BSM_CTRL_NONE)
+                return fail(p + "pyramidal control must be INTRA_COLUMN, PYR_OUTPUT or NONE");
+            // src/cortex/pyramidals.rs:88-126: at most one tuft of each kind.
+            int seen[3] = {0, 0, 0};
+            for (uint32_t t = 0; t < L.n_tufts; t++) {
+                const bsm_tuft_desc& T = L.tufts[t];
+                int k = (T.den_class == BSM_DEN_BASAL && T.den_kind == BSM_DEN_PROXIMAL) ? 0
+                        : (T.den_class == BSM_DEN_BASAL && T.den_kind == BSM_DEN_DISTAL)  ? 1
+                        : (T.den_class == BSM_DEN_APICAL && T.den_kind == BSM_DEN_DISTAL) ? 2
+                                                                                           : -1;
+                if (k < 0) return fail(p + "unsupported tuft class/kind");
+                if (seen[k]++) return fail(p + "duplicate tuft kind");
+            }
+        }
+        if (L.control == BSM_CTRL_INHIB_SIMPLE && (L.inhib_radius < 0 || L.inhib_radius > 64))
+            return fail(p + "inhib_radius must be in [0, 64]");
+        uint64_t cells = (uint64_t)L.depth * d->v_size * d->u_size;
+        uint64_t ns = 0;
+        for (uint32_t t = 0; t < L.n_tufts; t++) {
+            const bsm_tuft_desc& T = L.tufts[t];
+            if (T.dens_per_tft == 0 || T.dens_per_tft > 255)
+                return fail(p + "dens_per_tft must be in [1, 255]");
+            if (T.syns_per_den == 0 || T.syns_per_den > 256)
+                return fail(p + "syns_per_den must be in [1, 256]");
+            if (T.max_active_dens_l2 > 23) return fail(p + "max_active_dens_l2 too large");
+            if (T.n_src_slcs > BSM_MAX_TUFT_SRCS) return fail(p + "too many tuft sources");
+            for (uint32_t k = 0; k < T.n_src_slcs; k++)
+                if (T.src_slc_ids[k] >= d->n_slices) return fail(p + "tuft source slice out of range");
+            ns += cells * T.dens_per_tft * T.syns_per_den;
+        }
+        if (ns >= (1ull << 32)) return fail(p + "more than 2^32 synapses in one layer");
+    }
+    return 0;
+}
+
+struct ArenaPlan {
+    std::vector<std::pair<void**, size_t>> parts;
+    size_t total = 0;
+    void add(void* pp, size_t bytes) {
+        parts.push_back({(void**)pp, bytes});
+        total = align_up(total, 256) + align_up(bytes ? bytes : 1, 256);
+    }
+};
+
+static void plan_area(bsm_area* a, ArenaPlan& ap) {
+    ap.add(&a->axn, a->axn_len);
+    ap.add(&a->slices_d, sizeof(SliceRt) * a->n_slices);
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        size_t ns = L.n_syn, nd = L.n_den, nct = L.n_celtft, C = L.cells;
+        ap.add(&L.syn_state[0], ns);
+        ap.add(&L.syn_state[1], ns);
+        ap.add(&L.syn_flag_static, ns);
+        ap.add(&L.syn_slc, ns);
+        ap.add(&L.syn_v, ns);
+        ap.add(&L.syn_u, ns);
+        ap.add(&L.syn_str, ns);
+        ap.add(&L.den_states, nd);
+        ap.add(&L.den_raw, nd);
+        ap.add(&L.den_energies, nd);
+        ap.add(&L.den_act, nd);
+        ap.add(&L.den_thr, nd);
+        for (int k = 0; k < 8; k++) ap.add(&L.tft[k], nct);
+        ap.add(&L.soma, C);
+        ap.add(&L.cel_flags, C);
+        ap.add(&L.cel_energies, C);
+        ap.add(&L.cel_act, C);
+    }
+}
+
+static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
+    uint32_t slot = 0;
+    for (uint32_t l = 0; l < d->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        L.d = d->layers[l];
+        L.cells = (uint32_t)L.d.depth * d->v_size * d->u_size;
+        uint64_t ns = 0, nd = 0;
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+            TuftRt& T = L.t[t];
+            T.d = L.d.tufts[t];
+            T.spt = T.d.dens_per_tft * T.d.syns_per_den;
+            T.syn_idz = ns;
+            T.den_idz = nd;
+            ns += (uint64_t)L.cells * T.spt;
+            nd += (uint64_t)L.cells * T.d.dens_per_tft;
+        }
+        L.n_syn = ns;
+        L.n_den = nd;
+        L.n_celtft = (uint64_t)L.cells * L.d.n_tufts;
+        L.cur = 0;
+        L.slot0 = slot;
+        slot += 2u * L.d.n_tufts + 1u;
+    }
+    a->n_slots = slot;
+}
+
+}  // namespace bsm
+
+using namespace bsm;
+
+extern "C" {
+
+const char* bsm_last_error(void) { return g_err.c_str(); }
+
+uint64_t bsm_splitmix64(uint64_t* state) { return splitmix64_next(state); }
+
+int bsm_xorshift128_fill(uint64_t seed, uint32_t* out, uint64_t n) {
+    if (!out && n) return fail("null output");
+    uint32_t st[4];
+    rng_seed(st, seed);
+    for (uint64_t i = 0; i < n; i++) out[i] = rng_next(st);
+    return 0;
+}
+
+int bsm_ctx_create(int device, bsm_ctx** out) {
+    if (!out) return fail("null out");
+    int n = 0;
+    BSM_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail("no such HIP device");
+    BSM_HIP(hipSetDevice(device));
+    bsm_ctx* c = new bsm_ctx();
+    c->device = device;
+    hipError_t e = hipGetDeviceProperties(&c->prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipGetDeviceProperties");
+    }
+    *out = c;
+    return 0;
+}
+
+int bsm_ctx_destroy(bsm_ctx* ctx) {
+    delete ctx;
+    return 0;
+}
+
+int bsm_sync(bsm_ctx* ctx) {
+    if (!ctx) return fail("null ctx");
+    BSM_HIP(hipSetDevice(ctx->device));
+    BSM_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+int bsm_device_info(bsm_ctx* ctx, char* name_buf, size_t name_len, int* cu_count,
+                    size_t* total_mem) {
+    if (!ctx) return fail("null ctx");
+    if (name_buf && name_len) {
+        std::snprintf(name_buf, name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = ctx->prop.multiProcessorCount;
+    if (total_mem) *total_mem = ctx->prop.totalGlobalMem;
+    return 0;
+}
+
+int bsm_area_footprint(const bsm_area_desc* desc, size_t* bytes) {
+    if (validate(desc)) return 1;
+    bsm_area tmp{};
+    tmp.n_slices = desc->n_slices;
+    tmp.n_layers = desc->n_layers;
+    uint64_t tot = 0;
+    for (uint32_t s = 0; s < desc->n_slices; s++)
+        tot += (uint64_t)desc->slices[s].v_size * desc->slices[s].u_size;
+    tmp.axn_len = (uint32_t)tot;
+    fill_layers(&tmp, desc);
+    ArenaPlan ap;
+    plan_area(&tmp, ap);
+    *bytes = ap.total;
+    return 0;
+}
+
+int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, bsm_area** out) {
+    if (!ctx || !out) return fail("null ctx/out");
+    if (validate(desc)) return 1;
+    BSM_HIP(hipSetDevice(ctx->device));
+    bsm_area* a = new bsm_area();
+    a->ctx = ctx;
+    a->V = desc->v_size;
+    a->U = desc->u_size;
+    a->n_slices = desc->n_slices;
+    a->n_layers = desc->n_layers;
+    a->settings = desc->settings;
+    a->slice_desc.assign(desc->slices, desc->slices + desc->n_slices);
+    uint32_t tot = 0;
+    for (uint32_t s = 0; s < desc->n_slices; s++) {
+        const bsm_slice_desc& sl = desc->slices[s];
+        SliceRt r{};
+        r.idz = tot;
+        r.v_size = (uint16_t)sl.v_size;
+        r.u_size = (uint16_t)sl.u_size;
+        r.v_scale = sl.v_scale;
+        r.u_scale = sl.u_scale;
+        r.v_mid = sl.v_mid;
+        r.u_mid = sl.u_mid;
+        a->slices_h.push_back(r);
+        tot += sl.v_size * sl.u_size;
+    }
+    a->axn_len = tot;
+    fill_layers(a, desc);
+    rng_seed(a->rng, rnd_seed);
+    ArenaPlan ap;
+    plan_area(a, ap);
+    hipError_t e = hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete a;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    e = hipMalloc((void**)&a->arena, ap.total);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(a->stream);
+        delete a;
+        return hip_fail(e, "hipMalloc(area arena)");
+    }
+    a->arena_bytes = ap.total;
+    size_t off = 0;
+    for (auto& p : ap.parts) {
+        off = align_up(off, 256);
+        *p.first = a->arena + off;
+        off += align_up(p.second ? p.second : 1, 256);
+    }
+    // Buffers start zero-filled like Buffer::builder().fill_val(0).
+    BSM_HIP(hipMemsetAsync(a->arena, 0, ap.total, a->stream));
+    BSM_HIP(hipMemcpyAsync(a->slices_d, a->slices_h.data(), sizeof(SliceRt) * a->n_slices,
+                           hipMemcpyHostToDevice, a->stream));
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    BSM_HIP(hipEventCreate(&a->ev0));
+    BSM_HIP(hipEventCreate(&a->ev1));
+    for (int i = 0; i < 2; i++) {
+        BSM_HIP(hipEventCreateWithFlags(&a->rnd_ev[i], hipEventDisableTiming));
+        a->rnd_pin[i] = nullptr;
+        a->rnd_pin_cap[i] = 0;
+    }
+    a->rnd_d = nullptr;
+    a->rnd_cap = 0;
+    a->rnd_pin_next = 0;
+    a->tjobs_d = nullptr;
+    a->sjobs_d = nullptr;
+    a->pjobs_d = nullptr;
+    a->items_d = nullptr;
+    a->plan_dirty = true;
+    a->batch_stream = nullptr;
+    *out = a;
+    return 0;
+}
+
+int bsm_area_destroy(bsm_area* a) {
+    if (!a) return 0;
+    (void)hipSetDevice(a->ctx->device);
+    (void)hipStreamSynchronize(a->stream);
+    (void)hipFree(a->arena);
+    (void)hipFree(a->rnd_d);
+    (void)hipFree(a->tjobs_d);
+    (void)hipFree(a->sjobs_d);
+    (void)hipFree(a->pjobs_d);
+    (void)hipFree(a->items_d);
+    for (int i = 0; i < 2; i++) {
+        if (a->rnd_pin[i]) hipHostFree(a->rnd_pin[i]);
+        (void)hipEventDestroy(a->rnd_ev[i]);
+    }
+    (void)hipEventDestroy(a->ev0);
+    (void)hipEventDestroy(a->ev1);
+    (void)hipStreamDestroy(a->stream);
+    delete a;
+    return 0;
+}
+
+int bsm_area_sync(bsm_area* a) {
+    if (!a) return fail("null area");
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    return 0;
+}
+
+int bsm_area_axn_len(bsm_area* a, uint32_t* len) {
+    if (!a || !len) return fail("null arg");
+    *len = a->axn_len;
+    return 0;
+}
+
+int bsm_area_syn_count(bsm_area* a, uint64_t* n) {
+    if (!a || !n) return fail("null arg");
+    uint64_t k = 0;
+    for (uint32_t l = 0; l < a->n_layers; l++) k += a->L[l].n_syn;
+    *n = k;
+    return 0;
+}
+
+int bsm_area_slice_idz(bsm_area* a, uint32_t slc, uint32_t* idz) {
+    if (!a || !idz) return fail("null arg");
+    if (slc >= a->n_slices) return fail("slice out of range");
+    *idz = a->slices_h[slc].idz;
+    return 0;
+}
+
+// Resolve (buf, layer) to a device range.  SYN_FLAG_SETS is virtual: static
+// bits in syn_flag_static, SYN_PREV_ACTIVE from the previous state buffer.
+static int resolve(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len) {
+    if (buf == BSM_AXN_STATES) {
+        *ptr = a->axn;
+        *len = a->axn_len;
+        return 0;
+    }
+    if (layer >= a->n_layers) return fail("layer out of range");
+    LayerRt& L = a->L[layer];
+    bool ssc = L.d.kind == BSM_LAYER_SSC;
+    size_t nct = L.n_celtft;
+    switch (buf) {
+    case BSM_SOMA_STATES:
+        if (ssc) { *ptr = L.den_states; *len = L.n_den; }
+        else { *ptr = L.soma; *len = L.cells; }
+        return 0;
+    case BSM_SOMA_ENERGIES: *ptr = L.cel_energies; *len = L.cells; return 0;
+    case BSM_SOMA_ACTIVITIES: *ptr = L.cel_act; *len = L.cells; return 0;
+    case BSM_SOMA_FLAG_SETS:
+        if (ssc) return fail("spiny stellate layers have no soma flag sets");
+        *ptr = L.cel_flags; *len = L.cells; return 0;
+    case BSM_TFT_STATES: case BSM_TFT_BEST_DEN_IDS: case BSM_TFT_BEST_DEN_STATES_RAW:
+    case BSM_TFT_BEST_DEN_STATES: case BSM_TFT_PREV_STATES: case BSM_TFT_PREV_BEST_DEN_IDS:
+    case BSM_TFT_PREV_BEST_DEN_STATES_RAW: case BSM_TFT_PREV_BEST_DEN_STATES: {
+        if (ssc) return fail("spiny stellate layers have no tuft buffers");
+        static const int map[8] = {3, 0, 1, 2, 7, 4, 5, 6};
+        *ptr = L.tft[map[buf - BSM_TFT_STATES]];
+        *len = nct;
+        return 0;
+    }
+    case BSM_DEN_STATES: *ptr = L.den_states; *len = L.n_den; return 0;
+    case BSM_DEN_STATES_RAW: *ptr = L.den_raw; *len = L.n_den; return 0;
+    case BSM_DEN_ENERGIES: *ptr = L.den_energies; *len = L.n_den; return 0;
+    case BSM_DEN_ACTIVITIES: *ptr = L.den_act; *len = L.n_den; return 0;
+    case BSM_DEN_THRESHOLDS: *ptr = L.den_thr; *len = L.n_den; return 0;
+    case BSM_SYN_STATES: *ptr = L.syn_state[L.cur]; *len = L.n_syn; return 0;
+    case BSM_SYN_STRENGTHS: *ptr = (uint8_t*)L.syn_str; *len = L.n_syn; return 0;
+    case BSM_SYN_SRC_SLC_IDS: *ptr = L.syn_slc; *len = L.n_syn; return 0;
+    case BSM_SYN_SRC_COL_V_OFFS: *ptr = (uint8_t*)L.syn_v; *len = L.n_syn; return 0;
+    case BSM_SYN_SRC_COL_U_OFFS: *ptr = (uint8_t*)L.syn_u; *len = L.n_syn; return 0;
+    case BSM_SYN_FLAG_SETS: *ptr = L.syn_flag_static; *len = L.n_syn; return 0;
+    default: return fail("unknown buffer kind");
+    }
+}
+
+int bsm_area_buf_len(bsm_area* a, int buf, uint32_t layer, size_t* len) {
+    if (!a || !len) return fail("null arg");
+    uint8_t* p;
+    return resolve(a, buf, layer, &p, len);
+}
+
+int bsm_area_read(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len, void* dst) {
+    if (!a || (!dst && len)) return fail("null arg");
+    uint8_t* p;
+    size_t n;
+    if (resolve(a, buf, layer, &p, &n)) return 1;
+    if (offset > n || len > n - offset) return fail("read range out of bounds");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
+    if (len == 0) return 0;
+    BSM_HIP(hipMemcpy(dst, p + offset, len, hipMemcpyDeviceToHost));
+    if (buf == BSM_SYN_FLAG_SETS) {
+        LayerRt& L = a->L[layer];
+        std::vector<uint8_t> prev(len);
+        BSM_HIP(hipMemcpy(prev.data(), L.syn_state[L.cur ^ 1] + offset, len,
+                          hipMemcpyDeviceToHost));
+        uint8_t* o = (uint8_t*)dst;
+        for (size_t i = 0; i < len; i++)
+            o[i] = (uint8_t)((o[i] & ~SYN_PREV_ACTIVE) | (prev[i] ? SYN_PREV_ACTIVE : 0));
+    }
+    return 0;
+}
+
+int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len,
+                   const void* src) {
+    if (!a || (!src && len)) return fail("null arg");
+    uint8_t* p;
+    size_t n;
+    if (resolve(a, buf, layer, &p, &n)) return 1;
+    if (offset > n || len > n - offset) return fail("write range out of bounds");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
+    if (len == 0) return 0;
+    if (buf == BSM_SYN_FLAG_SETS) {
+        LayerRt& L = a->L[layer];
+        const uint8_t* s = (const uint8_t*)src;
+        std::vector<uint8_t> stat(len), prev(len);
+        for (size_t i = 0; i < len; i++) {
+            stat[i] = (uint8_t)(s[i] & ~SYN_PREV_ACTIVE);
+            prev[i] = (s[i] & SYN_PREV_ACTIVE) ? 0x80 : 0;
+        }
+        BSM_HIP(hipMemcpy(p + offset, stat.data(), len, hipMemcpyHostToDevice));
+        BSM_HIP(hipMemcpy(L.syn_state[L.cur ^ 1] + offset, prev.data(), len,
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
+    BSM_HIP(hipMemcpy(p + offset, src, len, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
+    if (!a) return fail("null area");
+    uint8_t* p;
+    size_t n;
+    if (resolve(a, buf, layer, &p, &n)) return 1;
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
+    if (buf == BSM_SYN_FLAG_SETS) {
+        LayerRt& L = a->L[layer];
+        BSM_HIP(hipMemsetAsync(p, val & ~SYN_PREV_ACTIVE, n, a->stream));
+        BSM_HIP(hipMemsetAsync(L.syn_state[L.cur ^ 1], (val & SYN_PREV_ACTIVE) ? 0x80 : 0, n,
+                               a->stream));
+    } else {
+        BSM_HIP(hipMemsetAsync(p, val, n, a->stream));
+    }
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    return 0;
+}
+
+int bsm_area_intake(bsm_area* a, uint32_t axn_offset, const uint8_t* src, uint32_t len,
+                    int src_is_device) {
+    if (!a || (!src && len)) return fail("null arg");
+    if (axn_offset > a->axn_len || len > a->axn_len - axn_offset)
+        return fail("intake range out of the axon space");
+    if (!len) return 0;
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (src_is_device) {
+        BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, src, len, hipMemcpyDeviceToDevice, a->stream));
+    } else {
+        BSM_HIP(hipStreamSynchronize(a->stream));
+        BSM_HIP(hipMemcpy(a->axn + axn_offset, src, len, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int bsm_area_output(bsm_area* a, uint32_t axn_offset, uint8_t* dst, uint32_t len) {
+    return bsm_area_read(a, BSM_AXN_STATES, 0, axn_offset, len, dst);
+}
+
+int bsm_area_rnd_slots(bsm_area* a, uint32_t* n) {
+    if (!a || !n) return fail("null arg");
+    *n = a->n_slots;
+    return 0;
+}
+
+int bsm_area_rnd_slot_info(bsm_area* a, uint32_t slot, int* kind, uint32_t* layer,
+                           uint32_t* tuft) {
+    if (!a) return fail("null area");
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        const LayerRt& L = a->L[l];
+        uint32_t T = L.d.n_tufts;
+        if (slot >= L.slot0 && slot < L.slot0 + 2 * T + 1) {
+            uint32_t k = slot - L.slot0;
+            if (layer) *layer = l;
+            if (k < T) { if (kind) *kind = BSM_RND_DEN; if (tuft) *tuft = k; }
+            else if (k == T) { if (kind) *kind = BSM_RND_CTRL; if (tuft) *tuft = 0; }
+            else { if (kind) *kind = BSM_RND_MTP; if (tuft) *tuft = k - T - 1; }
+            return 0;
+        }
+    }
+    return fail("slot out of range");
+}
+
+int bsm_area_draw_rnd(bsm_area* a, int32_t* out, uint32_t n_cycles) {
+    if (!a || (!out && n_cycles)) return fail("null arg");
+    uint64_t n = (uint64_t)n_cycles * a->n_slots;
+    for (uint64_t i = 0; i < n; i++) out[i] = (int32_t)rng_next(a->rng);
+    return 0;
+}
+
+int bsm_area_set_tuning(bsm_area* a, const bsm_tuning* t) {
+    if (!a || !t) return fail("null arg");
+    if (a->batch_stream) return fail("area belongs to a batch");
+    Tuning nt = a->tune;
+    if (t->tile_v) nt.tile_v = t->tile_v;
+    if (t->tile_u) nt.tile_u = t->tile_u;
+    if (t->halo) nt.halo = t->halo;
+    if (t->chunk) nt.chunk = t->chunk;
+    if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
+        return fail("tile too large");
+    if (nt.halo > 64) return fail("halo too large");
+    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
+        return fail("chunk must be 1, 4, 8 or 16");
+    a->tune = nt;
+    a->plan_dirty = true;
+    return 0;
+}
+
+int bsm_area_get_tuning(bsm_area* a, bsm_tuning* t) {
+    if (!a || !t) return fail("null arg");
+    t->tile_v = a->tune.tile_v;
+    t->tile_u = a->tune.tile_u;
+    t->halo = a->tune.halo;
+    t->chunk = a->tune.chunk;
+    return 0;
+}
+
+int bsm_dev_alloc(bsm_ctx* ctx, size_t bytes, void** dptr) {
+    if (!ctx || !dptr) return fail("null arg");
+    BSM_HIP(hipSetDevice(ctx->device));
+    BSM_HIP(hipMalloc(dptr, bytes ? bytes : 1));
+    return 0;
+}
+
+int bsm_dev_free(bsm_ctx* ctx, void* dptr) {
+    if (!ctx) return fail("null ctx");
+    BSM_HIP(hipSetDevice(ctx->device));
+    BSM_HIP(hipFree(dptr));
+    return 0;
+}
+
+int bsm_dev_upload(bsm_ctx* ctx, void* dptr, const void* src, size_t bytes) {
+    if (!ctx) return fail("null ctx");
+    BSM_HIP(hipSetDevice(ctx->device));
+    BSM_HIP(hipMemcpy(dptr, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Deterministic layout: h = mix64(key + (i+1)*golden) per synapse, with the
+// splitmix64 finalizer; key = seed ^ ((layer*8 + tuft) << 56).
+int bsm_gen_tuft_layout(uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n_syns,
+                        uint32_t n_src, const uint8_t* src_ids, int32_t reach, uint8_t* out_slc,
+                        int8_t* out_v, int8_t* out_u) {
+    if (n_src == 0 || !src_ids) return fail("tuft has no source slices");
+    if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
+    if (n_syns && (!out_slc || !out_v || !out_u)) return fail("null output");
+    const uint64_t key = seed ^ ((uint64_t)(layer * 8u + tuft) << 56);
+    const uint32_t span = 2u * (uint32_t)reach + 1u;
+    for (uint64_t i = 0; i < n_syns; i++) {
+        uint64_t x = key + i * 0x9E3779B97F4A7C15ull;
+        uint64_t h = splitmix64_next(&x);
+        out_slc[i] = src_ids[(uint32_t)(h & 0xFFFF) % n_src];
+        out_v[i] = (int8_t)((int32_t)((uint32_t)((h >> 16) & 0xFFFF) % span) - reach);
+        out_u[i] = (int8_t)((int32_t)((uint32_t)((h >> 32) & 0xFFFF) % span) - reach);
+    }
+    return 0;
+}
+
+int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
+    if (!a) return fail("null area");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+            TuftRt& T = L.t[t];
+            uint64_t n = (uint64_t)L.cells * T.spt;
+            std::vector<uint8_t> s(n);
+            std::vector<int8_t> v(n), u(n);
+            if (bsm_gen_tuft_layout(seed, l, t, n, T.d.n_src_slcs, T.d.src_slc_ids, reach,
+                                    s.data(), v.data(), u.data()))
+                return 1;
+            BSM_HIP(hipMemcpy(L.syn_slc + T.syn_idz, s.data(), n, hipMemcpyHostToDevice));
+            BSM_HIP(hipMemcpy(L.syn_v + T.syn_idz, v.data(), n, hipMemcpyHostToDevice));
+            BSM_HIP(hipMemcpy(L.syn_u + T.syn_idz, u.data(), n, hipMemcpyHostToDevice));
+        }
+        BSM_HIP(hipMemsetAsync(L.syn_str, 0, L.n_syn, a->stream));
+    }
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    return 0;
+}
+
+}  // extern "C"

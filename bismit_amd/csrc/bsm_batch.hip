@@ -1,0 +1,398 @@
+// bsm_batch.hip -- batched form: many independent areas of uneven sizes step
+// through each fused stage in one persistent launch over a flattened work-item
+// list, with the whole cycle captured as a hipGraph.
+//
+// Per batch cycle (stream order == reference enqueue order per area,
+// src/cortex/cortical_area.rs:716-789):
+//   prologue : per area, draw its rnd slots from its xorshift128 stream and copy
+//              its next input frame into its axon space (AxonSpace::intake)
+//   for each spatial layer rank k : F1 over all areas' SSC tufts, F2 over all SSC layers
+//   for each pyramidal layer rank k: F1 over all areas' pyramidal tufts, F3
+//   tick     : advance the device frame counter
+// Each area's layers keep their own state-buffer parity through a per-job xor.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "bsm_internal.h"
+
+namespace bsm {
+
+struct AreaIo {
+    uint32_t* rng;          // device xorshift128 state (4 words)
+    int32_t* rnd;           // the area's per-cycle rnd slots
+    uint32_t n_slots;
+    uint32_t len;           // frame bytes (0: no intake)
+    const uint8_t* frames;  // device frame ring
+    uint64_t stride;
+    uint8_t* dst;           // axon space + offset
+    uint32_t n_frames;
+    uint32_t _pad;
+};
+
+// One block per area: thread 0 draws the rnd slots (xorshift128, Marsaglia
+// 2003), all threads copy the frame.
+__global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io, uint32_t n_areas,
+                                                   const uint32_t* __restrict__ counter) {
+    for (uint32_t a = blockIdx.x; a < n_areas; a += gridDim.x) {
+        const AreaIo& A = io[a];
+        if (threadIdx.x == 0) {
+            uint32_t x = A.rng[0], y = A.rng[1], z = A.rng[2], w = A.rng[3];
+            for (uint32_t k = 0; k < A.n_slots; k++) {
+                uint32_t t = x ^ (x << 11);
+                x = y;
+                y = z;
+                z = w;
+                w = w ^ (w >> 19) ^ (t ^ (t >> 8));
+                A.rnd[k] = (int32_t)w;
+            }
+            A.rng[0] = x;
+            A.rng[1] = y;
+            A.rng[2] = z;
+            A.rng[3] = w;
+        }
+        if (A.len && A.n_frames) {
+            const uint8_t* src = A.frames + (size_t)(*counter % A.n_frames) * A.stride;
+            for (uint32_t i = threadIdx.x; i < A.len; i += blockDim.x) A.dst[i] = src[i];
+        }
+    }
+}
+
+__global__ void kb_tick(uint32_t* counter) { *counter += 1; }
+
+}  // namespace bsm
+
+struct bsm_batch {
+    bsm_ctx* ctx;
+    hipStream_t stream;
+    std::vector<bsm_area*> areas;
+    std::vector<bsm::TuftJob> tjobs;
+    std::vector<bsm::SscJob> sjobs;
+    std::vector<bsm::PyrJob> pjobs;
+    std::vector<bsm::Stage> stages;
+    std::vector<bsm::AreaIo> io;
+    bsm::TuftJob* tjobs_d = nullptr;
+    bsm::SscJob* sjobs_d = nullptr;
+    bsm::PyrJob* pjobs_d = nullptr;
+    bsm::WorkItem* items_d = nullptr;
+    bsm::AreaIo* io_d = nullptr;
+    uint32_t* rng_d = nullptr;      // 4 words per area
+    uint32_t* counter_d = nullptr;
+    uint32_t bpar = 0;
+    bsm::Tuning tune;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    bool dirty = true;
+    hipEvent_t ev0, ev1;
+};
+
+namespace bsm {
+
+static void free_graphs(bsm_batch* b) {
+    for (int i = 0; i < 2; i++) {
+        if (b->exec[i]) hipGraphExecDestroy(b->exec[i]);
+        if (b->graph[i]) hipGraphDestroy(b->graph[i]);
+        b->exec[i] = nullptr;
+        b->graph[i] = nullptr;
+    }
+}
+
+static int batch_plan(bsm_batch* b) {
+    free_graphs(b);
+    // jobs are planned so that the next cycle (par = 1) writes each layer's cur ^ 1
+    b->bpar = 0;
+    b->tjobs.clear();
+    b->sjobs.clear();
+    b->pjobs.clear();
+    b->stages.clear();
+    std::vector<WorkItem> items;
+    const uint32_t na = (uint32_t)b->areas.size();
+    // rank of each layer among the area's spatial / pyramidal layers
+    uint32_t max_rank[2] = {0, 0};
+    for (bsm_area* a : b->areas) {
+        uint32_t r[2] = {0, 0};
+        for (uint32_t l = 0; l < a->n_layers; l++) r[a->L[l].d.spatial ? 0 : 1]++;
+        max_rank[0] = std::max(max_rank[0], r[0]);
+        max_rank[1] = std::max(max_rank[1], r[1]);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t k = 0; k < max_rank[pass]; k++) {
+            std::vector<uint32_t> t_ids, t_w, s_ids, p_ids;
+            uint32_t w3 = 16;
+            for (uint32_t ai = 0; ai < na; ai++) {
+                bsm_area* a = b->areas[ai];
+                const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
+                const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
+                uint32_t rank = 0;
+                for (uint32_t l = 0; l < a->n_layers; l++) {
+                    LayerRt& L = a->L[l];
+                    const bool spatial = L.d.spatial != 0;
+                    if ((pass == 0) != spatial) continue;
+                    if (rank++ != k) continue;
+                    if ((spatial && !sscs) || (!spatial && !pyrs)) continue;
+                    // first batch cycle runs with par = 1 and must write buffer cur ^ 1
+                    const uint32_t px = (uint32_t)L.cur;
+                    for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+                        TuftJob j = make_tuft_job(a, l, t, b->tune, a->rnd_d);
+                        j.par_xor = px;
+                        t_ids.push_back((uint32_t)b->tjobs.size());
+                        t_w.push_back(tuft_chunk(a, l, t, b->tune.chunk));
+                        b->tjobs.push_back(j);
+                    }
+                    if (spatial) {
+                        SscJob j = make_ssc_job(a, l, a->rnd_d);
+                        j.par_xor = px;
+                        s_ids.push_back((uint32_t)b->sjobs.size());
+                        b->sjobs.push_back(j);
+                    } else {
+                        uint32_t w;
+                        PyrJob j = make_pyr_job(a, l, a->rnd_d, &w);
+                        j.par_xor = px;
+                        w3 = std::min(w3, w);
+                        p_ids.push_back((uint32_t)b->pjobs.size());
+                        b->pjobs.push_back(j);
+                    }
+                }
+            }
+            if (t_ids.empty()) continue;
+            Stage s1{};
+            if (plan_f1(b->tjobs, t_ids, t_w, b->tune, items, s1)) return 1;
+            b->stages.push_back(s1);
+            Stage s2{};
+            if (pass == 0) {
+                if (plan_f2(b->sjobs, s_ids, b->tune, items, s2)) return 1;
+            } else {
+                // a common chunk width for every job of the launch
+                for (uint32_t id : p_ids) {
+                    PyrJob& j = b->pjobs[id];
+                    uint32_t mc = 1;
+                    for (uint32_t t = 0; t < j.n_tufts; t++)
+                        if ((j.distal_mask >> t) & 1u) mc = std::max(mc, j.t_spd[t] / w3);
+                    j.max_chunks = mc;
+                }
+                if (plan_f3(b->pjobs, p_ids, items, s2)) return 1;
+                s2.W = w3;
+            }
+            b->stages.push_back(s2);
+        }
+    }
+    (void)hipFree(b->tjobs_d);
+    (void)hipFree(b->sjobs_d);
+    (void)hipFree(b->pjobs_d);
+    (void)hipFree(b->items_d);
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        *dst = nullptr;
+        BSM_HIP(hipMalloc(dst, bytes ? bytes : 16));
+        if (bytes) BSM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (up((void**)&b->tjobs_d, b->tjobs.data(), b->tjobs.size() * sizeof(TuftJob))) return 1;
+    if (up((void**)&b->sjobs_d, b->sjobs.data(), b->sjobs.size() * sizeof(SscJob))) return 1;
+    if (up((void**)&b->pjobs_d, b->pjobs.data(), b->pjobs.size() * sizeof(PyrJob))) return 1;
+    if (up((void**)&b->items_d, items.data(), items.size() * sizeof(WorkItem))) return 1;
+    b->dirty = false;
+    return 0;
+}
+
+static int enqueue_cycle(bsm_batch* b, uint32_t par) {
+    const uint32_t na = (uint32_t)b->areas.size();
+    kb_prologue<<<na, 256, 0, b->stream>>>(b->io_d, na, b->counter_d);
+    BSM_HIP(hipGetLastError());
+    for (const Stage& st : b->stages)
+        if (launch_stage(b->stream, st, b->tjobs_d, b->sjobs_d, b->pjobs_d, b->items_d, par, 0))
+            return 1;
+    kb_tick<<<1, 1, 0, b->stream>>>(b->counter_d);
+    BSM_HIP(hipGetLastError());
+    return 0;
+}
+
+static int capture(bsm_batch* b) {
+    free_graphs(b);
+    for (int par = 0; par < 2; par++) {
+        BSM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int e = enqueue_cycle(b, (uint32_t)par);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(b->stream, &g);
+        if (e) {
+            if (g) hipGraphDestroy(g);
+            return 1;
+        }
+        if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture");
+        b->graph[par] = g;
+        BSM_HIP(hipGraphInstantiate(&b->exec[par], g, nullptr, nullptr, 0));
+    }
+    return 0;
+}
+
+static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
+    if (b->dirty && batch_plan(b)) return 1;
+    if (use_graph && !b->exec[0] && capture(b)) return 1;
+    for (uint32_t c = 0; c < n; c++) {
+        b->bpar ^= 1u;
+        if (use_graph) {
+            BSM_HIP(hipGraphLaunch(b->exec[b->bpar], b->stream));
+        } else if (enqueue_cycle(b, b->bpar)) {
+            return 1;
+        }
+    }
+    // host bookkeeping: every batched layer's current buffer flipped n times
+    for (bsm_area* a : b->areas) {
+        const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
+        const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
+        for (uint32_t l = 0; l < a->n_layers; l++) {
+            LayerRt& L = a->L[l];
+            const bool spatial = L.d.spatial != 0;
+            if ((spatial && sscs) || (!spatial && pyrs)) L.cur ^= (int)(n & 1u);
+        }
+    }
+    return 0;
+}
+
+}  // namespace bsm
+
+using namespace bsm;
+
+extern "C" {
+
+int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm_batch** out) {
+    if (!ctx || !out || (!areas && n_areas)) return fail("null arg");
+    if (n_areas == 0 || n_areas > 65535) return fail("n_areas must be in [1, 65535]");
+    for (uint32_t i = 0; i < n_areas; i++) {
+        if (!areas[i] || areas[i]->ctx != ctx) return fail("area not on this context");
+        if (areas[i]->batch_stream) return fail("area already belongs to a batch");
+        for (uint32_t k = 0; k < i; k++)
+            if (areas[k] == areas[i]) return fail("duplicate area in batch");
+    }
+    BSM_HIP(hipSetDevice(ctx->device));
+    bsm_batch* b = new bsm_batch();
+    b->ctx = ctx;
+    b->areas.assign(areas, areas + n_areas);
+    BSM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BSM_HIP(hipEventCreate(&b->ev0));
+    BSM_HIP(hipEventCreate(&b->ev1));
+    BSM_HIP(hipMalloc((void**)&b->rng_d, sizeof(uint32_t) * 4 * n_areas));
+    BSM_HIP(hipMalloc((void**)&b->counter_d, sizeof(uint32_t)));
+    BSM_HIP(hipMemset(b->counter_d, 0, sizeof(uint32_t)));
+    std::vector<uint32_t> rng(4 * n_areas);
+    b->io.resize(n_areas);
+    for (uint32_t i = 0; i < n_areas; i++) {
+        bsm_area* a = areas[i];
+        BSM_HIP(hipStreamSynchronize(a->stream));
+        if (ensure_rnd(a, std::max<uint32_t>(a->n_slots, 1))) return 1;
+        std::memcpy(&rng[4 * i], a->rng, sizeof(a->rng));
+        AreaIo& io = b->io[i];
+        io = AreaIo{};
+        io.rng = b->rng_d + 4 * i;
+        io.rnd = a->rnd_d;
+        io.n_slots = a->n_slots;
+        a->batch_stream = b->stream;
+    }
+    BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    BSM_HIP(hipMalloc((void**)&b->io_d, sizeof(AreaIo) * n_areas));
+    BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n_areas, hipMemcpyHostToDevice));
+    b->dirty = true;
+    *out = b;
+    return 0;
+}
+
+int bsm_batch_sync(bsm_batch* b) {
+    if (!b) return fail("null batch");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    // the areas' host rnd streams continue where the device streams stopped
+    std::vector<uint32_t> rng(4 * b->areas.size());
+    BSM_HIP(hipMemcpy(rng.data(), b->rng_d, rng.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < b->areas.size(); i++)
+        std::memcpy(b->areas[i]->rng, &rng[4 * i], sizeof(b->areas[i]->rng));
+    return 0;
+}
+
+int bsm_batch_destroy(bsm_batch* b) {
+    if (!b) return 0;
+    bsm_batch_sync(b);
+    (void)hipSetDevice(b->ctx->device);
+    free_graphs(b);
+    (void)hipFree(b->tjobs_d);
+    (void)hipFree(b->sjobs_d);
+    (void)hipFree(b->pjobs_d);
+    (void)hipFree(b->items_d);
+    (void)hipFree(b->io_d);
+    (void)hipFree(b->rng_d);
+    (void)hipFree(b->counter_d);
+    (void)hipEventDestroy(b->ev0);
+    (void)hipEventDestroy(b->ev1);
+    (void)hipStreamDestroy(b->stream);
+    for (bsm_area* a : b->areas) {
+        a->plan_dirty = true;
+        a->batch_stream = nullptr;
+    }
+    delete b;
+    return 0;
+}
+
+int bsm_batch_set_inputs(bsm_batch* b, const uint8_t* const* dev_frames, const uint64_t* stride,
+                         const uint32_t* len, const uint32_t* axn_offset, uint32_t n_frames) {
+    if (!b) return fail("null batch");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->areas.size(); i++) {
+        AreaIo& io = b->io[i];
+        bsm_area* a = b->areas[i];
+        if (!dev_frames || !dev_frames[i] || !len || !len[i]) {
+            io.len = 0;
+            io.frames = nullptr;
+            continue;
+        }
+        uint32_t ofs = axn_offset ? axn_offset[i] : 0;
+        if (ofs > a->axn_len || len[i] > a->axn_len - ofs)
+            return fail("frame range out of the axon space of area " + std::to_string(i));
+        io.len = len[i];
+        io.frames = dev_frames[i];
+        io.stride = stride ? stride[i] : len[i];
+        io.dst = a->axn + ofs;
+        io.n_frames = n_frames;
+    }
+    BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * b->io.size(), hipMemcpyHostToDevice));
+    BSM_HIP(hipMemset(b->counter_d, 0, sizeof(uint32_t)));
+    return 0;
+}
+
+int bsm_batch_cycle(bsm_batch* b, uint32_t n_cycles, int use_graph) {
+    if (!b) return fail("null batch");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    return run_cycles(b, n_cycles, use_graph);
+}
+
+int bsm_batch_time_cycles(bsm_batch* b, uint32_t n_cycles, int use_graph, float* ms_total) {
+    if (!b || !ms_total) return fail("null arg");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    if (b->dirty && batch_plan(b)) return 1;
+    if (use_graph && !b->exec[0] && capture(b)) return 1;
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    BSM_HIP(hipEventRecord(b->ev0, b->stream));
+    if (run_cycles(b, n_cycles, use_graph)) return 1;
+    BSM_HIP(hipEventRecord(b->ev1, b->stream));
+    BSM_HIP(hipEventSynchronize(b->ev1));
+    BSM_HIP(hipEventElapsedTime(ms_total, b->ev0, b->ev1));
+    return 0;
+}
+
+int bsm_batch_set_tuning(bsm_batch* b, const bsm_tuning* t) {
+    if (!b || !t) return fail("null arg");
+    Tuning nt = b->tune;
+    if (t->tile_v) nt.tile_v = t->tile_v;
+    if (t->tile_u) nt.tile_u = t->tile_u;
+    if (t->halo) nt.halo = t->halo;
+    if (t->chunk) nt.chunk = t->chunk;
+    if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
+        return fail("tile too large");
+    if (nt.halo > 64) return fail("halo too large");
+    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
+        return fail("chunk must be 1, 4, 8 or 16");
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    b->tune = nt;
+    b->dirty = true;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,943 @@
+// bsm_fused.hip -- fused gfx950 kernels of the per-cycle pipeline and the
+// plain per-area orchestrator (CorticalArea::cycle, src/cortex/cortical_area.rs:716-789).
+//
+// Stages per area per cycle:
+//   F1 kf_tuft      : tft_set_syn_flags + layer_cycle_syns_wow_vec4 + den_cycle_tft
+//                     (+ tft_cycle for pyramidal tufts, + ssc_cycle for the SSC soma)
+//                     cl/syns.cl:23-33,285-388, cl/bismit.cl:695-884
+//   F2 kf_ssc_post  : inhib_simple | inhib_passthrough + ssc_mtp_simple
+//                     cl/control.cl:28-217, cl/bismit.cl:888-915
+//   F3 kf_pyr_post  : pyr_cycle + inhib_intra_column | pyr_output + tft_dst_mtp
+//                     cl/bismit.cl:961-1022,1195-1268, cl/control.cl:298-351
+//
+// Every kernel is a persistent grid-stride loop over a flattened work-item list
+// (tiles of columns), so one launch can step any number of areas (batched form)
+// or one area (plain form).
+//
+// Synapse states are double-buffered: F1 writes st[par] and never reads the
+// old state; SYN_PREV_ACTIVE of the flags is "st[par^1] != 0", read by the
+// distal LTP of F3 and materialised only at readback.  F1 therefore streams
+// 3 B of metadata + 1 B strength in and 1 B state out per synapse.
+#include <algorithm>
+#include <cstring>
+
+#include "bsm_internal.h"
+
+namespace bsm {
+
+// ------------------------------------------------------------------ F1
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int W>
+struct Chunk;
+template <>
+struct Chunk<16> {
+    static constexpr int NW = 4;
+    __device__ static inline void load(const void* p, uint32_t* o) {
+        u32x4 x = __builtin_nontemporal_load((const u32x4*)p);
+        o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w;
+    }
+    __device__ static inline void store(void* p, const uint32_t* o) {
+        *(uint4*)p = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+};
+template <>
+struct Chunk<8> {
+    static constexpr int NW = 2;
+    __device__ static inline void load(const void* p, uint32_t* o) {
+        u32x2 x = __builtin_nontemporal_load((const u32x2*)p);
+        o[0] = x.x; o[1] = x.y;
+    }
+    __device__ static inline void store(void* p, const uint32_t* o) {
+        *(uint2*)p = make_uint2(o[0], o[1]);
+    }
+};
+template <>
+struct Chunk<4> {
+    static constexpr int NW = 1;
+    __device__ static inline void load(const void* p, uint32_t* o) {
+        o[0] = __builtin_nontemporal_load((const uint32_t*)p);
+    }
+    __device__ static inline void store(void* p, const uint32_t* o) { *(uint32_t*)p = o[0]; }
+};
+template <>
+struct Chunk<1> {
+    static constexpr int NW = 1;
+    __device__ static inline void load(const void* p, uint32_t* o) { o[0] = *(const uint8_t*)p; }
+    __device__ static inline void store(void* p, const uint32_t* o) { *(uint8_t*)p = (uint8_t)o[0]; }
+};
+
+template <int W, int NST>
+__global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
+                                               const WorkItem* __restrict__ items, uint32_t n_items,
+                                               uint32_t par, uint32_t rnd_ofs, uint32_t TV,
+                                               uint32_t TU, uint32_t stage_cap) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int NB = W < 4 ? W : 4;               // bytes per word
+    constexpr int NW = Chunk<W>::NW;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    uint32_t* part = (uint32_t*)(lds + stage_cap);
+
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem wi = items[it];
+        const TuftJob& J = jobs[wi.job];
+        const uint32_t V = J.V, U = J.U, depth = J.depth, spt = J.spt, halo = J.halo;
+        const uint32_t v0 = (uint32_t)wi.tv * TV, u0 = (uint32_t)wi.tu * TU;
+        const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
+        const uint32_t p = par ^ J.par_xor;
+        const bool quirk = J.scalar_quirk != 0;
+
+        // ---- stage the tile's hex neighbourhood of each staged source slice
+        int32_t vlo[NST], ulo[NST], rows[NST], cols[NST], lbase[NST];
+        uint32_t sid[NST], vsc[NST], usc[NST], vmd[NST], umd[NST];
+        uint32_t off = 0;
+#pragma unroll
+        for (int j = 0; j < NST; j++) {
+            rows[j] = 0; cols[j] = 0; lbase[j] = 0; vlo[j] = 0; ulo[j] = 0;
+            sid[j] = 0xFFFFFFFFu; vsc[j] = usc[j] = vmd[j] = umd[j] = 0;
+            if ((uint32_t)j < J.n_staged) {
+                sid[j] = J.staged[j];
+                SliceRt s = J.slices[sid[j]];
+                vsc[j] = s.v_scale; usc[j] = s.u_scale; vmd[j] = s.v_mid; umd[j] = s.u_mid;
+                int32_t a0 = (int32_t)((v0 * vsc[j]) >> 4) + (int32_t)vmd[j] - (int32_t)halo;
+                int32_t a1 = (int32_t)(((v0 + tvn - 1) * vsc[j]) >> 4) + (int32_t)vmd[j] + (int32_t)halo;
+                int32_t b0 = (int32_t)((u0 * usc[j]) >> 4) + (int32_t)umd[j] - (int32_t)halo;
+                int32_t b1 = (int32_t)(((u0 + tun - 1) * usc[j]) >> 4) + (int32_t)umd[j] + (int32_t)halo;
+                a0 = max(a0, 0); b0 = max(b0, 0);
+                a1 = min(a1, (int32_t)s.v_size - 1); b1 = min(b1, (int32_t)s.u_size - 1);
+                int32_t nr = a1 >= a0 ? a1 - a0 + 1 : 0, nc = b1 >= b0 ? b1 - b0 + 1 : 0;
+                if (off + (uint32_t)(nr * nc) > stage_cap) { nr = 0; nc = 0; }
+                vlo[j] = a0; ulo[j] = b0; rows[j] = nr; cols[j] = nc; lbase[j] = (int32_t)off;
+                const uint8_t* src = J.axn + s.idz;
+                const uint32_t n = (uint32_t)(nr * nc);
+                for (uint32_t x = tid; x < n; x += nth) {
+                    uint32_t rr = x / (uint32_t)nc, cc = x - rr * (uint32_t)nc;
+                    lds[off + x] = src[(uint32_t)(a0 + (int32_t)rr) * s.u_size + (uint32_t)b0 + cc];
+                }
+                off += n;
+            }
+        }
+        __syncthreads();
+
+        // ---- synapse chunks: gather + fire + store + per-chunk dendrite partials
+        const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
+        const uint32_t CPT = spt / W;
+        const uint32_t row_chunks = tun * CPT;
+        const uint32_t n_chunks = depth * tvn * row_chunks;
+        const uint32_t VU = V * U;
+        uint8_t* __restrict__ st_out = J.st[p & 1];
+        for (uint32_t q = tid; q < n_chunks; q += nth) {
+            const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
+            const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+            const uint32_t cc = rem / CPT, k = rem - cc * CPT;
+            const uint32_t v = v0 + r, u = u0 + cc;
+            const uint32_t cel = sl * VU + v * U + u;
+            const size_t so = (size_t)cel * spt + (size_t)k * W;
+            int32_t av[NST], au[NST];
+#pragma unroll
+            for (int j = 0; j < NST; j++) {
+                av[j] = (int32_t)((v * vsc[j]) >> 4) + (int32_t)vmd[j] - vlo[j];
+                au[j] = (int32_t)((u * usc[j]) >> 4) + (int32_t)umd[j] - ulo[j];
+            }
+            uint32_t ms[NW], mv[NW], mu[NW], mt[NW], out[NW];
+            Chunk<W>::load(J.slc + so, ms);
+            Chunk<W>::load(J.vofs + so, mv);
+            Chunk<W>::load(J.uofs + so, mu);
+            Chunk<W>::load(J.str + so, mt);
+            uint32_t psum = 0, praw = 0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                uint32_t ow = 0;
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const uint32_t sl_id = byte_of(ms[w], b);
+                    const int32_t vo = sbyte_of(mv[w], b), uo = sbyte_of(mu[w], b);
+                    int32_t lv = -1, lu = -1, nr = 0, nc = 0, lb = 0;
+#pragma unroll
+                    for (int j = 0; j < NST; j++) {
+                        if (sl_id == sid[j]) {
+                            lv = av[j] + vo; lu = au[j] + uo; nr = rows[j]; nc = cols[j]; lb = lbase[j];
+                        }
+                    }
+                    uint32_t ax;
+                    if ((uint32_t)lv < (uint32_t)nr && (uint32_t)lu < (uint32_t)nc) {
+                        ax = lds[lb + lv * nc + lu];
+                    } else {
+                        ax = axn_gather(J.axn, J.slices, J.n_slices, sl_id, (int32_t)v, vo,
+                                        (int32_t)u, uo, quirk);
+                    }
+                    const uint32_t stv = syn_fire(ax);
+                    ow |= stv << (8 * b);
+                    praw += stv;
+                    psum += (sbyte_of(mt[w], b) >= 0) ? stv : 0u;
+                }
+                out[w] = ow;
+            }
+            Chunk<W>::store(st_out + so, out);
+            part[q] = psum | (praw << 16);
+        }
+        __syncthreads();
+
+        // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
+        const uint32_t dens = J.dens, CPD = J.spd / W;
+        const uint32_t cells_t = depth * tvn * tun;
+        const uint32_t nd_t = cells_t * dens;
+        const bool do_tft = J.tft[0] != nullptr;
+        uint8_t* denc = lds + stage_cap + n_chunks * 4;
+        for (uint32_t x = tid; x < nd_t; x += nth) {
+            const uint32_t ct = x / dens, dj = x - ct * dens;
+            const uint32_t q0 = ct * CPT + dj * CPD;
+            uint32_t sum = 0, raw = 0;
+            for (uint32_t m = 0; m < CPD; m++) {
+                uint32_t pv = part[q0 + m];
+                sum += pv & 0xFFFFu;
+                raw += pv >> 16;
+            }
+            const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
+            const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+            const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+            const uint32_t dl = cel * dens + dj;
+            const uint32_t dg = J.den_idz + dl;
+            const int32_t active = sum > J.thresh;
+            sum = active ? sum : 0u;
+            const uint32_t raw_o = min(raw >> 4, 255u);
+            uint32_t st_o = min(sum >> 4, 255u);
+            const int32_t seed = (int32_t)((raw + J.thresh) | dg);
+            J.den_act[dl] = (uint8_t)activity(J.den_act[dl], active, rnd, seed, DEN_ACT_DECAY);
+            J.den_raw[dl] = (uint8_t)raw_o;
+            if (J.ssc_energy != nullptr && dg < J.C) {
+                // ssc_cycle (cl/bismit.cl:849-884) on soma index dg == dendrite dg
+                const uint32_t e = J.ssc_energy[dg];
+                const uint32_t restless = (e > 191) & (st_o != 0);
+                const uint32_t dark = e == 255;
+                if (dark | restless) J.ssc_energy[dg] = (uint8_t)(e - 64);
+                st_o = min(st_o + 255u * restless + 255u * dark, 255u);
+            }
+            J.den_st[dl] = (uint8_t)st_o;
+            if (do_tft) {
+                denc[x] = (uint8_t)raw_o;
+                denc[nd_t + x] = (uint8_t)st_o;
+            }
+        }
+
+        // ---- tufts (tft_cycle)
+        if (do_tft) {
+            __syncthreads();
+            const uint32_t l2 = J.l2;
+            for (uint32_t ct = tid; ct < cells_t; ct += nth) {
+                uint32_t best_id = 0, best_raw = 0, best = 0, asum = 0;
+                for (uint32_t j = 0; j < dens; j++) {
+                    const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
+                    if (sr > best_raw) { best_id = j; best_raw = sr; }
+                    best = max(best, s);
+                    asum += s;
+                }
+                const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
+                const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
+                const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+                const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+                J.tft[4][cel] = J.tft[0][cel];
+                J.tft[5][cel] = J.tft[1][cel];
+                J.tft[6][cel] = J.tft[2][cel];
+                J.tft[7][cel] = J.tft[3][cel];
+                J.tft[0][cel] = (uint8_t)best_id;
+                J.tft[1][cel] = (uint8_t)best_raw;
+                J.tft[2][cel] = (uint8_t)best;
+                J.tft[3][cel] = (uint8_t)tstate;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ F2
+template <int W>
+__device__ inline void ssc_ltp_chunk(const uint8_t* __restrict__ st, int8_t* __restrict__ str,
+                                     uint32_t z, uint32_t i0, int32_t rnd) {
+    constexpr int NB = W < 4 ? W : 4;
+    constexpr int NW = Chunk<W>::NW;
+    uint32_t ms[NW], mt[NW], out[NW];
+    Chunk<W>::load(st + i0, ms);
+    Chunk<W>::load(str + i0, mt);
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        uint32_t ow = 0;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const uint32_t i = i0 + (uint32_t)(w * 4 + b);
+            const int32_t seed = (int32_t)(z + i);
+            int32_t s = sbyte_of(mt[w], b);
+            if (byte_of(ms[w], b)) s += rnd_inc(rnd, seed, s, 0x7F);
+            else s -= rnd_dec(rnd, seed, s, 0x1FF);
+            ow |= ((uint32_t)s & 0xFFu) << (8 * b);
+        }
+        out[w] = ow;
+    }
+    Chunk<W>::store(str + i0, out);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jobs,
+                                                   const WorkItem* __restrict__ items,
+                                                   uint32_t n_items, uint32_t par, uint32_t rnd_ofs,
+                                                   uint32_t TV, uint32_t TU) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t n_act;
+    const uint32_t tid = threadIdx.x, nth = blockDim.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem wi = items[it];
+        const SscJob& J = jobs[wi.job];
+        const uint32_t V = J.V, U = J.U, VU = V * U, slc = wi.slc;
+        const uint32_t v0 = (uint32_t)wi.tv * TV, u0 = (uint32_t)wi.tu * TU;
+        const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
+        const int32_t r = J.mode == 0 ? (int32_t)J.radius : 0;
+        const uint32_t WV = tvn + 2 * r, WU = tun + 2 * r;
+        const uint32_t p = par ^ J.par_xor;
+        uint32_t* list = (uint32_t*)(lds + ((WV * WU + 15) & ~15u));
+        const uint8_t* soma = J.soma + (size_t)slc * VU;
+        for (uint32_t x = tid; x < WV * WU; x += nth) {
+            const uint32_t wr = x / WU, wc = x - wr * WU;
+            const int32_t nv = (int32_t)v0 - r + (int32_t)wr, nu = (int32_t)u0 - r + (int32_t)wc;
+            uint8_t val = 0;
+            if (nv >= 0 && nv < (int32_t)V && nu >= 0 && nu < (int32_t)U) val = soma[nv * U + nu];
+            lds[x] = val;
+        }
+        if (tid == 0) n_act = 0;
+        __syncthreads();
+
+        const int32_t rnd_c = J.mode == 0 ? J.rnd[rnd_ofs + J.slot_ctrl] : 0;
+        const int32_t imax = (r + INHIB_HORIZ_OFS) * (r + INHIB_HORIZ_OFS);
+        const SliceRt own = J.slices[J.base_slc + slc];
+        for (uint32_t x = tid; x < tvn * tun; x += nth) {
+            const uint32_t tr = x / tun, tc = x - tr * tun;
+            const uint32_t v = v0 + tr, u = u0 + tc;
+            const uint32_t c = slc * VU + v * U + u;
+            uint32_t out;
+            if (J.mode == 2) {
+                out = J.axn[own.idz + v * U + u];
+            } else {
+                int safe;
+                const uint32_t axi = cel_axn_idx(own, v, u, &safe);
+                uint32_t cs = lds[(tr + r) * WU + tc + r];
+                uint32_t is_act;
+                if (J.mode == 0) {
+                    cs = safe ? cs : 0u;
+                    int unin = 1;
+                    if (cs) {
+                        const int32_t ci = (int32_t)cs * imax;
+                        for (int32_t vo = -r; vo <= r && unin; vo++) {
+                            const int32_t uz = max(-r, -vo - r), um = min(r, -vo + r);
+                            const uint8_t* row = lds + (uint32_t)((int32_t)tr + r + vo) * WU + tc + r;
+                            for (int32_t uo = uz; uo <= um; uo++) {
+                                const int32_t w = -vo - uo;
+                                const int32_t dist = (abs(vo) + abs(uo) + abs(w)) >> 1;
+                                const int32_t dc = dist - INHIB_CENTER_OFS;
+                                unin &= ci >= (int32_t)row[uo] * (imax - dc * dc);
+                            }
+                        }
+                    }
+                    out = unin ? cs : 0u;
+                    is_act = unin & (cs != 0);
+                } else {
+                    out = cs;
+                    is_act = cs != 0;
+                }
+                J.axn[axi] = (uint8_t)out;
+                J.cel_act[c] = (uint8_t)activity(J.cel_act[c], (int32_t)is_act, rnd_c,
+                                                 (int32_t)(cs | c), CEL_ACT_DECAY);
+            }
+            if (J.learn && out) list[atomicAdd(&n_act, 1u)] = c;
+        }
+        __syncthreads();
+
+        if (J.learn) {
+            const uint32_t na = n_act, spt = J.spt, CPC = spt / W;
+            const int32_t rnd_m = J.rnd[rnd_ofs + J.slot_mtp];
+            const uint8_t* __restrict__ st = J.st[p & 1];
+            for (uint32_t y = tid; y < na * CPC; y += nth) {
+                const uint32_t ci = y / CPC, k = y - ci * CPC;
+                const uint32_t z = list[ci] * spt;
+                ssc_ltp_chunk<W>(st, J.str, z, z + k * W, rnd_m);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ F3
+template <int W>
+__device__ inline void dst_ltp_chunk(const uint8_t* __restrict__ prev, int8_t* __restrict__ str,
+                                     uint32_t z, uint32_t i0, int32_t rnd) {
+    constexpr int NB = W < 4 ? W : 4;
+    constexpr int NW = Chunk<W>::NW;
+    uint32_t ms[NW], mt[NW], out[NW];
+    Chunk<W>::load(prev + i0, ms);
+    Chunk<W>::load(str + i0, mt);
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        uint32_t ow = 0;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const uint32_t i = i0 + (uint32_t)(w * 4 + b);
+            const int32_t seed = (int32_t)(z + i);
+            int32_t s = sbyte_of(mt[w], b);
+            if (byte_of(ms[w], b)) s += rnd_inc(rnd, seed, s, 0x7F);
+            else s -= rnd_dec(rnd, seed, s, 0xFFF);
+            ow |= ((uint32_t)s & 0xFFu) << (8 * b);
+        }
+        out[w] = ow;
+    }
+    Chunk<W>::store(str + i0, out);
+}
+
+constexpr uint32_t F3_COLS = 256;
+
+template <int W>
+__global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jobs,
+                                                   const WorkItem* __restrict__ items,
+                                                   uint32_t n_items, uint32_t par, uint32_t rnd_ofs) {
+    __shared__ uint32_t list[F3_COLS * BSM_MAX_TUFTS];
+    __shared__ uint32_t n_l;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem wi = items[it];
+        const PyrJob& J = jobs[wi.job];
+        const uint32_t VU = J.V * J.U, C = J.C, depth = J.depth;
+        const uint32_t col = (uint32_t)wi.tv * F3_COLS + tid;
+        const bool valid = col < VU;
+        const uint32_t p = par ^ J.par_xor;
+        // pass 1: pyr_cycle for every slice of the column, running intra-column best
+        uint32_t best = 0;
+        if (valid) {
+            for (uint32_t s = 0; s < depth; s++) {
+                const uint32_t c = s * VU + col;
+                const uint32_t pp = (J.en & FLAG_BASAL_PROXIMAL) ? J.tft_states[J.prx * C + c] : 0u;
+                const uint32_t dd = (J.en & FLAG_BASAL_DISTAL) ? J.tft_best_raw[J.bdst * C + c] : 0u;
+                const uint32_t act = pp != 0;
+                const uint32_t cs = (pp >> 2) + ((pp <= 3) && act) + act * (dd >> 1);
+                J.soma[c] = (uint8_t)cs;
+                best = (cs >= best) ? cs : 0u;
+            }
+        }
+        // pass 2, per slice: axon, learning decisions, cooperative distal LTP
+        for (uint32_t s = 0; s < depth; s++) {
+            if (tid == 0) n_l = 0;
+            __syncthreads();
+            if (valid) {
+                const uint32_t c = s * VU + col;
+                const uint32_t cs = J.soma[c];
+                uint32_t act;
+                const uint32_t v = col / J.U, u = col - (col / J.U) * J.U;
+                if (J.control == BSM_CTRL_INTRA_COLUMN || J.control == BSM_CTRL_PYR_OUTPUT) {
+                    int safe;
+                    const uint32_t axi = cel_axn_idx(J.slices[J.base_slc + s], v, u, &safe);
+                    const uint32_t out =
+                        (J.control == BSM_CTRL_PYR_OUTPUT) ? cs : ((cs >= best) ? cs : 0u);
+                    J.axn[axi] = (uint8_t)out;
+                    act = out != 0;
+                } else {
+                    act = J.axn[J.slices[J.base_slc].idz + c] != 0;
+                }
+                if (J.learn && J.distal_mask) {
+                    uint32_t f = J.cel_flags[c];
+                    for (uint32_t t = 0; t < J.n_tufts; t++) {
+                        if (!((J.distal_mask >> t) & 1u)) continue;
+                        const uint32_t prev = (f & CEL_PREV_ACTIVE) != 0;
+                        const uint32_t tpa = J.prev_best_raw[t * C + c] != 0;
+                        if (!prev && act && tpa) list[atomicAdd(&n_l, 1u)] = (c << 2) | t;
+                        f = (f & ~(uint32_t)CEL_PREV_ACTIVE) | (act ? CEL_PREV_ACTIVE : 0u);
+                    }
+                    J.cel_flags[c] = (uint8_t)f;
+                }
+            }
+            __syncthreads();
+            if (J.learn && J.distal_mask) {
+                const uint32_t nl = n_l;
+                const uint8_t* __restrict__ prev_st = J.st[(p & 1) ^ 1];
+                const uint32_t maxch = J.max_chunks;
+                for (uint32_t y = tid; y < nl * maxch; y += blockDim.x) {
+                    const uint32_t li = y / maxch, k = y - li * maxch;
+                    const uint32_t e = list[li];
+                    const uint32_t c = e >> 2, t = e & 3u;
+                    const uint32_t spd = J.t_spd[t];
+                    if (k * W >= spd) continue;
+                    const uint32_t z = c * J.t_spt[t] + J.t_syn_idz[t] +
+                                       (uint32_t)J.prev_best_ids[t * C + c] * spd;
+                    const int32_t rnd = J.rnd[rnd_ofs + J.slot_mtp0 + t];
+                    dst_ltp_chunk<W>(prev_st, J.str, z, z + k * W, rnd);
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace bsm
+
+// ====================================================================== host
+namespace bsm {
+
+static constexpr uint32_t LDS_LIMIT = 64 * 1024;
+
+static uint32_t pick_w(uint64_t a, uint64_t b, uint64_t c, uint32_t max_w) {
+    const uint32_t ws[4] = {16, 8, 4, 1};
+    for (uint32_t w : ws)
+        if (w <= max_w && a % w == 0 && b % w == 0 && c % w == 0) return w;
+    return 1;
+}
+
+uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w) {
+    const TuftRt& T = a->L[l].t[t];
+    return pick_w(T.d.syns_per_den, T.syn_idz, T.spt, max_w);
+}
+
+uint32_t ssc_chunk(const bsm_area* a, uint32_t l) { return pick_w(a->L[l].t[0].spt, 0, 0, 16); }
+
+TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, const int32_t* rnd) {
+    LayerRt& L = a->L[l];
+    TuftRt& T = L.t[t];
+    TuftJob j{};
+    j.axn = a->axn;
+    j.slices = a->slices_d;
+    j.slc = L.syn_slc + T.syn_idz;
+    j.vofs = L.syn_v + T.syn_idz;
+    j.uofs = L.syn_u + T.syn_idz;
+    j.str = L.syn_str + T.syn_idz;
+    j.st[0] = L.syn_state[0] + T.syn_idz;
+    j.st[1] = L.syn_state[1] + T.syn_idz;
+    j.den_raw = L.den_raw + T.den_idz;
+    j.den_st = L.den_states + T.den_idz;
+    j.den_act = L.den_act + T.den_idz;
+    j.ssc_energy = (L.d.kind == BSM_LAYER_SSC) ? L.cel_energies : nullptr;
+    if (L.d.kind == BSM_LAYER_PYR)
+        for (int k = 0; k < 8; k++) j.tft[k] = L.tft[k] + (size_t)t * L.cells;
+    j.rnd = rnd;
+    j.rnd_slot = L.slot0 + t;
+    j.n_slices = a->n_slices;
+    j.V = a->V;
+    j.U = a->U;
+    j.depth = L.d.depth;
+    j.C = L.cells;
+    j.dens = T.d.dens_per_tft;
+    j.spd = T.d.syns_per_den;
+    j.spt = T.spt;
+    j.thresh = T.d.threshold;
+    j.den_idz = (uint32_t)T.den_idz;
+    j.l2 = T.d.max_active_dens_l2;
+    j.scalar_quirk = (T.spt % 4) != 0;
+    j.halo = tn.halo;
+    j.par_xor = 0;
+    // Stage up to 4 distinct source slices; with the scalar quirk only slices
+    // without mids (where char and int offsets agree).
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < T.d.n_src_slcs && n < 4; k++) {
+        uint32_t sid = T.d.src_slc_ids[k];
+        bool dup = false;
+        for (uint32_t m = 0; m < n; m++) dup |= j.staged[m] == sid;
+        if (dup) continue;
+        const SliceRt& s = a->slices_h[sid];
+        if (j.scalar_quirk && (s.v_mid || s.u_mid)) continue;
+        j.staged[n] = sid;
+        j.st_vsize[n] = s.v_size;
+        j.st_usize[n] = s.u_size;
+        j.st_vsc[n] = s.v_scale;
+        j.st_usc[n] = s.u_scale;
+        n++;
+    }
+    j.n_staged = n;
+    return j;
+}
+
+SscJob make_ssc_job(bsm_area* a, uint32_t l, const int32_t* rnd) {
+    LayerRt& L = a->L[l];
+    SscJob j{};
+    j.soma = L.den_states;
+    j.axn = a->axn;
+    j.slices = a->slices_d;
+    j.cel_act = L.cel_act;
+    j.st[0] = L.syn_state[0];
+    j.st[1] = L.syn_state[1];
+    j.str = L.syn_str;
+    j.rnd = rnd;
+    j.slot_ctrl = L.slot0 + L.d.n_tufts;
+    j.slot_mtp = L.slot0 + L.d.n_tufts + 1;
+    j.V = a->V;
+    j.U = a->U;
+    j.depth = L.d.depth;
+    j.C = L.cells;
+    j.spt = L.t[0].spt;
+    j.base_slc = L.d.base_axn_slc;
+    j.radius = (uint32_t)L.d.inhib_radius;
+    j.mode = L.d.control == BSM_CTRL_INHIB_SIMPLE ? ((a->settings & BSM_SET_BYPASS_INHIB) ? 1u : 0u)
+                                                   : 2u;
+    j.learn = !(a->settings & BSM_SET_DISABLE_LEARNING);
+    j.par_xor = 0;
+    return j;
+}
+
+PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3) {
+    LayerRt& L = a->L[l];
+    PyrJob j{};
+    j.tft_states = L.tft[3];
+    j.tft_best_raw = L.tft[1];
+    j.prev_best_raw = L.tft[5];
+    j.prev_best_ids = L.tft[4];
+    j.soma = L.soma;
+    j.cel_flags = L.cel_flags;
+    j.axn = a->axn;
+    j.slices = a->slices_d;
+    j.st[0] = L.syn_state[0];
+    j.st[1] = L.syn_state[1];
+    j.str = L.syn_str;
+    j.rnd = rnd;
+    j.slot_mtp0 = L.slot0 + L.d.n_tufts + 1;
+    j.V = a->V;
+    j.U = a->U;
+    j.depth = L.d.depth;
+    j.C = L.cells;
+    j.control = L.d.control;
+    j.n_tufts = L.d.n_tufts;
+    j.base_slc = L.d.base_axn_slc;
+    j.learn = !(a->settings & BSM_SET_DISABLE_LEARNING);
+    uint32_t w = 16;
+    for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+        const TuftRt& T = L.t[t];
+        const bsm_tuft_desc& d = T.d;
+        if (d.den_class == BSM_DEN_BASAL && d.den_kind == BSM_DEN_PROXIMAL) {
+            j.en |= FLAG_BASAL_PROXIMAL;
+            j.prx = t;
+        } else if (d.den_class == BSM_DEN_BASAL && d.den_kind == BSM_DEN_DISTAL) {
+            j.en |= FLAG_BASAL_DISTAL;
+            j.bdst = t;
+        } else {
+            j.en |= FLAG_APICAL_DISTAL;
+            j.adst = t;
+        }
+        if (d.den_kind == BSM_DEN_DISTAL) {
+            j.distal_mask |= 1u << t;
+            w = std::min(w, pick_w(d.syns_per_den, T.spt, T.syn_idz, 16));
+        }
+        j.t_spd[t] = d.syns_per_den;
+        j.t_spt[t] = T.spt;
+        j.t_syn_idz[t] = (uint32_t)T.syn_idz;
+    }
+    uint32_t mc = 1;
+    for (uint32_t t = 0; t < L.d.n_tufts; t++)
+        if ((j.distal_mask >> t) & 1u) mc = std::max(mc, j.t_spd[t] / w);
+    j.max_chunks = mc;
+    j.par_xor = 0;
+    *w3 = w;
+    return j;
+}
+
+static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU) {
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < j.n_staged; k++) {
+        uint32_t r = std::min<uint32_t>(j.st_vsize[k], (((TV - 1) * j.st_vsc[k]) >> 4) + 2 + 2 * j.halo);
+        uint32_t c = std::min<uint32_t>(j.st_usize[k], (((TU - 1) * j.st_usc[k]) >> 4) + 2 + 2 * j.halo);
+        tot += r * c;
+    }
+    return tot;
+}
+
+int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
+            const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
+            Stage& st) {
+    uint32_t W = 16, nst = 0;
+    for (uint32_t k = 0; k < job_ids.size(); k++) {
+        W = std::min(W, job_w[k]);
+        nst = std::max(nst, jobs[job_ids[k]].n_staged);
+    }
+    uint32_t TV = tn.tile_v, TU = tn.tile_u;
+    uint32_t cap = 0, lds = 0;
+    for (;;) {
+        cap = 0;
+        uint32_t chunks = 0, denc = 0;
+        for (uint32_t id : job_ids) {
+            const TuftJob& j = jobs[id];
+            cap = std::max(cap, stage_bytes(j, TV, TU));
+            chunks = std::max(chunks, j.depth * TV * TU * (j.spt / W));
+            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * TV * TU * j.dens);
+        }
+        cap = (cap + 15) & ~15u;
+        lds = cap + chunks * 4 + denc;
+        if (lds <= LDS_LIMIT) break;
+        if (TV == 1 && TU == 1) return fail("fused tuft kernel does not fit in LDS");
+        if (TV >= TU && TV > 1) TV = (TV + 1) / 2;
+        else TU = (TU + 1) / 2;
+    }
+    st.kind = 1;
+    st.item0 = (uint32_t)items.size();
+    for (uint32_t id : job_ids) {
+        const TuftJob& j = jobs[id];
+        uint32_t ntv = (j.V + TV - 1) / TV, ntu = (j.U + TU - 1) / TU;
+        if (ntv > 65535 || ntu > 65535 || id > 65535) return fail("too many tiles");
+        for (uint32_t a = 0; a < ntv; a++)
+            for (uint32_t b = 0; b < ntu; b++)
+                items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)a, (uint16_t)b});
+    }
+    st.n_items = (uint32_t)items.size() - st.item0;
+    st.lds_bytes = lds;
+    st.W = W;
+    st.nst = nst <= 1 ? 1 : (nst <= 2 ? 2 : 4);
+    st.stage_cap = cap;
+    st.tv = TV;
+    st.tu = TU;
+    st.grid = std::min<uint32_t>(st.n_items, 65535);
+    return 0;
+}
+
+int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_ids, const Tuning& tn,
+            std::vector<WorkItem>& items, Stage& st) {
+    uint32_t TV = tn.inhib_tile_v, TU = tn.inhib_tile_u, rmax = 0, W = 16;
+    for (uint32_t id : job_ids) {
+        const SscJob& j = jobs[id];
+        if (j.mode == 0) rmax = std::max(rmax, j.radius);
+        W = std::min(W, pick_w(j.spt, 0, 0, 16));
+    }
+    st.kind = 2;
+    st.item0 = (uint32_t)items.size();
+    for (uint32_t id : job_ids) {
+        const SscJob& j = jobs[id];
+        uint32_t ntv = (j.V + TV - 1) / TV, ntu = (j.U + TU - 1) / TU;
+        for (uint32_t s = 0; s < j.depth; s++)
+            for (uint32_t a = 0; a < ntv; a++)
+                for (uint32_t b = 0; b < ntu; b++)
+                    items.push_back(WorkItem{(uint16_t)id, (uint16_t)s, (uint16_t)a, (uint16_t)b});
+    }
+    st.n_items = (uint32_t)items.size() - st.item0;
+    uint32_t win = (TV + 2 * rmax) * (TU + 2 * rmax);
+    st.lds_bytes = ((win + 15) & ~15u) + 4 * TV * TU;
+    if (st.lds_bytes > LDS_LIMIT) return fail("inhibition window does not fit in LDS");
+    st.W = W;
+    st.tv = TV;
+    st.tu = TU;
+    st.grid = std::min<uint32_t>(st.n_items, 65535);
+    return 0;
+}
+
+int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_ids,
+            std::vector<WorkItem>& items, Stage& st) {
+    st.kind = 3;
+    st.item0 = (uint32_t)items.size();
+    for (uint32_t id : job_ids) {
+        const PyrJob& j = jobs[id];
+        uint32_t nb = (j.V * j.U + F3_COLS - 1) / F3_COLS;
+        if (nb > 65535) return fail("too many column blocks");
+        for (uint32_t b = 0; b < nb; b++) items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)b, 0});
+    }
+    st.n_items = (uint32_t)items.size() - st.item0;
+    st.lds_bytes = 0;
+    st.grid = std::min<uint32_t>(st.n_items, 65535);
+    return 0;
+}
+
+int ensure_rnd(bsm_area* a, size_t n_ints) {
+    if (n_ints <= a->rnd_cap) return 0;
+    size_t cap = std::max<size_t>(n_ints, 4096);
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    if (a->rnd_d) BSM_HIP(hipFree(a->rnd_d));
+    a->rnd_d = nullptr;
+    BSM_HIP(hipMalloc((void**)&a->rnd_d, cap * sizeof(int32_t)));
+    BSM_HIP(hipMemsetAsync(a->rnd_d, 0, cap * sizeof(int32_t), a->stream));
+    a->rnd_cap = cap;
+    a->plan_dirty = true;
+    return 0;
+}
+
+int build_plan(bsm_area* a) {
+    if (ensure_rnd(a, 4096)) return 1;
+    a->tjobs.clear();
+    a->sjobs.clear();
+    a->pjobs.clear();
+    a->stages.clear();
+    std::vector<WorkItem> items;
+    const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
+    const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
+    for (int pass = 0; pass < 2; pass++) {
+        for (uint32_t l = 0; l < a->n_layers; l++) {
+            LayerRt& L = a->L[l];
+            bool spatial = L.d.spatial != 0;
+            if ((pass == 0) != spatial) continue;
+            if (spatial && !sscs) continue;
+            if (!spatial && !pyrs) continue;
+            std::vector<uint32_t> ids, ws;
+            for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+                ids.push_back((uint32_t)a->tjobs.size());
+                ws.push_back(tuft_chunk(a, l, t, a->tune.chunk));
+                a->tjobs.push_back(make_tuft_job(a, l, t, a->tune, a->rnd_d));
+            }
+            Stage s1{};
+            if (plan_f1(a->tjobs, ids, ws, a->tune, items, s1)) return 1;
+            s1.layer = l;
+            a->stages.push_back(s1);
+            Stage s2{};
+            if (spatial) {
+                std::vector<uint32_t> sid{(uint32_t)a->sjobs.size()};
+                a->sjobs.push_back(make_ssc_job(a, l, a->rnd_d));
+                if (plan_f2(a->sjobs, sid, a->tune, items, s2)) return 1;
+            } else {
+                uint32_t w3;
+                std::vector<uint32_t> pid{(uint32_t)a->pjobs.size()};
+                a->pjobs.push_back(make_pyr_job(a, l, a->rnd_d, &w3));
+                if (plan_f3(a->pjobs, pid, items, s2)) return 1;
+                s2.W = w3;
+            }
+            s2.layer = l;
+            a->stages.push_back(s2);
+        }
+    }
+    (void)hipFree(a->tjobs_d);
+    (void)hipFree(a->sjobs_d);
+    (void)hipFree(a->pjobs_d);
+    (void)hipFree(a->items_d);
+    a->tjobs_d = nullptr;
+    a->sjobs_d = nullptr;
+    a->pjobs_d = nullptr;
+    a->items_d = nullptr;
+    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+        BSM_HIP(hipMalloc(dst, bytes ? bytes : 16));
+        if (bytes) BSM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (up((void**)&a->tjobs_d, a->tjobs.data(), a->tjobs.size() * sizeof(TuftJob))) return 1;
+    if (up((void**)&a->sjobs_d, a->sjobs.data(), a->sjobs.size() * sizeof(SscJob))) return 1;
+    if (up((void**)&a->pjobs_d, a->pjobs.data(), a->pjobs.size() * sizeof(PyrJob))) return 1;
+    if (up((void**)&a->items_d, items.data(), items.size() * sizeof(WorkItem))) return 1;
+    a->plan_dirty = false;
+    return 0;
+}
+
+#define F1_CASE(w, n)                                                                       \
+    case (w * 8 + n):                                                                       \
+        kf_tuft<w, n><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par, rnd_ofs, \
+                                                         st.tv, st.tu, st.stage_cap);       \
+        break;
+
+int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
+                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs) {
+    if (st.n_items == 0) return 0;
+    const WorkItem* it = items + st.item0;
+    if (st.kind == 1) {
+        switch (st.W * 8 + st.nst) {
+            F1_CASE(16, 1) F1_CASE(16, 2) F1_CASE(16, 4)
+            F1_CASE(8, 1) F1_CASE(8, 2) F1_CASE(8, 4)
+            F1_CASE(4, 1) F1_CASE(4, 2) F1_CASE(4, 4)
+            F1_CASE(1, 1) F1_CASE(1, 2) F1_CASE(1, 4)
+        default: return fail("bad F1 variant");
+        }
+    } else if (st.kind == 2) {
+        switch (st.W) {
+        case 16: kf_ssc_post<16><<<st.grid, 256, st.lds_bytes, s>>>(sj, it, st.n_items, par, rnd_ofs, st.tv, st.tu); break;
+        case 8: kf_ssc_post<8><<<st.grid, 256, st.lds_bytes, s>>>(sj, it, st.n_items, par, rnd_ofs, st.tv, st.tu); break;
+        case 4: kf_ssc_post<4><<<st.grid, 256, st.lds_bytes, s>>>(sj, it, st.n_items, par, rnd_ofs, st.tv, st.tu); break;
+        default: kf_ssc_post<1><<<st.grid, 256, st.lds_bytes, s>>>(sj, it, st.n_items, par, rnd_ofs, st.tv, st.tu); break;
+        }
+    } else {
+        switch (st.W) {
+        case 16: kf_pyr_post<16><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
+        case 8: kf_pyr_post<8><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
+        case 4: kf_pyr_post<4><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
+        default: kf_pyr_post<1><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
+        }
+    }
+    BSM_HIP(hipGetLastError());
+    return 0;
+}
+
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs) {
+    for (const Stage& st : a->stages) {
+        LayerRt& L = a->L[st.layer];
+        if (st.kind == 1) L.cur ^= 1;
+        if (launch_stage(a->stream, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
+                         (uint32_t)L.cur, rnd_ofs))
+            return 1;
+    }
+    return 0;
+}
+
+// Upload n_cycles of rnd draws (user-given or drawn from the area stream) to
+// the device through a pinned double buffer.
+static int upload_rnd(bsm_area* a, const int32_t* rnd, uint32_t n_cycles) {
+    size_t n = (size_t)n_cycles * a->n_slots;
+    if (ensure_rnd(a, n)) return 1;
+    if (a->plan_dirty && build_plan(a)) return 1;
+    if (!n) return 0;
+    int k = a->rnd_pin_next;
+    a->rnd_pin_next ^= 1;
+    BSM_HIP(hipEventSynchronize(a->rnd_ev[k]));
+    if (a->rnd_pin_cap[k] < n) {
+        if (a->rnd_pin[k]) BSM_HIP(hipHostFree(a->rnd_pin[k]));
+        a->rnd_pin[k] = nullptr;
+        size_t cap = std::max<size_t>(n, 4096);
+        BSM_HIP(hipHostMalloc((void**)&a->rnd_pin[k], cap * sizeof(int32_t), hipHostMallocDefault));
+        a->rnd_pin_cap[k] = cap;
+    }
+    int32_t* h = a->rnd_pin[k];
+    if (rnd) {
+        std::memcpy(h, rnd, n * sizeof(int32_t));
+    } else {
+        for (size_t i = 0; i < n; i++) h[i] = (int32_t)rng_next(a->rng);
+    }
+    BSM_HIP(hipMemcpyAsync(a->rnd_d, h, n * sizeof(int32_t), hipMemcpyHostToDevice, a->stream));
+    BSM_HIP(hipEventRecord(a->rnd_ev[k], a->stream));
+    return 0;
+}
+
+}  // namespace bsm
+
+using namespace bsm;
+
+extern "C" {
+
+int bsm_area_cycle(bsm_area* a, const int32_t* rnd, uint32_t n_cycles) {
+    if (!a) return fail("null area");
+    if (a->batch_stream) return fail("area belongs to a batch; cycle it through the batch");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (upload_rnd(a, rnd, n_cycles)) return 1;
+    for (uint32_t c = 0; c < n_cycles; c++)
+        if (cycle_fused(a, c * a->n_slots)) return 1;
+    return 0;
+}
+
+int bsm_area_run(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
+                 uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles) {
+    if (!a) return fail("null area");
+    if (a->batch_stream) return fail("area belongs to a batch; cycle it through the batch");
+    if (dev_frames && (axn_offset > a->axn_len || frame_len > a->axn_len - axn_offset))
+        return fail("frame range out of the axon space");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (upload_rnd(a, nullptr, n_cycles)) return 1;
+    for (uint32_t c = 0; c < n_cycles; c++) {
+        if (dev_frames && frame_len)
+            BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
+                                   frame_len, hipMemcpyDeviceToDevice, a->stream));
+        if (cycle_fused(a, c * a->n_slots)) return 1;
+    }
+    return 0;
+}
+
+int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
+                         uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles,
+                         float* ms_total) {
+    if (!a || !ms_total) return fail("null arg");
+    if (a->batch_stream) return fail("area belongs to a batch; cycle it through the batch");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    // draw + upload outside the timed span
+    if (upload_rnd(a, nullptr, n_cycles)) return 1;
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    BSM_HIP(hipEventRecord(a->ev0, a->stream));
+    for (uint32_t c = 0; c < n_cycles; c++) {
+        if (dev_frames && frame_len)
+            BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
+                                   frame_len, hipMemcpyDeviceToDevice, a->stream));
+        if (cycle_fused(a, c * a->n_slots)) return 1;
+    }
+    BSM_HIP(hipEventRecord(a->ev1, a->stream));
+    BSM_HIP(hipEventSynchronize(a->ev1));
+    BSM_HIP(hipEventElapsedTime(ms_total, a->ev0, a->ev1));
+    return 0;
+}
+
+}  // extern "C"

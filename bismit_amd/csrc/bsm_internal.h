@@ -1,0 +1,215 @@
+// bsm_internal.h -- host-side engine structures shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/bismit_hip.h"
+#include "bsm_device.h"
+
+namespace bsm {
+
+void set_error(const std::string& msg);
+int fail(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define BSM_HIP(call)                                            \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::bsm::hip_fail(e_, #call); \
+    } while (0)
+
+// ---------------------------------------------------------------- jobs
+// F1 work unit: one tuft of one layer of one area (gather + dendrites [+ tufts]).
+struct TuftJob {
+    const uint8_t* axn;
+    const SliceRt* slices;
+    const uint8_t* slc;
+    const int8_t* vofs;
+    const int8_t* uofs;
+    const int8_t* str;
+    uint8_t* st[2];        // tuft base in the two state buffers; st[par] is written
+    uint8_t* den_raw;      // tuft dendrite base
+    uint8_t* den_st;
+    uint8_t* den_act;
+    uint8_t* ssc_energy;   // SSC layers: cell energies (ssc_cycle fused on dens < C)
+    uint8_t* tft[8];       // cell-tuft arrays (offset t*C) or null: best_ids, best_raw,
+                           // best, states, prev_best_ids, prev_best_raw, prev_best, prev_states
+    const int32_t* rnd;    // per-area rnd array
+    uint32_t rnd_slot;
+    uint32_t n_slices;
+    uint32_t V, U, depth, C;
+    uint32_t dens, spd, spt, thresh;
+    uint32_t den_idz;
+    uint32_t l2;
+    uint32_t n_staged;
+    uint32_t staged[4];
+    uint32_t scalar_quirk; // syns_per_tft % 4 != 0: char-wrapped mids (scalar kernel)
+    uint32_t halo;         // staged reach around the tile, in source-slice columns
+    uint32_t par_xor;      // state buffer written = st[(par ^ par_xor) & 1]
+    uint32_t _pad;
+    // host-side planning copies of the staged slices' geometry
+    uint16_t st_vsize[4], st_usize[4];
+    uint8_t st_vsc[4], st_usc[4];
+};
+
+// F2 work unit: SSC layer post-processing (inhibition + SSC LTP).
+struct SscJob {
+    const uint8_t* soma;   // dens.states of the layer (soma alias)
+    uint8_t* axn;
+    const SliceRt* slices;
+    uint8_t* cel_act;
+    uint8_t* st[2];        // layer synapse state buffers
+    int8_t* str;
+    const int32_t* rnd;
+    uint32_t slot_ctrl, slot_mtp;
+    uint32_t V, U, depth, C;
+    uint32_t spt, base_slc, radius;
+    uint32_t mode;         // 0 inhib_simple, 1 passthrough(rnd 0), 2 none
+    uint32_t learn;
+    uint32_t par_xor;
+};
+
+// F3 work unit: pyramidal layer post-processing (soma + column control + distal LTP).
+struct PyrJob {
+    const uint8_t* tft_states;
+    const uint8_t* tft_best_raw;
+    const uint8_t* prev_best_raw;
+    const uint8_t* prev_best_ids;
+    uint8_t* soma;
+    uint8_t* cel_flags;
+    uint8_t* axn;
+    const SliceRt* slices;
+    uint8_t* st[2];
+    int8_t* str;
+    const int32_t* rnd;
+    uint32_t slot_mtp0;
+    uint32_t V, U, depth, C;
+    uint32_t en, prx, bdst, adst;
+    uint32_t control;      // BSM_CTRL_INTRA_COLUMN / BSM_CTRL_PYR_OUTPUT / NONE
+    uint32_t n_tufts;
+    uint32_t base_slc;
+    uint32_t learn;
+    uint32_t distal_mask;
+    uint32_t par_xor;
+    uint32_t max_chunks;   // max over distal tufts of syns_per_den / chunk
+    uint32_t t_spd[BSM_MAX_TUFTS];
+    uint32_t t_spt[BSM_MAX_TUFTS];
+    uint32_t t_syn_idz[BSM_MAX_TUFTS];
+};
+
+struct WorkItem {
+    uint16_t job, slc, tv, tu;
+};
+
+// ---------------------------------------------------------------- area
+struct TuftRt {
+    bsm_tuft_desc d;
+    uint32_t spt;
+    uint64_t syn_idz, den_idz;
+};
+
+struct LayerRt {
+    bsm_layer_desc d;
+    uint32_t cells;
+    uint64_t n_syn, n_den, n_celtft;
+    TuftRt t[BSM_MAX_TUFTS];
+    uint8_t* syn_state[2];   // double-buffered synapse states
+    int cur;                 // index of the current state buffer
+    uint8_t* syn_flag_static;  // flag bits other than SYN_PREV_ACTIVE (never touched by kernels)
+    uint8_t* syn_slc;
+    int8_t* syn_v;
+    int8_t* syn_u;
+    int8_t* syn_str;
+    uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;
+    uint8_t* tft[8];
+    uint8_t *soma, *cel_flags, *cel_energies, *cel_act;
+    uint32_t slot0;          // first rnd slot of this layer
+};
+
+struct Tuning {
+    uint32_t tile_v = 8, tile_u = 8, halo = 8, chunk = 16;
+    uint32_t inhib_tile_v = 16, inhib_tile_u = 16;
+};
+
+// One launch of a fused stage over a contiguous range of work items.
+struct Stage {
+    int kind;            // 1 = F1 tufts, 2 = F2 ssc post, 3 = F3 pyr post
+    uint32_t item0, n_items;
+    uint32_t lds_bytes;
+    uint32_t layer;      // plain form: layer whose state parity flips at F1
+    uint32_t W;          // synapse chunk width (bytes per lane)
+    uint32_t nst;        // F1: staged-slice template (1, 2 or 4)
+    uint32_t stage_cap;  // F1: LDS bytes reserved for staged windows
+    uint32_t tv, tu;     // tile extents in columns
+    uint32_t grid;
+};
+
+}  // namespace bsm
+
+struct bsm_ctx {
+    int device;
+    hipDeviceProp_t prop;
+};
+
+struct bsm_area {
+    bsm_ctx* ctx;
+    hipStream_t stream;
+    uint32_t V, U, n_slices, n_layers, settings;
+    std::vector<bsm_slice_desc> slice_desc;
+    std::vector<bsm::SliceRt> slices_h;
+    bsm::SliceRt* slices_d;
+    uint32_t axn_len;
+    uint8_t* axn;
+    bsm::LayerRt L[BSM_MAX_LAYERS];
+    uint32_t n_slots;
+    uint32_t rng[4];         // xorshift128 state (host side)
+    uint8_t* arena;          // single device allocation backing all buffers
+    size_t arena_bytes;
+    int32_t* rnd_d;          // device rnd draws (capacity rnd_cap ints)
+    size_t rnd_cap;
+    int32_t* rnd_pin[2];     // pinned staging for per-call rnd uploads
+    size_t rnd_pin_cap[2];
+    hipEvent_t rnd_ev[2];
+    int rnd_pin_next;
+    bsm::Tuning tune;
+    // fused-path job tables (device) and launch geometry
+    bsm::TuftJob* tjobs_d;
+    bsm::SscJob* sjobs_d;
+    bsm::PyrJob* pjobs_d;
+    bsm::WorkItem* items_d;
+    std::vector<bsm::TuftJob> tjobs;
+    std::vector<bsm::SscJob> sjobs;
+    std::vector<bsm::PyrJob> pjobs;
+    std::vector<bsm::Stage> stages;
+    bool plan_dirty;
+    hipEvent_t ev0, ev1;
+    hipStream_t batch_stream;  // non-null while the area belongs to a batch
+};
+
+namespace bsm {
+// planning and launchers implemented in bsm_fused.hip
+TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, const int32_t* rnd);
+SscJob make_ssc_job(bsm_area* a, uint32_t l, const int32_t* rnd);
+PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3);
+uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w);
+uint32_t ssc_chunk(const bsm_area* a, uint32_t l);
+// Appends the items of one stage and fills its geometry; jobs[j] indexes the
+// job arrays.  Returns 0 or an error.
+int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
+            const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
+            Stage& st);
+int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_ids, const Tuning& tn,
+            std::vector<WorkItem>& items, Stage& st);
+int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_ids,
+            std::vector<WorkItem>& items, Stage& st);
+int build_plan(bsm_area* a);
+int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
+                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs);
+int ensure_rnd(bsm_area* a, size_t n_ints);
+void rng_seed(uint32_t st[4], uint64_t seed);
+uint32_t rng_next(uint32_t st[4]);
+}  // namespace bsm

@@ -1,0 +1,269 @@
+/*
+ * bismit_hip.h -- C ABI of the MI355X cortical-cycle engine.
+ *
+ * This is the drop-in boundary that replaces bismit's OpenCL work queue (the
+ * `ocl` crate's ProQue/Buffer/Kernel/Event use inside src/cortex, src/map and
+ * src/flywheel.rs).  A Rust FFI crate would declare every function below as
+ * `extern "C"`; this repository's C++ orchestrator, batch/serve loops and the
+ * Python test harness (ctypes) call it directly.
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = OK, non-zero = error.  The
+ *     message of the last error on the calling thread is bsm_last_error().
+ *     No exception or panic crosses the ABI.
+ *   - The library owns all device memory.  Host pointers stay owned by the
+ *     caller; a read/write is complete when the call returns.
+ *   - Buffer layouts are the reference's (SoA, tuft-major synapse space):
+ *       axon   (slice s, v, u)          -> axn_idz[s] + v*u_size[s] + u
+ *       synapse(tuft t, cell c, den d, k) -> tft_syn_idz[t] + c*dens*spd + d*spd + k
+ *       dendrite(tuft t, cell c, den d)   -> tft_den_idz[t] + c*dens + d
+ *       cell-tuft(tuft t, cell c)         -> t*C + c,   c = slc*v*u + v_id*u_size + u_id
+ *     (src/map/slice_map.rs:43-65, src/cortex/synapses.rs:816-844,
+ *      src/cortex/dendrites.rs:485-508, src/cortex/tufts.rs:91).
+ *   - One HIP stream per area (or per batch).  Calls on one area are not
+ *     re-entrant; different areas may be driven from different host threads.
+ */
+#ifndef BISMIT_HIP_H
+#define BISMIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSM_MAX_SLICES 255
+#define BSM_MAX_LAYERS 8
+#define BSM_MAX_TUFTS 3
+#define BSM_MAX_TUFT_SRCS 8
+
+/* Layer kinds (src/cortex/spiny_stellates.rs, src/cortex/pyramidals.rs). */
+enum bsm_layer_kind { BSM_LAYER_SSC = 1, BSM_LAYER_PYR = 2 };
+/* Dendrite class/kind of a tuft (src/map/scheme/cell.rs TuftScheme). */
+enum bsm_den_class { BSM_DEN_BASAL = 0, BSM_DEN_APICAL = 1 };
+enum bsm_den_kind { BSM_DEN_PROXIMAL = 0, BSM_DEN_DISTAL = 1 };
+/* Control layer attached to a host layer (src/cortex/iinn.rs,
+ * src/cortex/intra_column_inhib.rs, src/cortex/pyr_outputter.rs). */
+enum bsm_control_kind {
+    BSM_CTRL_NONE = 0,
+    BSM_CTRL_INHIB_SIMPLE = 1,   /* basket-cell hex WTA (passthrough when bypassed) */
+    BSM_CTRL_INTRA_COLUMN = 2,
+    BSM_CTRL_PYR_OUTPUT = 3
+};
+/* CorticalAreaSettings switches (src/cortex/cortical_area.rs:253-338).
+ * Regrowth is always off in this engine (host-side, entropy-seeded upstream). */
+enum bsm_settings {
+    BSM_SET_BYPASS_INHIB = 1u << 0,
+    BSM_SET_DISABLE_PYRS = 1u << 1,
+    BSM_SET_DISABLE_SSCS = 1u << 2,
+    BSM_SET_DISABLE_LEARNING = 1u << 3
+};
+
+/* One axon slice: the runtime form of the -D slice tables baked into every
+ * reference program (cl/bismit.cl:108-114, src/map/area_map.rs:51-77).
+ * axn_idz is the prefix sum of v_size*u_size in slice order. */
+typedef struct bsm_slice_desc {
+    uint32_t v_size;
+    uint32_t u_size;
+    uint8_t v_scale;  /* 16 = 1:1, 0 = non-spatial (src/cmn/slice_dims.rs:200-223) */
+    uint8_t u_scale;
+    uint8_t v_mid;    /* non-spatial slices: size/2 */
+    uint8_t u_mid;
+} bsm_slice_desc;
+
+typedef struct bsm_tuft_desc {
+    uint8_t den_class;          /* enum bsm_den_class */
+    uint8_t den_kind;           /* enum bsm_den_kind */
+    uint8_t max_active_dens_l2; /* tft_cycle scaling (cl/bismit.cl:825) */
+    uint8_t n_src_slcs;         /* source slices (layout generation, LDS staging) */
+    uint8_t src_slc_ids[BSM_MAX_TUFT_SRCS];
+    uint32_t dens_per_tft;
+    uint32_t syns_per_den;
+    uint32_t threshold;         /* den_cycle_tft den_threshold (src/cortex/dendrites.rs:102) */
+} bsm_tuft_desc;
+
+typedef struct bsm_layer_desc {
+    uint8_t kind;          /* enum bsm_layer_kind */
+    uint8_t spatial;       /* LayerTags::SPATIAL role (cycled+learned before the others) */
+    uint8_t depth;         /* slices of this layer */
+    uint8_t base_axn_slc;  /* first axon slice of this layer */
+    uint8_t n_tufts;       /* SSC: exactly 1; PYR: 1..3 */
+    uint8_t control;       /* enum bsm_control_kind */
+    uint8_t _pad[2];
+    int32_t inhib_radius;  /* BSM_CTRL_INHIB_SIMPLE field radius */
+    bsm_tuft_desc tufts[BSM_MAX_TUFTS];
+} bsm_layer_desc;
+
+typedef struct bsm_area_desc {
+    uint32_t v_size;       /* area (cell layer) dimensions */
+    uint32_t u_size;
+    uint32_t n_slices;
+    uint32_t n_layers;
+    uint32_t settings;     /* enum bsm_settings */
+    uint32_t _pad;
+    const bsm_slice_desc* slices;
+    const bsm_layer_desc* layers;   /* in the area's cycle order */
+} bsm_area_desc;
+
+/* Buffers, mirroring SamplerKind (src/cortex/cortical_area.rs:61-90). */
+enum bsm_buf {
+    BSM_AXN_STATES = 0,             /* whole axon space (layer ignored) */
+    BSM_SOMA_STATES,                /* PYR: soma u8[C]; SSC: dens.states (aliased) */
+    BSM_SOMA_ENERGIES,
+    BSM_SOMA_ACTIVITIES,
+    BSM_SOMA_FLAG_SETS,             /* PYR only */
+    BSM_TFT_STATES,
+    BSM_TFT_BEST_DEN_IDS,
+    BSM_TFT_BEST_DEN_STATES_RAW,
+    BSM_TFT_BEST_DEN_STATES,
+    BSM_TFT_PREV_STATES,
+    BSM_TFT_PREV_BEST_DEN_IDS,
+    BSM_TFT_PREV_BEST_DEN_STATES_RAW,
+    BSM_TFT_PREV_BEST_DEN_STATES,
+    BSM_DEN_STATES,
+    BSM_DEN_STATES_RAW,
+    BSM_DEN_ENERGIES,
+    BSM_DEN_ACTIVITIES,
+    BSM_DEN_THRESHOLDS,
+    BSM_SYN_STATES,
+    BSM_SYN_STRENGTHS,              /* i8 */
+    BSM_SYN_SRC_SLC_IDS,
+    BSM_SYN_SRC_COL_V_OFFS,         /* i8 */
+    BSM_SYN_SRC_COL_U_OFFS,         /* i8 */
+    BSM_SYN_FLAG_SETS,
+    BSM_BUF_COUNT
+};
+
+/* rnd slot kinds: one int32 per launch per cycle (src/cortex/dendrites.rs:198,
+ * src/cortex/iinn.rs:113, src/cortex/spiny_stellates.rs:259, src/cortex/tufts.rs:312).
+ * Slot order: for each layer in descriptor order: DEN[t] for each tuft, CTRL,
+ * MTP[t] for each tuft (always allocated, used or not). */
+enum bsm_rnd_kind { BSM_RND_DEN = 0, BSM_RND_CTRL = 1, BSM_RND_MTP = 2 };
+
+typedef struct bsm_ctx bsm_ctx;
+typedef struct bsm_area bsm_area;
+typedef struct bsm_batch bsm_batch;
+
+/* ---- context (replaces ProQue/Context construction, src/cortex/cortex.rs:51-90) ---- */
+const char* bsm_last_error(void);
+int bsm_ctx_create(int device, bsm_ctx** out);
+int bsm_ctx_destroy(bsm_ctx* ctx);
+int bsm_sync(bsm_ctx* ctx);
+int bsm_device_info(bsm_ctx* ctx, char* name_buf, size_t name_len, int* cu_count,
+                    size_t* total_mem);
+
+/* ---- area (replaces CorticalArea::new buffer/kernel setup,
+ *      src/cortex/cortical_area.rs:376-642) ---- */
+int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed,
+                    bsm_area** out);
+int bsm_area_destroy(bsm_area* area);
+int bsm_area_sync(bsm_area* area);
+/* Bytes of device memory the area holds (admission by HBM footprint). */
+int bsm_area_footprint(const bsm_area_desc* desc, size_t* bytes);
+int bsm_area_axn_len(bsm_area* area, uint32_t* len);
+int bsm_area_syn_count(bsm_area* area, uint64_t* n);
+int bsm_area_slice_idz(bsm_area* area, uint32_t slc, uint32_t* idz);
+
+/* ---- buffer I/O (replaces Buffer read/write/fill and the test traits,
+ *      src/cortex/synapses.rs:536-549, src/cortex/axon_space.rs:650-668) ---- */
+int bsm_area_buf_len(bsm_area* area, int buf, uint32_t layer, size_t* len);
+int bsm_area_read(bsm_area* area, int buf, uint32_t layer, size_t offset, size_t len,
+                  void* host_dst);
+int bsm_area_write(bsm_area* area, int buf, uint32_t layer, size_t offset, size_t len,
+                   const void* host_src);
+int bsm_area_fill(bsm_area* area, int buf, uint32_t layer, uint8_t val);
+
+/* ---- axon intake/output (replaces AxonSpace::intake/output map+memcpy,
+ *      src/cortex/axon_space.rs:509-629).  src_is_device != 0: src is a device
+ *      pointer and the copy is enqueued on the area stream (async). ---- */
+int bsm_area_intake(bsm_area* area, uint32_t axn_offset, const uint8_t* src, uint32_t len,
+                    int src_is_device);
+int bsm_area_output(bsm_area* area, uint32_t axn_offset, uint8_t* host_dst, uint32_t len);
+
+/* ---- per-kernel entry points, reference argument semantics (parity units) ---- */
+int bsm_k_syn_flags(bsm_area* a, uint32_t layer);                        /* cl/syns.cl:23-33 */
+int bsm_k_syn_cycle(bsm_area* a, uint32_t layer, uint32_t tuft);         /* cl/syns.cl:285-388 */
+int bsm_k_den_cycle(bsm_area* a, uint32_t layer, uint32_t tuft, int32_t rnd); /* cl/bismit.cl:695-747 */
+int bsm_k_tft_cycle(bsm_area* a, uint32_t layer, uint32_t tuft);         /* cl/bismit.cl:755-842 */
+int bsm_k_ssc_cycle(bsm_area* a, uint32_t layer);                        /* cl/bismit.cl:849-884 */
+int bsm_k_inhib_simple(bsm_area* a, uint32_t layer, int32_t rnd);        /* cl/control.cl:28-191 */
+int bsm_k_inhib_passthrough(bsm_area* a, uint32_t layer, int32_t rnd);   /* cl/control.cl:194-217 */
+int bsm_k_ssc_mtp(bsm_area* a, uint32_t layer, uint32_t rnd);            /* cl/bismit.cl:888-915 */
+int bsm_k_pyr_cycle(bsm_area* a, uint32_t layer);                        /* cl/bismit.cl:961-1022 */
+int bsm_k_inhib_intra_column(bsm_area* a, uint32_t layer);               /* cl/control.cl:319-351 */
+int bsm_k_pyr_output(bsm_area* a, uint32_t layer);                       /* cl/control.cl:298-314 */
+int bsm_k_tft_dst_mtp(bsm_area* a, uint32_t layer, uint32_t tuft, int32_t rnd); /* cl/bismit.cl:1195-1268 */
+
+/* ---- whole cycle (replaces CorticalArea::cycle's enqueue sequence,
+ *      src/cortex/cortical_area.rs:716-789) ---- */
+int bsm_area_rnd_slots(bsm_area* area, uint32_t* n_slots);
+int bsm_area_rnd_slot_info(bsm_area* area, uint32_t slot, int* kind, uint32_t* layer,
+                           uint32_t* tuft);
+/* rnd: n_cycles*n_slots int32 draws (cycle-major) or NULL for the area's own
+ * xorshift128 stream (seeded by splitmix64(rnd_seed) at bsm_area_create).
+ * The intake for each cycle must be enqueued before the call (or use
+ * bsm_area_run). */
+int bsm_area_cycle(bsm_area* area, const int32_t* rnd, uint32_t n_cycles);
+/* n_cycles of intake + cycle: frame i (len bytes at dev_frames + i*frame_stride,
+ * a device pointer) is copied to axn_offset before cycle i. */
+int bsm_area_run(bsm_area* area, uint32_t axn_offset, const uint8_t* dev_frames,
+                 uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles);
+/* Draw n_cycles*n_slots values from the area's own stream (host side) without
+ * running anything; advances the stream exactly as bsm_area_cycle(NULL) would. */
+int bsm_area_draw_rnd(bsm_area* area, int32_t* out, uint32_t n_cycles);
+
+/* Tile parameters of the fused kernels (autotuner hook). 0 = default. */
+typedef struct bsm_tuning {
+    uint32_t tile_v;     /* columns per workgroup tile, v */
+    uint32_t tile_u;     /* columns per workgroup tile, u */
+    uint32_t halo;       /* LDS halo (columns of source reach staged) */
+    uint32_t chunk;      /* synapses per lane chunk: 4, 8 or 16 */
+} bsm_tuning;
+int bsm_area_set_tuning(bsm_area* area, const bsm_tuning* t);
+int bsm_area_get_tuning(bsm_area* area, bsm_tuning* t);
+
+/* ---- device memory helpers for resident inputs ---- */
+int bsm_dev_alloc(bsm_ctx* ctx, size_t bytes, void** dptr);
+int bsm_dev_free(bsm_ctx* ctx, void* dptr);
+int bsm_dev_upload(bsm_ctx* ctx, void* dptr, const void* host_src, size_t bytes);
+
+/* ---- timing (HIP events on the area/batch stream) ---- */
+int bsm_area_time_cycles(bsm_area* area, uint32_t axn_offset, const uint8_t* dev_frames,
+                         uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles,
+                         float* ms_total);
+
+/* ---- host-side synapse layout generator (replaces Synapses::grow(init=true),
+ *      src/cortex/synapses.rs:454-501, entropy-seeded upstream): src slice
+ *      uniform over the tuft's sources, v/u offsets uniform in [-reach, reach],
+ *      from splitmix64(seed, layer, tuft, index). ---- */
+int bsm_gen_tuft_layout(uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n_syns,
+                        uint32_t n_src, const uint8_t* src_ids, int32_t reach,
+                        uint8_t* out_slc, int8_t* out_v, int8_t* out_u);
+/* Fill every tuft of every layer of the area with bsm_gen_tuft_layout and
+ * zero strengths (runs the generator on the host, then uploads). */
+int bsm_area_init_layout(bsm_area* area, uint64_t seed, int32_t reach);
+
+/* ---- deterministic rnd schedule helpers ---- */
+uint64_t bsm_splitmix64(uint64_t* state);
+/* xorshift128 (Marsaglia 2003) stream seeded from splitmix64(seed). */
+int bsm_xorshift128_fill(uint64_t seed, uint32_t* out, uint64_t n);
+
+/* ---- batched form: many areas per stage in one persistent launch ---- */
+int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm_batch** out);
+int bsm_batch_destroy(bsm_batch* b);
+/* use_graph != 0: each cycle replays a captured hipGraph. */
+int bsm_batch_cycle(bsm_batch* b, uint32_t n_cycles, int use_graph);
+/* Per-area input frames: frame i of area a comes from dev_frames[a] + i*stride[a]
+ * (len[a] bytes, to axn_offset[a]).  NULL dev_frames: no intake. */
+int bsm_batch_set_inputs(bsm_batch* b, const uint8_t* const* dev_frames,
+                         const uint64_t* stride, const uint32_t* len,
+                         const uint32_t* axn_offset, uint32_t n_frames);
+int bsm_batch_sync(bsm_batch* b);
+int bsm_batch_time_cycles(bsm_batch* b, uint32_t n_cycles, int use_graph, float* ms_total);
+int bsm_batch_set_tuning(bsm_batch* b, const bsm_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BISMIT_HIP_H */

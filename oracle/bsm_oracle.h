@@ -1,0 +1,62 @@
+/*
+ * bsm_oracle.h -- CPU restatement of bismit's per-cycle cortical pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP engine is compared
+ * against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product path (bismit_amd) never links, loads or calls it.
+ *
+ * Every function restates one reference kernel or helper and cites the
+ * reference file:line it follows.  Buffers use the reference layouts exactly
+ * (see include/bismit_hip.h) and the same area descriptor type.
+ */
+#ifndef BSM_ORACLE_H
+#define BSM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "../include/bismit_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_area orc_area;
+
+/* helpers (pinned individually by tests) */
+int32_t orc_rnd_mix(int32_t rnd, int32_t seed);
+int32_t orc_rnd_inc(int32_t rnd, int32_t seed, int8_t val, int32_t mask);
+int32_t orc_rnd_dec(int32_t rnd, int32_t seed, int8_t val, int32_t mask);
+uint8_t orc_update_activity_rating(uint8_t a, int32_t is_active, int32_t rnd, int32_t seed,
+                                   uint16_t decay);
+uint8_t orc_syn_fire(uint8_t axn);
+int32_t orc_lshft_mask(int32_t l2i);
+
+/* area */
+int orc_area_create(const bsm_area_desc* desc, orc_area** out);
+void orc_area_destroy(orc_area* a);
+int orc_buf_ptr(orc_area* a, int buf, uint32_t layer, void** ptr, size_t* len);
+int orc_slice_idz(orc_area* a, uint32_t slc, uint32_t* idz);
+void orc_set_threads(int n);
+int orc_rnd_slots(orc_area* a, uint32_t* n);
+
+/* kernels (same argument meaning as the bsm_k_* entry points) */
+int orc_k_syn_flags(orc_area* a, uint32_t layer);
+int orc_k_syn_cycle(orc_area* a, uint32_t layer, uint32_t tuft);
+int orc_k_den_cycle(orc_area* a, uint32_t layer, uint32_t tuft, int32_t rnd);
+int orc_k_tft_cycle(orc_area* a, uint32_t layer, uint32_t tuft);
+int orc_k_ssc_cycle(orc_area* a, uint32_t layer);
+int orc_k_inhib_simple(orc_area* a, uint32_t layer, int32_t rnd);
+int orc_k_inhib_passthrough(orc_area* a, uint32_t layer, int32_t rnd);
+int orc_k_ssc_mtp(orc_area* a, uint32_t layer, uint32_t rnd);
+int orc_k_pyr_cycle(orc_area* a, uint32_t layer);
+int orc_k_inhib_intra_column(orc_area* a, uint32_t layer);
+int orc_k_pyr_output(orc_area* a, uint32_t layer);
+int orc_k_tft_dst_mtp(orc_area* a, uint32_t layer, uint32_t tuft, int32_t rnd);
+
+/* whole cycle, reference enqueue order; rnd = n_cycles * n_slots draws */
+int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
