@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark: synapse-updates/s of the per-cycle cortical pipeline (cycle + LTP)
+on one MI355X, Config 2 workload: one 256x256-column area in the plain form,
+SSC 1x4x32 + pyramidal 2 slices x 2 tufts x 8x64 (142,606,336 synapses), 2%
+SDR input with 50% frame-to-frame overlap, learning on.  Input frames are
+resident in HBM before the timed span; each timed cycle copies its frame into
+the axon space and runs every fused stage.
+
+Prints one JSON line (see the driver contract).  --gpus N > 1 exits: this
+engine runs on one GPU.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
+
+
+def stage_bytes(scheme, li, kind):
+    """Algorithmic HBM bytes of one fused stage launch on layer li:
+    F1: per synapse 3 B metadata + 1 B strength read, 1 B state write; per
+        dendrite raw/state writes + activity read/write (4 B); per cell-tuft 4 B
+        read + 8 B written (pyramidal), per SSC cell 1 B energy read.
+    F2/F3 are reported but not used for the roofline."""
+    lay = scheme.layers[li]
+    C = scheme.cells(li)
+    if kind == 1:
+        b = 0
+        for t in lay.tufts:
+            b += C * t.syns_per_tft * 5 + C * t.dens_per_tft * 4
+            if lay.kind == 2:
+                b += C * 12
+        if lay.kind == 1:
+            b += C
+        return b
+    return None
+
+
+def read_traffic():
+    """HBM bytes per dominant-kernel launch from the committed rocprof PMC summary."""
+    try:
+        for line in open(PMC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(scheme, n_cycles, frames):
+    """The oracle (plain-C restatement, OpenMP) timed on this host on a bounded
+    sample of the same workload."""
+    import numpy as np
+    from oracle import pyoracle as P
+    from bismit_amd import _lib as L
+    threads = min(16, os.cpu_count() or 1)
+    P.set_threads(threads)
+    o = P.OracleArea(scheme)
+    for li, lay in enumerate(scheme.layers):
+        cells = scheme.cells(li)
+        parts = [L.gen_tuft_layout(0, li, t, cells * ts.syns_per_tft, ts.src_slcs, 6)
+                 for t, ts in enumerate(lay.tufts)]
+        for k, name in enumerate(["SYN_SRC_SLC_IDS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS"]):
+            o.write(name, li, np.concatenate([p[k] for p in parts]))
+    rnd = L.xorshift128(0, (n_cycles + 2) * o.n_slots).view(np.int32)
+    for c in range(2):
+        o.intake(frames[c])
+        o.cycle(rnd[c * o.n_slots:(c + 1) * o.n_slots], 1)
+    t0 = time.perf_counter()
+    for c in range(2, n_cycles + 2):
+        o.intake(frames[c % len(frames)])
+        o.cycle(rnd[c * o.n_slots:(c + 1) * o.n_slots], 1)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": scheme.n_synapses() * n_cycles / dt, "unit": "synapse-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{n_cycles} cycles of the same Config 2 area (after 2 warm-up cycles), "
+                      f"oracle/bsm_oracle.c with OpenMP over {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000, help="timed cycles per run")
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--runs", type=int, default=5, help="timed runs (median reported)")
+    ap.add_argument("--cpu-cycles", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="profiling helper: one copy-kernel calibration launch, then 3 cycles")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from bismit_amd import inputs, recipes
+    from bismit_amd.cortex import Context, CorticalArea
+
+    ctx = Context(0)
+    scheme = recipes.config2_area()
+    area = CorticalArea(scheme, ctx, rnd_seed=0, layout_seed=0, reach=6)
+    in_ofs, in_len = scheme.input_range()
+    n_frames = max(args.steps, args.warmup, 8)
+    frames = inputs.SdrStream(in_len, 0.02, 0.5, seed=0).frames(n_frames)
+    dev = ctx.alloc(frames.nbytes)
+    ctx.upload(dev, frames)
+
+    if args.pmc_probe:
+        n = 256 << 20
+        a, b = ctx.alloc(n), ctx.alloc(n)
+        ms = C_float_copy(ctx, a, b, n)
+        print(f"calib copy {n} bytes {ms:.3f} ms", flush=True)
+        area.run(dev, in_len, in_len, 3)
+        area.sync()
+        print("pmc probe done", flush=True)
+        return
+
+    area.run(dev, in_len, in_len, args.warmup)
+    area.sync()
+    times = []
+    for r in range(args.runs):
+        ctx.sync()
+        times.append(area.time_cycles(dev, in_len, in_len, args.steps))
+        print(f"# run {r}: {args.steps} cycles in {times[-1]:.2f} ms", file=sys.stderr, flush=True)
+    ms = statistics.median(times)
+    n_syn = scheme.n_synapses()
+    value = n_syn * args.steps / (ms / 1000.0)
+
+    # roofline of the dominant kernel: HIP events around every launch
+    prof = area.profile_stages(50)
+    kinds = {1: "kf_tuft", 2: "kf_ssc_post", 3: "kf_pyr_post"}
+    stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name,
+               "avg_us": 1000.0 * t / 50} for k, li, t in prof]
+    dom_i = max(range(len(prof)), key=lambda i: prof[i][2])
+    dk, dl, dt = prof[dom_i]
+    avg_s = dt / 50 / 1000.0
+    alg = stage_bytes(scheme, dl, dk)
+    achieved = alg / avg_s / 1e9
+    traffic = read_traffic()
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
+                "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2)}
+
+    out = {
+        "metric": "synapse-updates/s (cycle+LTP), batched vs plain on 1 GPU; p50/p99 frame latency",
+        "value": value,
+        "unit": "synapse-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "config2: one 256x256-column area, plain form, 142,606,336 synapses "
+                               "(SSC 1x4x32, pyramidal 2 slices x 2 tufts x 8x64), 2% SDR input "
+                               "with 50% overlap, learning on",
+                   "synapses": n_syn, "runs": args.runs, "device": ctx.info()["name"]},
+        "roofline": roofline,
+        "stages": stages,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scheme, args.cpu_cycles,
+                                           inputs.SdrStream(in_len, 0.02, 0.5, seed=0).frames(8))
+    area.close()
+    print(json.dumps(out), flush=True)
+
+
+def C_float_copy(ctx, dst, src, n):
+    import ctypes as C
+    from bismit_amd import _lib as L
+    ms = C.c_float()
+    L.check(ctx.lib.bsm_calib_copy(ctx.h, dst, src, n, C.byref(ms)))
+    return ms.value
+
+
+if __name__ == "__main__":
+    main()

@@ -445,7 +445,9 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                         if (!((J.distal_mask >> t) & 1u)) continue;
                         const uint32_t prev = (f & CEL_PREV_ACTIVE) != 0;
                         const uint32_t tpa = J.prev_best_raw[t * C + c] != 0;
-                        if (!prev && act && tpa) list[atomicAdd(&n_l, 1u)] = (c << 2) | t;
+                        const uint32_t bid = J.prev_best_ids[t * C + c];
+                        if (!prev && act && tpa && bid * J.t_spd[t] < J.t_spt[t])
+                            list[atomicAdd(&n_l, 1u)] = (c << 2) | t;
                         f = (f & ~(uint32_t)CEL_PREV_ACTIVE) | (act ? CEL_PREV_ACTIVE : 0u);
                     }
                     J.cel_flags[c] = (uint8_t)f;
@@ -937,6 +939,82 @@ int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_fr
     BSM_HIP(hipEventRecord(a->ev1, a->stream));
     BSM_HIP(hipEventSynchronize(a->ev1));
     BSM_HIP(hipEventElapsedTime(ms_total, a->ev0, a->ev1));
+    return 0;
+}
+
+}  // extern "C"
+
+namespace bsm {
+__global__ __launch_bounds__(256) void k_calib_copy(const uint4* __restrict__ src,
+                                                    uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace bsm
+
+extern "C" {
+
+int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages, float* stage_ms,
+                            int* stage_kind, uint32_t* stage_layer, uint32_t* n_stages) {
+    if (!a || !n_stages) return fail("null arg");
+    if (a->batch_stream) return fail("area belongs to a batch");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (upload_rnd(a, nullptr, n_cycles)) return 1;
+    const uint32_t ns = (uint32_t)a->stages.size();
+    *n_stages = ns;
+    if (ns > max_stages) return fail("max_stages too small");
+    std::vector<hipEvent_t> ev((size_t)2 * ns * n_cycles);
+    for (auto& e : ev) BSM_HIP(hipEventCreate(&e));
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    for (uint32_t c = 0; c < n_cycles; c++) {
+        for (uint32_t i = 0; i < ns; i++) {
+            const Stage& st = a->stages[i];
+            LayerRt& L = a->L[st.layer];
+            if (st.kind == 1) L.cur ^= 1;
+            BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i)], a->stream));
+            if (launch_stage(a->stream, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
+                             (uint32_t)L.cur, c * a->n_slots))
+                return 1;
+            BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i) + 1], a->stream));
+        }
+    }
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    for (uint32_t i = 0; i < ns; i++) {
+        float tot = 0;
+        for (uint32_t c = 0; c < n_cycles; c++) {
+            float ms = 0;
+            BSM_HIP(hipEventElapsedTime(&ms, ev[2 * ((size_t)c * ns + i)],
+                                        ev[2 * ((size_t)c * ns + i) + 1]));
+            tot += ms;
+        }
+        if (stage_ms) stage_ms[i] = tot;
+        if (stage_kind) stage_kind[i] = a->stages[i].kind;
+        if (stage_layer) stage_layer[i] = a->stages[i].layer;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return 0;
+}
+
+int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+    if (!ctx || !dst || !src) return fail("null arg");
+    if (bytes % 16) return fail("bytes must be a multiple of 16");
+    BSM_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    BSM_HIP(hipEventCreate(&e0));
+    BSM_HIP(hipEventCreate(&e1));
+    size_t n = bytes / 16;
+    uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
+    BSM_HIP(hipEventRecord(e0, 0));
+    k_calib_copy<<<grid, 256, 0, 0>>>((const uint4*)src, (uint4*)dst, n);
+    BSM_HIP(hipGetLastError());
+    BSM_HIP(hipEventRecord(e1, 0));
+    BSM_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BSM_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return 0;
 }
 

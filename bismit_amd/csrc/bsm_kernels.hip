@@ -202,7 +202,9 @@ __global__ void kr_tft_dst_mtp(const uint8_t* __restrict__ axn, uint32_t axn_idz
     uint32_t f = cel_flags[c];
     uint32_t act = axn[axn_idz + c] != 0;
     uint32_t prev = (f & CEL_PREV_ACTIVE) == CEL_PREV_ACTIVE;
-    if (!prev && act && pbraw[c] != 0) {
+    // a best-dendrite id beyond the tuft (only possible from a user write) is
+    // undefined upstream; it is skipped here
+    if (!prev && act && pbraw[c] != 0 && (uint32_t)pbid[c] * spd < spt) {
         uint32_t z = c * spt + tft_syn_idz + (uint32_t)pbid[c] * spd;
         for (uint32_t i = z; i < z + spd; i++) {
             int32_t s = str[i];

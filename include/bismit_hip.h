@@ -233,6 +233,17 @@ int bsm_area_time_cycles(bsm_area* area, uint32_t axn_offset, const uint8_t* dev
                          uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles,
                          float* ms_total);
 
+/* Per-stage device time of n_cycles plain cycles (HIP events around every fused
+ * launch on the area stream; no intake).  stage_ms[i] is the total over the
+ * cycles; stage_kind[i] is 1 (gather+dendrites+tufts), 2 (SSC inhibition+LTP)
+ * or 3 (pyramidal soma+column inhibition+LTP); stage_layer[i] its layer. */
+int bsm_area_profile_stages(bsm_area* area, uint32_t n_cycles, uint32_t max_stages,
+                            float* stage_ms, int* stage_kind, uint32_t* stage_layer,
+                            uint32_t* n_stages);
+/* Plain device-to-device copy kernel of `bytes` (multiple of 16): the known-size
+ * reference for scaling HBM traffic counters. */
+int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
+
 /* ---- host-side synapse layout generator (replaces Synapses::grow(init=true),
  *      src/cortex/synapses.rs:454-501, entropy-seeded upstream): src slice
  *      uniform over the tuft's sources, v/u offsets uniform in [-reach, reach],

@@ -615,7 +615,8 @@ int orc_k_tft_dst_mtp(orc_area* a, uint32_t layer, uint32_t tuft, int32_t rnd) {
         int newly = !prev & act;
         uint64_t ct = (uint64_t)tuft * C + c;
         int tpa = L->tft_prev_best_raw[ct] != 0;
-        if (newly & tpa) {
+        /* an out-of-range best-dendrite id (user-written) is undefined upstream; skipped */
+        if (newly & tpa && (uint32_t)L->tft_prev_best_ids[ct] * spd < spt) {
             uint32_t z = (uint32_t)((uint32_t)c * spt + (uint32_t)T->syn_idz) +
                          (uint32_t)L->tft_prev_best_ids[ct] * spd;
             for (uint32_t i = z; i < z + spd; i++) {

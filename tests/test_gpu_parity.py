@@ -1,0 +1,212 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle,
+bit-exact on every buffer.  Integer path: the bar is exact equality."""
+import os
+
+import numpy as np
+import pytest
+
+from bismit_amd import _lib as L
+from bismit_amd import inputs, recipes
+from bismit_amd.cortex import CorticalArea
+from oracle import pyoracle as P
+
+from helpers import compare, host_layout, load_layout, layer_bufs
+
+pytestmark = pytest.mark.gpu
+
+P.set_threads(min(16, os.cpu_count() or 1))
+
+
+def make_pair(ctx, scheme, layout_seed=0, reach=6, rnd_seed=0):
+    g = CorticalArea(scheme, ctx, rnd_seed=rnd_seed, init_layout=False)
+    o = P.OracleArea(scheme)
+    lay = host_layout(scheme, layout_seed, reach)
+    load_layout(g, scheme, lay)
+    load_layout(o, scheme, lay)
+    return g, o
+
+
+def randomize(g, o, scheme, rng, strengths=True):
+    """Random contents for every mutable buffer, identical on both sides."""
+    def put(name, li, arr):
+        g.write(name, li, arr)
+        o.write(name, li, arr)
+
+    axn = rng.choice(np.array([0, 0, 0, 0, 17, 128, 255], dtype=np.uint8), g.axn_len)
+    put("AXN_STATES", 0, axn)
+    for li, lay in enumerate(scheme.layers):
+        for name in layer_bufs(scheme, li, synapses=False):
+            if name == "SOMA_STATES" and lay.kind == L.LAYER_SSC:
+                continue  # alias of DEN_STATES
+            n = g.buf_len(name, li)
+            vals = rng.integers(0, 256, n).astype(np.uint8)
+            if name in ("TFT_BEST_DEN_IDS", "TFT_PREV_BEST_DEN_IDS"):
+                dmin = min(t.dens_per_tft for t in lay.tufts)
+                vals = rng.integers(0, dmin, n).astype(np.uint8)
+            if name in ("SOMA_ENERGIES",):
+                vals = rng.choice(np.array([0, 0, 100, 192, 255], dtype=np.uint8), n)
+            put(name, li, vals)
+        n = g.buf_len("SYN_STATES", li)
+        put("SYN_STATES", li, rng.choice(np.array([0, 0, 0, 128, 200, 255], dtype=np.uint8), n))
+        put("SYN_FLAG_SETS", li, rng.integers(0, 256, n).astype(np.uint8))
+        if strengths:
+            put("SYN_STRENGTHS", li, rng.integers(-127, 128, n).astype(np.int8))
+
+
+SCHEMES = {
+    "small": lambda: recipes.cortical_area("s", 16, input_side=12, pyr_depth=2, ssc_dens=2,
+                                           ssc_spd=8, pyr_dens=3, pyr_spd=12, inhib_radius=3,
+                                           thresholds=(200, 0, 300)),
+    "c1": recipes.config1_area,
+    "apical": lambda: _with_apical(),
+}
+
+
+def _with_apical():
+    s = recipes.cortical_area("a", 24, pyr_depth=3, ssc_dens=4, ssc_spd=16, pyr_dens=2,
+                              pyr_spd=20, thresholds=(300, 0, 100))
+    s.layers[1].tufts.append(recipes.TuftScheme.apical_distal(3, 8, 50, [1, 2]))
+    return s
+
+
+@pytest.mark.parametrize("which", ["small", "c1", "apical"])
+def test_kernels_one_by_one(ctx, which):
+    """Each bsm_k_* entry point against its oracle restatement on random state."""
+    scheme = SCHEMES[which]()
+    print("case", which, flush=True)
+    g, o = make_pair(ctx, scheme)
+    rng = np.random.default_rng(11)
+    randomize(g, o, scheme, rng)
+    compare(g, o, scheme, "init")
+    for li, lay in enumerate(scheme.layers):
+        steps = [("syn_flags", (li,))]
+        for t in range(len(lay.tufts)):
+            steps.append(("syn_cycle", (li, t)))
+            steps.append(("den_cycle", (li, t, int(rng.integers(-2**31, 2**31)))))
+        if lay.kind == L.LAYER_SSC:
+            steps += [("ssc_cycle", (li,)), ("inhib_simple", (li, int(rng.integers(-2**31, 2**31)))),
+                      ("ssc_mtp", (li, int(rng.integers(0, 2**32)))),
+                      ("inhib_passthrough", (li, int(rng.integers(-2**31, 2**31))))]
+        else:
+            for t in range(len(lay.tufts)):
+                steps.append(("tft_cycle", (li, t)))
+            steps += [("pyr_cycle", (li,)), ("inhib_intra_column", (li,)), ("pyr_output", (li,))]
+            for t in range(len(lay.tufts)):
+                if lay.tufts[t].den_kind == L.DEN_DISTAL:
+                    steps.append(("tft_dst_mtp", (li, t, int(rng.integers(-2**31, 2**31)))))
+            steps.append(("inhib_intra_column", (li,)))
+            for t in range(len(lay.tufts)):
+                if lay.tufts[t].den_kind == L.DEN_DISTAL:
+                    steps.append(("tft_dst_mtp", (li, t, int(rng.integers(-2**31, 2**31)))))
+        for name, args in steps:
+            getattr(g, "k_" + name)(*args)
+            o.k(name, *args)
+            compare(g, o, scheme, f"{which} after {name}{args}")
+    g.close()
+
+
+def run_cycles(g, o, scheme, frames, n, rng_seed=0, every=1, tag=""):
+    rnd = g.draw_rnd(n)
+    for c in range(n):
+        if frames is not None:
+            g.intake(frames[c])
+            o.intake(frames[c])
+        r = rnd[c * g.n_slots:(c + 1) * g.n_slots]
+        g.cycle(1, r)
+        o.cycle(r, 1)
+        if (c + 1) % every == 0 or c == n - 1:
+            compare(g, o, scheme, f"{tag} cycle {c}")
+
+
+def test_cycle_config1_1000_cycles(ctx):
+    """Config 1: 64x64, 24x24 glyph stream (p=0.15, new glyph every 10 cycles),
+    learning on, 1000 cycles, every buffer compared after every cycle."""
+    scheme = recipes.config1_area()
+    g, o = make_pair(ctx, scheme, layout_seed=0)
+    frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=0).frames(1000)
+    run_cycles(g, o, scheme, frames, 1000, tag="c1")
+    # the run did learn and fire
+    assert np.count_nonzero(g.read("SYN_STRENGTHS", 0)) > 0
+    assert np.count_nonzero(g.read("SYN_STRENGTHS", 1)) > 0
+    assert np.count_nonzero(g.axon_states()[576:]) > 0
+    g.close()
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(tile_v=4, tile_u=4, halo=1, chunk=4),
+    dict(tile_v=16, tile_u=8, halo=8, chunk=8),
+    dict(tile_v=8, tile_u=16, halo=3, chunk=16),
+    dict(tile_v=3, tile_u=5, halo=2, chunk=1),
+])
+def test_cycle_tunings(ctx, tuning):
+    """Any tile/halo/chunk choice gives identical results (small halos force
+    the out-of-window global gather path)."""
+    print("tuning", tuning, flush=True)
+    scheme = recipes.cortical_area("t", 40, input_side=32, pyr_depth=3, ssc_dens=4, ssc_spd=16,
+                                   pyr_dens=4, pyr_spd=24, inhib_radius=4)
+    g, o = make_pair(ctx, scheme, reach=7)
+    g.set_tuning(**tuning)
+    rng = np.random.default_rng(3)
+    randomize(g, o, scheme, rng)
+    frames = inputs.SdrStream(32 * 32, 0.05, 0.5, seed=1).frames(40)
+    run_cycles(g, o, scheme, frames, 40, tag=str(tuning))
+    g.close()
+
+
+def test_cycle_scalar_quirk_nonspatial(ctx):
+    """syns_per_tft % 4 != 0 takes the scalar gather (char-wrapped mids) on a
+    non-spatial source slice with a large mid (cl/bismit.cl:331-332)."""
+    s = recipes.cortical_area("q", 24, pyr_depth=2, ssc_dens=3, ssc_spd=5, pyr_dens=3,
+                              pyr_spd=7, thresholds=(100, 0, 100))
+    s.slices[0] = recipes.SliceScheme.nonspatial(230, 230)
+    g, o = make_pair(ctx, s, reach=60)
+    rng = np.random.default_rng(8)
+    randomize(g, o, s, rng)
+    frames = (rng.random((30, 230 * 230)) < 0.1).astype(np.uint8) * 255
+    run_cycles(g, o, s, frames, 30, tag="quirk")
+    g.close()
+
+
+def test_cycle_settings(ctx):
+    """bypass_inhib (passthrough with rnd 0) and disable_learning."""
+    for st in (L.SET_BYPASS_INHIB, L.SET_DISABLE_LEARNING, L.SET_DISABLE_PYRS):
+        print("settings", st, flush=True)
+        s = recipes.cortical_area("x", 32, pyr_depth=2, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                                  pyr_spd=16, settings=st)
+        g, o = make_pair(ctx, s)
+        frames = inputs.SdrStream(32 * 32, 0.05, 0.5, seed=2).frames(25)
+        run_cycles(g, o, s, frames, 25, tag=f"settings {st}")
+        g.close()
+
+
+def test_cycle_config2_full_size(ctx):
+    """Config 2 at its full size (256x256, 142.6M synapses): 3 cycles of 2%
+    SDR input compared buffer for buffer with the (multithreaded) oracle."""
+    scheme = recipes.config2_area()
+    g, o = make_pair(ctx, scheme)
+    frames = inputs.SdrStream(256 * 256, 0.02, 0.5, seed=0).frames(3)
+    run_cycles(g, o, scheme, frames, 3, tag="c2")
+    g.close()
+    o.close()
+
+
+def test_buffer_io_roundtrip(ctx):
+    s = recipes.cortical_area("io", 16, ssc_spd=8, ssc_dens=2, pyr_dens=2, pyr_spd=8)
+    g = CorticalArea(s, ctx)
+    rng = np.random.default_rng(1)
+    for name in ("SYN_FLAG_SETS", "SYN_STATES", "SYN_STRENGTHS", "DEN_ACTIVITIES"):
+        n = g.buf_len(name, 1)
+        v = rng.integers(0, 256, n).astype(np.uint8)
+        if name in L.SIGNED_BUFS:
+            v = v.view(np.int8)
+        g.write(name, 1, v)
+        assert np.array_equal(g.read(name, 1), v)
+        part = g.read(name, 1, offset=5, length=17)
+        assert np.array_equal(part, v[5:22])
+    g.fill("SYN_FLAG_SETS", 1, 0x13)
+    assert np.all(g.read("SYN_FLAG_SETS", 1) == 0x13)
+    with pytest.raises(L.BsmError):
+        g.read("TFT_STATES", 0)   # SSC layers have no tufts
+    with pytest.raises(L.BsmError):
+        g.read("SYN_STATES", 1, offset=10**9, length=1)
+    g.close()

@@ -1,0 +1,430 @@
+"""Pins of the CPU oracle (oracle/bsm_oracle.c) on the CPU.
+
+The reference ships no golden vectors or known-answer tests for this path and
+its OpenCL/Rust build cannot run here, so each oracle function is pinned by
+what the mathematics fixes: published known answers of the textbook generators
+it builds on, exact counting identities, closed forms of hand-built inputs,
+brute force against independent formulations on tiny grids, and the invariants
+the reference's own tests assert (src/tests/dens_tfts.rs:196-363,
+src/tests/kernels.rs:21-91).
+"""
+import numpy as np
+import pytest
+
+from bismit_amd import _lib as L
+from bismit_amd import recipes
+from oracle import pyoracle as P
+
+M32 = 0xFFFFFFFF
+
+
+def s32(x):
+    x &= M32
+    return x - (1 << 32) if x & 0x80000000 else x
+
+
+def xorshift32(y):
+    """Marsaglia (2003) xorshift32, triple (13, 17, 5), logical shifts."""
+    y ^= (y << 13) & M32
+    y ^= y >> 17
+    y ^= (y << 5) & M32
+    return y
+
+
+def inv_lshift13(x):
+    """Inverse of x -> x ^ (x << 13) on 32 bits."""
+    return (x ^ (x << 13) ^ (x << 26)) & M32
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return P.load()
+
+
+# ---------------------------------------------------------------- rnd_mix
+
+
+def test_xorshift32_known_answer():
+    # Marsaglia, "Xorshift RNGs", J. Stat. Softw. 8(14), 2003: y=2463534242 -> 723471715
+    assert xorshift32(2463534242) == 723471715
+
+
+def test_rnd_mix_is_xorshift32_with_arithmetic_shift(lib):
+    """rnd_mix(0, s) equals the textbook xorshift32 step except that the >> 17 is
+    arithmetic: when s ^ (s << 13) is negative the result differs by exactly
+    0x000F8000 (sign bits 15..31 from the shift, folded by the << 5)."""
+    rng = np.random.default_rng(1)
+    seeds = list(rng.integers(0, 1 << 32, 4000, dtype=np.uint64)) + [0, 1, 0x7FFFFFFF, 0x80000000,
+                                                                       0xFFFFFFFF, 2463534242]
+    n_neg = 0
+    for s in seeds:
+        s = int(s)
+        x1 = (s ^ (s << 13)) & M32
+        want = xorshift32(s)
+        if x1 & 0x80000000:
+            want ^= 0x000F8000
+            n_neg += 1
+        assert lib.orc_rnd_mix(0, s32(s)) & M32 == want, hex(s)
+    assert 100 < n_neg < len(seeds) - 100
+
+
+def test_rnd_mix_rnd_enters_before_the_first_shift(lib):
+    """(seed ^ rnd) << 13: rnd_mix(rnd, s) == rnd_mix(0, s') where s' solves
+    s' ^ (s' << 13) == s ^ (s << 13) ^ (rnd << 13)."""
+    rng = np.random.default_rng(2)
+    for _ in range(3000):
+        s, r = (int(v) for v in rng.integers(0, 1 << 32, 2, dtype=np.uint64))
+        x1 = (s ^ (s << 13) ^ (r << 13)) & M32
+        s2 = inv_lshift13(x1)
+        assert ((s2 ^ (s2 << 13)) & M32) == x1
+        assert lib.orc_rnd_mix(s32(r), s32(s)) == lib.orc_rnd_mix(0, s32(s2))
+
+
+def test_lshft_masks(lib):
+    # learning-rate masks of the three LTP paths (cl/bismit.cl:572-577, 616-623)
+    assert lib.orc_lshft_mask(0) == 0x7F
+    assert lib.orc_lshft_mask(2) == 0x1FF
+    assert lib.orc_lshft_mask(5) == 0xFFF
+
+
+@pytest.mark.parametrize("mask_l2i", [0, 2, 5])
+def test_inc_dec_exact_counts(lib, mask_l2i):
+    """Over an aligned block of 2^bits seeds the low `bits` of rnd_mix take every
+    value once (seed bits 0..12 enter the low bits bijectively), so the number
+    of increments is exactly 127-|v| and of decrements 127-|v|+(v==127)."""
+    mask = lib.orc_lshft_mask(mask_l2i)
+    bits = 7 + mask_l2i
+    rng = np.random.default_rng(3 + mask_l2i)
+    for v in [-128, -127, -100, -1, 0, 1, 5, 64, 126, 127]:
+        for _ in range(2):
+            r = s32(int(rng.integers(0, 1 << 32)))
+            base = int(rng.integers(0, 1 << (32 - bits))) << bits
+            inc = sum(lib.orc_rnd_inc(r, s32(base + k), v, mask) for k in range(1 << bits))
+            dec = sum(lib.orc_rnd_dec(r, s32(base + k), v, mask) for k in range(1 << bits))
+            av = abs(v)
+            assert inc == max(0, 127 - av), (v, mask)
+            assert dec == max(0, 127 - av + (v == 127)), (v, mask)
+
+
+def test_activity_rating_composition(lib):
+    """update_activity_rating: +1 when (mix(rnd, seed) & 0xFF) > a and active,
+    then -1 when (mix(rnd, seed << 1) & 0xFFFF) < decay and the *updated* a > 0."""
+    rng = np.random.default_rng(4)
+    for _ in range(4000):
+        a = int(rng.integers(0, 256))
+        act = int(rng.integers(0, 2))
+        r, s = (s32(int(v)) for v in rng.integers(0, 1 << 32, 2))
+        decay = int(rng.choice([768, 1536, 40000]))
+        a1 = a + (((lib.orc_rnd_mix(r, s) & 0xFF) > a) and act)
+        s2 = s32((s << 1) & M32)
+        a2 = a1 - (((lib.orc_rnd_mix(r, s2) & 0xFFFF) < decay) and a1 > 0)
+        assert lib.orc_update_activity_rating(a, act, r, s, decay) == a2
+
+
+def test_activity_rating_bounds_and_rates(lib):
+    # rating never wraps: 0 stays >= 0, 255 cannot increment
+    for s in range(2000):
+        assert lib.orc_update_activity_rating(0, 0, 12345, s, 65535) == 0
+        assert lib.orc_update_activity_rating(255, 1, 777, s, 0) == 255
+    # the decay draw fires at rate decay/65536 (16-bit mask): 2^16 trials at
+    # 768/65536 expect 768 decrements, sigma ~28
+    dec = sum(1 - lib.orc_update_activity_rating(1, 0, 99, s, 768) for s in range(1 << 16))
+    assert 768 - 150 < dec < 768 + 150
+
+
+def test_syn_fire_closed_form(lib):
+    vals = [lib.orc_syn_fire(a) for a in range(256)]
+    assert vals[0] == 0 and vals[1] == 128 and vals[2] == 129 and vals[255] == 255
+    assert all(128 <= v <= 255 for v in vals[1:])
+    assert all(vals[i] <= vals[i + 1] for i in range(255))
+
+
+# ---------------------------------------------------------------- kernels
+
+
+def tiny_area(side=16, input_side=None, pyr_depth=2, ssc_spd=8, pyr_spd=8, dens=2, radius=2,
+              thresholds=(0, 0, 0), **kw):
+    return recipes.cortical_area("t", side, input_side=input_side, pyr_depth=pyr_depth,
+                                 ssc_dens=dens, ssc_spd=ssc_spd, pyr_dens=dens, pyr_spd=pyr_spd,
+                                 inhib_radius=radius, thresholds=thresholds, **kw)
+
+
+def test_gather_all_synapses_on_one_axon():
+    """Reference invariant (src/tests/dens_tfts.rs:196-363): all synapses of a
+    cell-tuft sourced from one active axon -> every dendrite of it is active."""
+    sch = tiny_area(thresholds=(0, 0, 0))
+    o = P.OracleArea(sch)
+    rng = np.random.default_rng(5)
+    idz, _ = sch.slice_idzs()
+    for trial in range(40):
+        o.buf("AXN_STATES")[:] = 0
+        li = int(rng.integers(0, 2))
+        lay = sch.layers[li]
+        t = int(rng.integers(0, len(lay.tufts)))
+        ts = lay.tufts[t]
+        src = ts.src_slcs[0]
+        v, u = (int(x) for x in rng.integers(0, 16, 2))
+        slc_l = int(rng.integers(0, lay.depth))
+        vo, uo = (int(x) for x in rng.integers(-3, 4, 2))
+        tv, tu = v + vo, u + uo
+        if not (0 <= tv < 16 and 0 <= tu < 16):
+            continue
+        o.buf("AXN_STATES")[idz[src] + tv * 16 + tu] = 200
+        cells = sch.cells(li)
+        tz = sum(cells * x.syns_per_tft for x in lay.tufts[:t])
+        cel = slc_l * 256 + v * 16 + u
+        z = tz + cel * ts.syns_per_tft
+        for name, val in (("SYN_SRC_SLC_IDS", src), ("SYN_SRC_COL_V_OFFS", vo),
+                          ("SYN_SRC_COL_U_OFFS", uo)):
+            o.buf(name, li)[z:z + ts.syns_per_tft] = np.array(val).astype(o.buf(name, li).dtype)
+        o.k("syn_cycle", li, t)
+        o.k("den_cycle", li, t, 0)
+        st = o.buf("SYN_STATES", li)[z:z + ts.syns_per_tft]
+        assert np.all(st == 128 + 100)
+        dz = sum(cells * x.dens_per_tft for x in lay.tufts[:t]) + cel * ts.dens_per_tft
+        dens = o.buf("DEN_STATES", li)[dz:dz + ts.dens_per_tft]
+        # 8 synapses x 228 = 1824 >> 4 = 114 per dendrite
+        assert np.all(dens == min(255, (ts.syns_per_den * 228) >> 4)), dens
+
+
+def test_gather_brute_force_geometry():
+    """Every synapse state equals the textbook hex-grid lookup: scaled cell
+    position plus offset, 0 off the slice, on a slice mapped from a smaller
+    source area (scale calc_scale(24, 64) = 6) and on local slices."""
+    sch = recipes.cortical_area("g", 64, input_side=24, ssc_dens=1, ssc_spd=4, pyr_dens=1,
+                                pyr_spd=4, pyr_depth=1, distal=False)
+    assert sch.slices[0].v_scale == 6 and sch.slices[1].v_scale == 16
+    o = P.OracleArea(sch)
+    rng = np.random.default_rng(6)
+    axn = o.buf("AXN_STATES")
+    axn[:] = rng.integers(0, 256, axn.size).astype(np.uint8)
+    idz, _ = sch.slice_idzs()
+    for li in range(2):
+        n = o.buf("SYN_STATES", li).size
+        srcs = sch.layers[li].tufts[0].src_slcs + [2]
+        o.buf("SYN_SRC_SLC_IDS", li)[:] = rng.choice(srcs, n)
+        o.buf("SYN_SRC_COL_V_OFFS", li)[:] = rng.integers(-30, 31, n)
+        o.buf("SYN_SRC_COL_U_OFFS", li)[:] = rng.integers(-30, 31, n)
+        o.k("syn_cycle", li, 0)
+        sl = o.buf("SYN_SRC_SLC_IDS", li)
+        vo = o.buf("SYN_SRC_COL_V_OFFS", li)
+        uo = o.buf("SYN_SRC_COL_U_OFFS", li)
+        st = o.buf("SYN_STATES", li)
+        spt = 4
+        for i in rng.choice(n, 3000, replace=False):
+            cel = i // spt
+            v, u = (cel % 4096) // 64, (cel % 4096) % 64
+            s = sch.slices[int(sl[i])]
+            # independent formulation: nearest source row/col by integer scaling
+            sv = v * s.v_scale // 16 + int(vo[i])
+            su = u * s.u_scale // 16 + int(uo[i])
+            a = 0
+            if 0 <= sv < s.v_size and 0 <= su < s.u_size:
+                a = int(axn[idz[int(sl[i])] + sv * s.u_size + su])
+            want = 0 if a == 0 else 128 + a // 2
+            assert st[i] == want, (li, i)
+
+
+def test_den_cycle_closed_forms():
+    sch = tiny_area(ssc_spd=32, dens=4, thresholds=(5000, 0, 0))
+    o = P.OracleArea(sch)
+    st = o.buf("SYN_STATES", 0)
+    sw = o.buf("SYN_STRENGTHS", 0)
+    st[:] = 128
+    sw[:] = 0
+    sw[32:64:2] = -1  # dendrite 1: half its synapses have negative strength
+    st[64:96] = 0     # dendrite 2: silent
+    o.k("den_cycle", 0, 0, 0)
+    raw = o.buf("DEN_STATES_RAW", 0)
+    ds = o.buf("DEN_STATES", 0)
+    # 32 x 128 = 4096 -> raw 256 clamps to 255; the sum never exceeds 5000 -> state 0
+    assert raw[0] == 255 and ds[0] == 0
+    assert raw[1] == 255 and ds[1] == 0
+    assert raw[2] == 0 and ds[2] == 0
+    sch2 = tiny_area(ssc_spd=32, dens=4, thresholds=(1000, 0, 0))
+    o2 = P.OracleArea(sch2)
+    o2.buf("SYN_STATES", 0)[:] = 128
+    o2.buf("SYN_STRENGTHS", 0)[32:64:2] = -1
+    o2.k("den_cycle", 0, 0, 0)
+    ds2 = o2.buf("DEN_STATES", 0)
+    assert ds2[0] == 255           # 4096 > 1000 -> min(256, 255)
+    assert ds2[1] == 2048 >> 4     # only the 16 non-negative synapses count
+
+
+def test_tft_cycle_first_max_and_cap():
+    sch = tiny_area(dens=4, pyr_spd=4)
+    sch.layers[1].tufts[0].max_active_dens_l2 = 0
+    o = P.OracleArea(sch)
+    raw = o.buf("DEN_STATES_RAW", 1)
+    ds = o.buf("DEN_STATES", 1)
+    raw[0:4] = [5, 9, 9, 3]
+    ds[0:4] = [1, 0, 7, 2]
+    raw[4:8] = [0, 0, 0, 0]
+    ds[4:8] = [100, 100, 50, 10]   # sum 260 >= 256 -> 255
+    o.k("tft_cycle", 1, 0)
+    ids = o.buf("TFT_BEST_DEN_IDS", 1)
+    braw = o.buf("TFT_BEST_DEN_STATES_RAW", 1)
+    best = o.buf("TFT_BEST_DEN_STATES", 1)
+    tst = o.buf("TFT_STATES", 1)
+    assert ids[0] == 1 and braw[0] == 9 and best[0] == 7 and tst[0] == 10
+    assert ids[1] == 0 and braw[1] == 0 and best[1] == 100 and tst[1] == 255
+    o.k("tft_cycle", 1, 0)  # previous values shift into prev_*
+    assert o.buf("TFT_PREV_BEST_DEN_IDS", 1)[0] == 1
+    assert o.buf("TFT_PREV_STATES", 1)[1] == 255
+
+
+def hex_dist(dv, du):
+    return (abs(dv) + abs(du) + abs(dv + du)) // 2
+
+
+@pytest.mark.parametrize("radius", [1, 2, 4])
+def test_inhib_simple_brute_force(radius):
+    """Winner-take-all against an independent formulation: for each cell, every
+    grid cell within hex distance r (cube-coordinate distance, all grid cells
+    enumerated) must satisfy s*H^2 >= n*(H^2 - (d-1)^2), H = r + 3."""
+    side = 12
+    sch = tiny_area(side=side, radius=radius)
+    o = P.OracleArea(sch)
+    rng = np.random.default_rng(10 + radius)
+    soma = o.buf("SOMA_STATES", 0)
+    soma[:side * side] = rng.choice([0, 0, 0, 10, 50, 120, 200, 255], side * side)
+    o.k("inhib_simple", 0, 0)
+    idz, _ = sch.slice_idzs()
+    axn = o.buf("AXN_STATES")[idz[1]:idz[1] + side * side]
+    H2 = (radius + 3) ** 2
+    n_offsets = sum(1 for dv in range(-radius, radius + 1) for du in range(-radius, radius + 1)
+                    if hex_dist(dv, du) <= radius)
+    assert n_offsets == 3 * radius * radius + 3 * radius + 1
+    s = soma[:side * side].reshape(side, side).astype(int)
+    for v in range(side):
+        for u in range(side):
+            ok = True
+            for nv in range(side):
+                for nu in range(side):
+                    d = hex_dist(nv - v, nu - u)
+                    if d <= radius and s[v, u] * H2 < s[nv, nu] * (H2 - (d - 1) ** 2):
+                        ok = False
+            assert axn[v * side + u] == (s[v, u] if ok else 0), (v, u)
+
+
+def test_intra_column_reset_quirk():
+    sch = tiny_area(pyr_depth=3)
+    o = P.OracleArea(sch)
+    soma = o.buf("SOMA_STATES", 1)
+    cols = 256
+    cases = [([5, 3, 7], [0, 0, 7]), ([7, 3, 5], [7, 0, 5]), ([4, 4, 4], [4, 4, 4]),
+             ([9, 2, 1], [9, 2, 1]), ([2, 9, 1], [2, 9, 1]), ([9, 2, 0], [9, 2, 0]),
+             ([1, 5, 5], [0, 5, 5]), ([0, 0, 0], [0, 0, 0])]
+    for c, (states, _) in enumerate(cases):
+        for s in range(3):
+            soma[s * cols + c] = states[s]
+    o.k("inhib_intra_column", 1)
+    idz, _ = sch.slice_idzs()
+    axn = o.buf("AXN_STATES")
+    base = idz[sch.layers[1].base_slc]
+    for c, (_, want) in enumerate(cases):
+        got = [int(axn[base + s * cols + c]) for s in range(3)]
+        assert got == want, (c, got)
+
+
+def test_pyr_cycle_closed_forms():
+    sch = tiny_area()
+    o = P.OracleArea(sch)
+    C_ = sch.cells(1)
+    tst = o.buf("TFT_STATES", 1)
+    braw = o.buf("TFT_BEST_DEN_STATES_RAW", 1)
+    cases = [(0, 200, 0), (3, 0, 1), (3, 100, 51), (200, 100, 100), (255, 255, 63 + 127),
+             (1, 1, 1), (4, 9, 1 + 4)]
+    for c, (p, d, _) in enumerate(cases):
+        tst[0 * C_ + c] = p          # proximal tuft 0: tuft state
+        braw[1 * C_ + c] = d         # distal tuft 1: best raw dendrite state
+    o.k("pyr_cycle", 1)
+    soma = o.buf("SOMA_STATES", 1)
+    for c, (_, _, want) in enumerate(cases):
+        assert soma[c] == want, c
+
+
+def test_ssc_mtp_seeds_are_twice_the_cell_base(lib):
+    """ssc_mtp_simple seeds synapse k of cell c with syn_idz + i = 2*c*spt + k."""
+    sch = tiny_area(ssc_spd=8, dens=2)
+    o = P.OracleArea(sch)
+    spt = 16
+    idz, _ = sch.slice_idzs()
+    axn = o.buf("AXN_STATES")
+    active = [3, 77, 200]
+    for c in active:
+        axn[idz[1] + c] = 9
+    st = o.buf("SYN_STATES", 0)
+    st[:] = 0
+    for c in active:
+        st[c * spt:c * spt + 8] = 200  # first half active, second half inactive
+    rnd = 0x1234567
+    o.k("ssc_mtp", 0, rnd)
+    sw = o.buf("SYN_STRENGTHS", 0)
+    for c in range(256):
+        for k in range(spt):
+            i = c * spt + k
+            if c not in active:
+                assert sw[i] == 0
+                continue
+            seed = 2 * c * spt + k
+            mix = lib.orc_rnd_mix(rnd, seed)
+            if k < 8:
+                assert sw[i] == ((mix & 0x7F) > 0)
+            else:
+                assert sw[i] == -((mix & 0x1FF) > 0x180)
+
+
+def test_tft_dst_mtp_newly_active_and_shared_flags(lib):
+    """Learning on the previous best dendrite of newly active cells only; the
+    cell flag is shared by all distal tufts so a second distal tuft never
+    learns (src/cortex/tufts.rs:205)."""
+    sch = tiny_area(pyr_depth=1, dens=2, pyr_spd=4)
+    sch.layers[1].tufts.append(recipes.TuftScheme.apical_distal(2, 4, 0, [2]))
+    o = P.OracleArea(sch)
+    C_ = sch.cells(1)
+    idz, _ = sch.slice_idzs()
+    axn = o.buf("AXN_STATES")
+    base = idz[sch.layers[1].base_slc]
+    flags = o.buf("SOMA_FLAG_SETS", 1)
+    pbid = o.buf("TFT_PREV_BEST_DEN_IDS", 1)
+    pbraw = o.buf("TFT_PREV_BEST_DEN_STATES_RAW", 1)
+    sflags = o.buf("SYN_FLAG_SETS", 1)
+    # cell 5: newly active, tuft prev active, best den 1; cell 6: was active
+    axn[base + 5] = 40
+    axn[base + 6] = 40
+    flags[6] = 0x08
+    for t in (1, 2):
+        pbid[t * C_ + 5] = 1
+        pbraw[t * C_ + 5] = 3
+        pbraw[t * C_ + 6] = 3
+    spt = 8
+    tz1 = C_ * spt
+    z = tz1 + 5 * spt + 1 * 4
+    sflags[z:z + 2] = 0x10
+    rnd = -99
+    o.k("tft_dst_mtp", 1, 1, rnd)
+    o.k("tft_dst_mtp", 1, 2, rnd)
+    sw = o.buf("SYN_STRENGTHS", 1)
+    changed = np.flatnonzero(sw)
+    assert set(changed.tolist()) <= set(range(z, z + 4))
+    for k in range(4):
+        i = z + k
+        mix = lib.orc_rnd_mix(rnd, z + i)
+        want = ((mix & 0x7F) > 0) if k < 2 else -((mix & 0xFFF) > 0xF80)
+        assert sw[i] == want
+    assert flags[5] == 0x08 and flags[6] == 0x08 and flags[7] == 0
+
+
+def test_ssc_cycle_energy_rules():
+    sch = tiny_area()
+    o = P.OracleArea(sch)
+    e = o.buf("SOMA_ENERGIES", 0)
+    s = o.buf("SOMA_STATES", 0)
+    cases = [(0, 10, 0, 10), (192, 10, 128, 255), (192, 0, 192, 0), (255, 0, 191, 255),
+             (255, 7, 191, 255), (191, 50, 191, 50)]
+    for c, (en, st, _, _) in enumerate(cases):
+        e[c], s[c] = en, st
+    o.k("ssc_cycle", 0)
+    for c, (_, _, en2, st2) in enumerate(cases):
+        assert (e[c], s[c]) == (en2, st2), c
