@@ -22,6 +22,7 @@ DEN_PROXIMAL, DEN_DISTAL = 0, 1
 CTRL_NONE, CTRL_INHIB_SIMPLE, CTRL_INTRA_COLUMN, CTRL_PYR_OUTPUT = 0, 1, 2, 3
 SET_BYPASS_INHIB, SET_DISABLE_PYRS, SET_DISABLE_SSCS, SET_DISABLE_LEARNING = 1, 2, 4, 8
 RND_DEN, RND_CTRL, RND_MTP = 0, 1, 2
+BATCH_DYNAMIC = 1
 
 # buffers, mirroring SamplerKind (src/cortex/cortical_area.rs:61-90)
 BUF_NAMES = [
@@ -59,6 +60,11 @@ class AreaDesc(C.Structure):
     _fields_ = [("v_size", C.c_uint32), ("u_size", C.c_uint32), ("n_slices", C.c_uint32),
                 ("n_layers", C.c_uint32), ("settings", C.c_uint32), ("_pad", C.c_uint32),
                 ("slices", C.POINTER(SliceDesc)), ("layers", C.POINTER(LayerDesc))]
+
+
+class Tract(C.Structure):
+    _fields_ = [("src_area", C.c_uint32), ("src_offset", C.c_uint32), ("dst_area", C.c_uint32),
+                ("dst_offset", C.c_uint32), ("len", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -126,6 +132,13 @@ SIGNATURES = [
     ("bsm_splitmix64", C.c_uint64, [C.POINTER(C.c_uint64)]),
     ("bsm_xorshift128_fill", C.c_int, [C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64]),
     ("bsm_batch_create", C.c_int, [_P, C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),
+    ("bsm_batch_create_ex", C.c_int, [_P, C.POINTER(_P), C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    ("bsm_batch_set_input_ranges", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("bsm_batch_cycle_masked", C.c_int, [_P, _U8P, _P, C.POINTER(C.c_uint64), C.c_uint64,
+                                         C.c_int]),
+    ("bsm_batch_query", C.c_int, [_P, C.POINTER(C.c_int)]),
+    ("bsm_batch_set_tracts", C.c_int, [_P, C.POINTER(Tract), C.c_uint32]),
+    ("bsm_area_copy_axons", C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint32]),
     ("bsm_batch_destroy", C.c_int, [_P]),
     ("bsm_batch_cycle", C.c_int, [_P, C.c_uint32, C.c_int]),
     ("bsm_batch_set_inputs", C.c_int, [_P, C.POINTER(_P), C.POINTER(C.c_uint64),

@@ -251,16 +251,39 @@ class CorticalArea:
 
 
 class Batch:
-    """Batched form: many areas stepped by one persistent launch per stage."""
+    """Batched form: many areas stepped by one persistent launch per stage.
+    dynamic=True: each cycle steps only the areas of an active mask (serving)."""
 
-    def __init__(self, areas, ctx: Context = None):
+    def __init__(self, areas, ctx: Context = None, dynamic=False):
         self.ctx = ctx or areas[0].ctx
         self.lib = self.ctx.lib
         self.areas = list(areas)
+        self.dynamic = dynamic
         arr = (C.c_void_p * len(areas))(*[a.h.value for a in areas])
         self.h = C.c_void_p()
-        L.check(self.lib.bsm_batch_create(self.ctx.h, arr, len(areas), C.byref(self.h)))
-        self._frames = None
+        L.check(self.lib.bsm_batch_create_ex(self.ctx.h, arr, len(areas),
+                                             L.BATCH_DYNAMIC if dynamic else 0, C.byref(self.h)))
+        if dynamic:
+            n = len(areas)
+            rng = [a.scheme.input_range() for a in areas]
+            of = (C.c_uint32 * n)(*[r[0] for r in rng])
+            ln = (C.c_uint32 * n)(*[r[1] for r in rng])
+            L.check(self.lib.bsm_batch_set_input_ranges(self.h, of, ln))
+
+    def cycle_masked(self, active, frames, frame_ofs, graph=True):
+        """One shared cycle of the areas with active[i] != 0; frames is one packed
+        uint8 array, area i's frame at frame_ofs[i] (-1: none)."""
+        active = np.ascontiguousarray(active, dtype=np.uint8)
+        frames = np.ascontiguousarray(frames, dtype=np.uint8)
+        fo = np.ascontiguousarray(frame_ofs, dtype=np.int64).view(np.uint64)
+        L.check(self.lib.bsm_batch_cycle_masked(
+            self.h, L.u8ptr(active), L.vptr(frames) if frames.size else None,
+            fo.ctypes.data_as(C.POINTER(C.c_uint64)), frames.size, 1 if graph else 0))
+
+    def query(self):
+        d = C.c_int()
+        L.check(self.lib.bsm_batch_query(self.h, C.byref(d)))
+        return bool(d.value)
 
     def set_inputs(self, dev_frames, strides, lens, offsets, n_frames):
         n = len(self.areas)

@@ -192,6 +192,38 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
     a->n_slots = slot;
 }
 
+// Device form of bsm_gen_tuft_layout (same hash, same fields): one thread per synapse.
+struct LayoutSrc {
+    uint8_t ids[BSM_MAX_TUFT_SRCS];
+};
+__global__ void k_layout(uint64_t key, uint64_t n, uint32_t n_src, LayoutSrc src, int32_t reach,
+                         uint8_t* __restrict__ slc, int8_t* __restrict__ v, int8_t* __restrict__ u) {
+    const uint32_t span = 2u * (uint32_t)reach + 1u;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t z = key + i * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        const uint64_t h = z ^ (z >> 31);
+        slc[i] = src.ids[(uint32_t)(h & 0xFFFF) % n_src];
+        v[i] = (int8_t)((int32_t)((uint32_t)((h >> 16) & 0xFFFF) % span) - reach);
+        u[i] = (int8_t)((int32_t)((uint32_t)((h >> 32) & 0xFFFF) % span) - reach);
+    }
+}
+
+int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n,
+                  uint32_t n_src, const uint8_t* src_ids, int32_t reach, uint8_t* slc, int8_t* v,
+                  int8_t* u) {
+    if (!n) return 0;
+    LayoutSrc src{};
+    for (uint32_t k = 0; k < n_src && k < BSM_MAX_TUFT_SRCS; k++) src.ids[k] = src_ids[k];
+    const uint64_t key = seed ^ ((uint64_t)(layer * 8u + tuft) << 56);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+    k_layout<<<grid, 256, 0, s>>>(key, n, n_src, src, reach, slc, v, u);
+    BSM_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace bsm
 
 using namespace bsm;
@@ -342,6 +374,7 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
 
 int bsm_area_destroy(bsm_area* a) {
     if (!a) return 0;
+    if (a->batch_stream) return fail("area belongs to a batch; destroy the batch first");
     (void)hipSetDevice(a->ctx->device);
     (void)hipStreamSynchronize(a->stream);
     (void)hipFree(a->arena);
@@ -631,25 +664,39 @@ int bsm_gen_tuft_layout(uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n
 
 int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
     if (!a) return fail("null area");
+    if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
     BSM_HIP(hipSetDevice(a->ctx->device));
-    BSM_HIP(hipStreamSynchronize(a->stream));
     for (uint32_t l = 0; l < a->n_layers; l++) {
         LayerRt& L = a->L[l];
         for (uint32_t t = 0; t < L.d.n_tufts; t++) {
             TuftRt& T = L.t[t];
+            if (T.d.n_src_slcs == 0) return fail("tuft has no source slices");
             uint64_t n = (uint64_t)L.cells * T.spt;
-            std::vector<uint8_t> s(n);
-            std::vector<int8_t> v(n), u(n);
-            if (bsm_gen_tuft_layout(seed, l, t, n, T.d.n_src_slcs, T.d.src_slc_ids, reach,
-                                    s.data(), v.data(), u.data()))
+            if (launch_layout(a->stream, seed, l, t, n, T.d.n_src_slcs, T.d.src_slc_ids, reach,
+                              L.syn_slc + T.syn_idz, L.syn_v + T.syn_idz, L.syn_u + T.syn_idz))
                 return 1;
-            BSM_HIP(hipMemcpy(L.syn_slc + T.syn_idz, s.data(), n, hipMemcpyHostToDevice));
-            BSM_HIP(hipMemcpy(L.syn_v + T.syn_idz, v.data(), n, hipMemcpyHostToDevice));
-            BSM_HIP(hipMemcpy(L.syn_u + T.syn_idz, u.data(), n, hipMemcpyHostToDevice));
         }
         BSM_HIP(hipMemsetAsync(L.syn_str, 0, L.n_syn, a->stream));
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
+    return 0;
+}
+
+int bsm_area_copy_axons(bsm_area* dst, uint32_t dst_offset, bsm_area* src, uint32_t src_offset,
+                        uint32_t len) {
+    if (!dst || !src) return fail("null area");
+    if (dst->ctx != src->ctx) return fail("areas on different contexts");
+    if (dst->batch_stream || src->batch_stream) return fail("area belongs to a batch");
+    if (src_offset > src->axn_len || len > src->axn_len - src_offset || dst_offset > dst->axn_len ||
+        len > dst->axn_len - dst_offset)
+        return fail("copy range out of the axon space");
+    if (!len) return 0;
+    BSM_HIP(hipSetDevice(dst->ctx->device));
+    if (src != dst) BSM_HIP(hipStreamSynchronize(src->stream));
+    BSM_HIP(hipMemcpyAsync(dst->axn + dst_offset, src->axn + src_offset, len,
+                           hipMemcpyDeviceToDevice, dst->stream));
+    // complete on return, so later work on src cannot overwrite the range first
+    BSM_HIP(hipStreamSynchronize(dst->stream));
     return 0;
 }
 

@@ -27,7 +27,11 @@ struct AreaIo {
     uint64_t stride;
     uint8_t* dst;           // axon space + offset
     uint32_t n_frames;
-    uint32_t _pad;
+    uint32_t layer_mask;    // dynamic batches: layers whose parity flips when the area cycles
+    uint32_t* dyn;          // dynamic batches: per-layer words (bit0 parity, bit1 active)
+    const uint8_t* act;     // dynamic batches: this cycle's active flag of the area
+    const uint64_t* frame_ofs;  // dynamic batches: frame offset in the staging buffer (~0: none)
+    const uint8_t* stage;   // dynamic batches: staged frames of this cycle
 };
 
 // One block per area: thread 0 draws the rnd slots (xorshift128, Marsaglia
@@ -36,6 +40,16 @@ __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io
                                                    const uint32_t* __restrict__ counter) {
     for (uint32_t a = blockIdx.x; a < n_areas; a += gridDim.x) {
         const AreaIo& A = io[a];
+        if (A.dyn) {
+            // dynamic batch: only areas with a frame this cycle step; their layers
+            // flip parity and are marked active, the others are marked idle
+            const uint32_t active = *A.act;
+            if (threadIdx.x < BSM_MAX_LAYERS && ((A.layer_mask >> threadIdx.x) & 1u)) {
+                const uint32_t d = A.dyn[threadIdx.x];
+                A.dyn[threadIdx.x] = active ? (((d & 1u) ^ 1u) | 2u) : (d & 1u);
+            }
+            if (!active) continue;
+        }
         if (threadIdx.x == 0) {
             uint32_t x = A.rng[0], y = A.rng[1], z = A.rng[2], w = A.rng[3];
             for (uint32_t k = 0; k < A.n_slots; k++) {
@@ -51,7 +65,13 @@ __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io
             A.rng[2] = z;
             A.rng[3] = w;
         }
-        if (A.len && A.n_frames) {
+        if (A.dyn) {
+            const uint64_t fo = *A.frame_ofs;
+            if (A.len && fo != ~0ull) {
+                const uint8_t* src = A.stage + fo;
+                for (uint32_t i = threadIdx.x; i < A.len; i += blockDim.x) A.dst[i] = src[i];
+            }
+        } else if (A.len && A.n_frames) {
             const uint8_t* src = A.frames + (size_t)(*counter % A.n_frames) * A.stride;
             for (uint32_t i = threadIdx.x; i < A.len; i += blockDim.x) A.dst[i] = src[i];
         }
@@ -59,6 +79,21 @@ __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io
 }
 
 __global__ void kb_tick(uint32_t* counter) { *counter += 1; }
+
+struct TractRt {
+    const uint8_t* src;
+    uint8_t* dst;
+    uint32_t len;
+    uint32_t _pad;
+};
+
+// One block per tract: copy the source area's axon range into the target's.
+__global__ __launch_bounds__(256) void kb_tracts(const TractRt* __restrict__ tr, uint32_t n) {
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
+        const TractRt T = tr[t];
+        for (uint32_t i = threadIdx.x; i < T.len; i += blockDim.x) T.dst[i] = T.src[i];
+    }
+}
 
 }  // namespace bsm
 
@@ -84,6 +119,17 @@ struct bsm_batch {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     bool dirty = true;
     hipEvent_t ev0, ev1;
+    // dynamic mode (serving): per-cycle active mask and staged frames
+    bool dynamic = false;
+    uint32_t* dyn_d = nullptr;      // BSM_MAX_LAYERS words per area
+    uint8_t* xfer_d = nullptr;      // [act n | pad][frame_ofs n x u64][frames]
+    size_t xfer_cap = 0;
+    uint8_t* xfer_h[2] = {nullptr, nullptr};
+    size_t xfer_h_cap[2] = {0, 0};
+    hipEvent_t xfer_ev[2];
+    int xfer_next = 0;
+    bsm::TractRt* tracts_d = nullptr;
+    uint32_t n_tracts = 0;
 };
 
 namespace bsm {
@@ -132,9 +178,11 @@ static int batch_plan(bsm_batch* b) {
                     if ((spatial && !sscs) || (!spatial && !pyrs)) continue;
                     // first batch cycle runs with par = 1 and must write buffer cur ^ 1
                     const uint32_t px = (uint32_t)L.cur;
+                    const uint32_t* dw = b->dynamic ? b->dyn_d + ai * BSM_MAX_LAYERS + l : nullptr;
                     for (uint32_t t = 0; t < L.d.n_tufts; t++) {
                         TuftJob j = make_tuft_job(a, l, t, b->tune, a->rnd_d);
                         j.par_xor = px;
+                        j.dyn = dw;
                         t_ids.push_back((uint32_t)b->tjobs.size());
                         t_w.push_back(tuft_chunk(a, l, t, b->tune.chunk));
                         b->tjobs.push_back(j);
@@ -142,12 +190,14 @@ static int batch_plan(bsm_batch* b) {
                     if (spatial) {
                         SscJob j = make_ssc_job(a, l, a->rnd_d);
                         j.par_xor = px;
+                        j.dyn = dw;
                         s_ids.push_back((uint32_t)b->sjobs.size());
                         b->sjobs.push_back(j);
                     } else {
                         uint32_t w;
                         PyrJob j = make_pyr_job(a, l, a->rnd_d, &w);
                         j.par_xor = px;
+                        j.dyn = dw;
                         w3 = std::min(w3, w);
                         p_ids.push_back((uint32_t)b->pjobs.size());
                         b->pjobs.push_back(j);
@@ -196,6 +246,10 @@ static int batch_plan(bsm_batch* b) {
 
 static int enqueue_cycle(bsm_batch* b, uint32_t par) {
     const uint32_t na = (uint32_t)b->areas.size();
+    if (b->n_tracts) {
+        kb_tracts<<<b->n_tracts, 256, 0, b->stream>>>(b->tracts_d, b->n_tracts);
+        BSM_HIP(hipGetLastError());
+    }
     kb_prologue<<<na, 256, 0, b->stream>>>(b->io_d, na, b->counter_d);
     BSM_HIP(hipGetLastError());
     for (const Stage& st : b->stages)
@@ -236,6 +290,8 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
         }
     }
     // host bookkeeping: every batched layer's current buffer flipped n times
+    // (dynamic batches read the parity words back at sync instead)
+    if (b->dynamic) return 0;
     for (bsm_area* a : b->areas) {
         const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
         const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
@@ -248,6 +304,31 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
     return 0;
 }
 
+// Device transfer buffer of a dynamic batch: [act n | pad to 16][frame_ofs n][frames].
+static size_t xfer_hdr(size_t n) { return ((n + 15) & ~(size_t)15) + 8 * n; }
+
+static int grow_xfer(bsm_batch* b, size_t frame_bytes) {
+    const size_t n = b->areas.size();
+    const size_t need = xfer_hdr(n) + frame_bytes;
+    if (need <= b->xfer_cap) return 0;
+    const size_t cap = std::max(need, b->xfer_cap * 2);
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    if (b->xfer_d) BSM_HIP(hipFree(b->xfer_d));
+    b->xfer_d = nullptr;
+    BSM_HIP(hipMalloc((void**)&b->xfer_d, cap));
+    BSM_HIP(hipMemset(b->xfer_d, 0, cap));
+    b->xfer_cap = cap;
+    const size_t a0 = (n + 15) & ~(size_t)15;
+    for (size_t i = 0; i < n; i++) {
+        b->io[i].act = b->xfer_d + i;
+        b->io[i].frame_ofs = (const uint64_t*)(b->xfer_d + a0) + i;
+        b->io[i].stage = b->xfer_d + xfer_hdr(n);
+    }
+    if (b->io_d)
+        BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
 }  // namespace bsm
 
 using namespace bsm;
@@ -255,6 +336,11 @@ using namespace bsm;
 extern "C" {
 
 int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm_batch** out) {
+    return bsm_batch_create_ex(ctx, areas, n_areas, 0, out);
+}
+
+int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, uint32_t flags,
+                        bsm_batch** out) {
     if (!ctx || !out || (!areas && n_areas)) return fail("null arg");
     if (n_areas == 0 || n_areas > 65535) return fail("n_areas must be in [1, 65535]");
     for (uint32_t i = 0; i < n_areas; i++) {
@@ -288,6 +374,27 @@ int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm
         a->batch_stream = b->stream;
     }
     BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    for (int i = 0; i < 2; i++) BSM_HIP(hipEventCreateWithFlags(&b->xfer_ev[i], hipEventDisableTiming));
+    if (flags & BSM_BATCH_DYNAMIC) {
+        b->dynamic = true;
+        std::vector<uint32_t> dw((size_t)BSM_MAX_LAYERS * n_areas, 0);
+        for (uint32_t i = 0; i < n_areas; i++) {
+            bsm_area* a = areas[i];
+            uint32_t mask = 0;
+            for (uint32_t l = 0; l < a->n_layers; l++) {
+                const bool spatial = a->L[l].d.spatial != 0;
+                const bool on = spatial ? !(a->settings & BSM_SET_DISABLE_SSCS)
+                                        : !(a->settings & BSM_SET_DISABLE_PYRS);
+                dw[(size_t)i * BSM_MAX_LAYERS + l] = (uint32_t)a->L[l].cur;
+                if (on) mask |= 1u << l;
+            }
+            b->io[i].layer_mask = mask;
+        }
+        BSM_HIP(hipMalloc((void**)&b->dyn_d, dw.size() * sizeof(uint32_t)));
+        BSM_HIP(hipMemcpy(b->dyn_d, dw.data(), dw.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        for (uint32_t i = 0; i < n_areas; i++) b->io[i].dyn = b->dyn_d + (size_t)i * BSM_MAX_LAYERS;
+        if (grow_xfer(b, 1 << 16)) return 1;
+    }
     BSM_HIP(hipMalloc((void**)&b->io_d, sizeof(AreaIo) * n_areas));
     BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n_areas, hipMemcpyHostToDevice));
     b->dirty = true;
@@ -304,6 +411,13 @@ int bsm_batch_sync(bsm_batch* b) {
     BSM_HIP(hipMemcpy(rng.data(), b->rng_d, rng.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < b->areas.size(); i++)
         std::memcpy(b->areas[i]->rng, &rng[4 * i], sizeof(b->areas[i]->rng));
+    if (b->dynamic) {
+        std::vector<uint32_t> dw((size_t)BSM_MAX_LAYERS * b->areas.size());
+        BSM_HIP(hipMemcpy(dw.data(), b->dyn_d, dw.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < b->areas.size(); i++)
+            for (uint32_t l = 0; l < b->areas[i]->n_layers; l++)
+                b->areas[i]->L[l].cur = (int)(dw[i * BSM_MAX_LAYERS + l] & 1u);
+    }
     return 0;
 }
 
@@ -317,6 +431,13 @@ int bsm_batch_destroy(bsm_batch* b) {
     (void)hipFree(b->pjobs_d);
     (void)hipFree(b->items_d);
     (void)hipFree(b->io_d);
+    (void)hipFree(b->dyn_d);
+    (void)hipFree(b->tracts_d);
+    (void)hipFree(b->xfer_d);
+    for (int i = 0; i < 2; i++) {
+        if (b->xfer_h[i]) (void)hipHostFree(b->xfer_h[i]);
+        (void)hipEventDestroy(b->xfer_ev[i]);
+    }
     (void)hipFree(b->rng_d);
     (void)hipFree(b->counter_d);
     (void)hipEventDestroy(b->ev0);
@@ -392,6 +513,96 @@ int bsm_batch_set_tuning(bsm_batch* b, const bsm_tuning* t) {
     BSM_HIP(hipStreamSynchronize(b->stream));
     b->tune = nt;
     b->dirty = true;
+    return 0;
+}
+
+
+int bsm_batch_set_input_ranges(bsm_batch* b, const uint32_t* axn_offset, const uint32_t* len) {
+    if (!b || !axn_offset || !len) return fail("null arg");
+    if (!b->dynamic) return fail("input ranges apply to dynamic batches");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->areas.size(); i++) {
+        bsm_area* a = b->areas[i];
+        if (axn_offset[i] > a->axn_len || len[i] > a->axn_len - axn_offset[i])
+            return fail("input range out of the axon space of area " + std::to_string(i));
+        b->io[i].dst = a->axn + axn_offset[i];
+        b->io[i].len = len[i];
+    }
+    BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * b->io.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* frames,
+                           const uint64_t* frame_ofs, uint64_t frame_bytes, int use_graph) {
+    if (!b || !active) return fail("null arg");
+    if (!b->dynamic) return fail("masked cycles need a dynamic batch");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    const size_t n = b->areas.size();
+    for (size_t i = 0; frame_ofs && i < n; i++)
+        if (frame_ofs[i] != ~0ull && (frame_ofs[i] > frame_bytes ||
+                                      b->io[i].len > frame_bytes - frame_ofs[i]))
+            return fail("frame of area " + std::to_string(i) + " out of the packed buffer");
+    if (grow_xfer(b, frame_bytes)) return 1;
+    if (b->dirty && batch_plan(b)) return 1;
+    if (use_graph && !b->exec[0] && capture(b)) return 1;
+    const size_t hdr = xfer_hdr(n), total = hdr + frame_bytes;
+    const int k = b->xfer_next;
+    b->xfer_next ^= 1;
+    BSM_HIP(hipEventSynchronize(b->xfer_ev[k]));
+    if (b->xfer_h_cap[k] < total) {
+        if (b->xfer_h[k]) BSM_HIP(hipHostFree(b->xfer_h[k]));
+        b->xfer_h[k] = nullptr;
+        const size_t cap = std::max(total, (size_t)1 << 16);
+        BSM_HIP(hipHostMalloc((void**)&b->xfer_h[k], cap, hipHostMallocDefault));
+        b->xfer_h_cap[k] = cap;
+    }
+    uint8_t* h = b->xfer_h[k];
+    std::memcpy(h, active, n);
+    uint64_t* fo = (uint64_t*)(h + ((n + 15) & ~(size_t)15));
+    for (size_t i = 0; i < n; i++) fo[i] = frame_ofs ? frame_ofs[i] : ~0ull;
+    if (frame_bytes) std::memcpy(h + hdr, frames, frame_bytes);
+    BSM_HIP(hipMemcpyAsync(b->xfer_d, h, total, hipMemcpyHostToDevice, b->stream));
+    BSM_HIP(hipEventRecord(b->xfer_ev[k], b->stream));
+    return run_cycles(b, 1, use_graph);
+}
+
+int bsm_batch_query(bsm_batch* b, int* done) {
+    if (!b || !done) return fail("null arg");
+    hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) { *done = 1; return 0; }
+    if (e == hipErrorNotReady) { *done = 0; return 0; }
+    return hip_fail(e, "hipStreamQuery");
+}
+
+int bsm_batch_set_tracts(bsm_batch* b, const bsm_tract* tracts, uint32_t n) {
+    if (!b || (!tracts && n)) return fail("null arg");
+    if (b->dynamic && n) return fail("tracts apply to static batches");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    std::vector<TractRt> rt(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const bsm_tract& t = tracts[i];
+        if (t.src_area >= b->areas.size() || t.dst_area >= b->areas.size())
+            return fail("tract " + std::to_string(i) + ": area index out of range");
+        bsm_area* sa = b->areas[t.src_area];
+        bsm_area* da = b->areas[t.dst_area];
+        if (t.src_offset > sa->axn_len || t.len > sa->axn_len - t.src_offset ||
+            t.dst_offset > da->axn_len || t.len > da->axn_len - t.dst_offset)
+            return fail("tract " + std::to_string(i) + ": range out of the axon space");
+        if (t.src_area == t.dst_area && t.src_offset < t.dst_offset + t.len &&
+            t.dst_offset < t.src_offset + t.len)
+            return fail("tract " + std::to_string(i) + ": overlapping ranges");
+        rt[i] = TractRt{sa->axn + t.src_offset, da->axn + t.dst_offset, t.len, 0};
+    }
+    (void)hipFree(b->tracts_d);
+    b->tracts_d = nullptr;
+    if (n) {
+        BSM_HIP(hipMalloc((void**)&b->tracts_d, sizeof(TractRt) * n));
+        BSM_HIP(hipMemcpy(b->tracts_d, rt.data(), sizeof(TractRt) * n, hipMemcpyHostToDevice));
+    }
+    b->n_tracts = n;
+    free_graphs(b);
     return 0;
 }
 

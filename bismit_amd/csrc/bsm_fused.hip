@@ -85,7 +85,12 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         const uint32_t V = J.V, U = J.U, depth = J.depth, spt = J.spt, halo = J.halo;
         const uint32_t v0 = (uint32_t)wi.tv * TV, u0 = (uint32_t)wi.tu * TU;
         const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
-        const uint32_t p = par ^ J.par_xor;
+        uint32_t p = par ^ J.par_xor;
+        if (J.dyn) {  // dynamic batch: skip areas that do not cycle now (uniform per block)
+            const uint32_t d = *J.dyn;
+            if (!(d & 2u)) continue;
+            p = d & 1u;
+        }
         const bool quirk = J.scalar_quirk != 0;
 
         // ---- stage the tile's hex neighbourhood of each staged source slice
@@ -293,7 +298,12 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
         const int32_t r = J.mode == 0 ? (int32_t)J.radius : 0;
         const uint32_t WV = tvn + 2 * r, WU = tun + 2 * r;
-        const uint32_t p = par ^ J.par_xor;
+        uint32_t p = par ^ J.par_xor;
+        if (J.dyn) {
+            const uint32_t d = *J.dyn;
+            if (!(d & 2u)) continue;
+            p = d & 1u;
+        }
         uint32_t* list = (uint32_t*)(lds + ((WV * WU + 15) & ~15u));
         const uint8_t* soma = J.soma + (size_t)slc * VU;
         for (uint32_t x = tid; x < WV * WU; x += nth) {
@@ -406,7 +416,12 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
         const uint32_t VU = J.V * J.U, C = J.C, depth = J.depth;
         const uint32_t col = (uint32_t)wi.tv * F3_COLS + tid;
         const bool valid = col < VU;
-        const uint32_t p = par ^ J.par_xor;
+        uint32_t p = par ^ J.par_xor;
+        if (J.dyn) {
+            const uint32_t d = *J.dyn;
+            if (!(d & 2u)) continue;
+            p = d & 1u;
+        }
         // pass 1: pyr_cycle for every slice of the column, running intra-column best
         uint32_t best = 0;
         if (valid) {

@@ -50,6 +50,7 @@ struct TuftJob {
     uint32_t halo;         // staged reach around the tile, in source-slice columns
     uint32_t par_xor;      // state buffer written = st[(par ^ par_xor) & 1]
     uint32_t _pad;
+    const uint32_t* dyn;   // dynamic batches: per (area, layer) word, bit0 parity, bit1 active
     // host-side planning copies of the staged slices' geometry
     uint16_t st_vsize[4], st_usize[4];
     uint8_t st_vsc[4], st_usc[4];
@@ -70,6 +71,7 @@ struct SscJob {
     uint32_t mode;         // 0 inhib_simple, 1 passthrough(rnd 0), 2 none
     uint32_t learn;
     uint32_t par_xor;
+    const uint32_t* dyn;
 };
 
 // F3 work unit: pyramidal layer post-processing (soma + column control + distal LTP).
@@ -95,6 +97,7 @@ struct PyrJob {
     uint32_t distal_mask;
     uint32_t par_xor;
     uint32_t max_chunks;   // max over distal tufts of syns_per_den / chunk
+    const uint32_t* dyn;
     uint32_t t_spd[BSM_MAX_TUFTS];
     uint32_t t_spt[BSM_MAX_TUFTS];
     uint32_t t_syn_idz[BSM_MAX_TUFTS];
@@ -210,6 +213,9 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs);
 int ensure_rnd(bsm_area* a, size_t n_ints);
+int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n,
+                  uint32_t n_src, const uint8_t* src_ids, int32_t reach, uint8_t* slc, int8_t* v,
+                  int8_t* u);
 void rng_seed(uint32_t st[4], uint64_t seed);
 uint32_t rng_next(uint32_t st[4]);
 }  // namespace bsm

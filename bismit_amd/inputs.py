@@ -40,13 +40,24 @@ class SdrStream:
 
     def next_active(self):
         kept = self.rng.choice(self.active, self.keep, replace=False)
-        mask = np.ones(self.n, dtype=bool)
-        mask[self.active] = False  # fresh axons come from outside the previous set
-        pool = np.flatnonzero(mask)
-        fresh = self.rng.choice(pool, self.k - self.keep, replace=False)
+        need = self.k - self.keep
+        fresh = np.empty(0, dtype=np.int64)
+        # fresh axons come from outside the previous set (rejection sampling)
+        while fresh.size < need:
+            cand = self.rng.integers(0, self.n, 2 * need + 8)
+            cand = cand[~np.isin(cand, self.active)]
+            cand = np.concatenate([fresh, cand])
+            _, first = np.unique(cand, return_index=True)
+            fresh = cand[np.sort(first)]
+        fresh = fresh[:need]
         cur = self.active
         self.active = np.concatenate([kept, fresh])
         return cur
+
+    def next_frame(self):
+        out = np.zeros(self.n, dtype=np.uint8)
+        out[self.next_active()] = 255
+        return out
 
     def frames(self, n_cycles):
         out = np.zeros((n_cycles, self.n), dtype=np.uint8)

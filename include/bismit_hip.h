@@ -262,6 +262,33 @@ int bsm_xorshift128_fill(uint64_t seed, uint32_t* out, uint64_t n);
 
 /* ---- batched form: many areas per stage in one persistent launch ---- */
 int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm_batch** out);
+/* flags: BSM_BATCH_DYNAMIC -- each cycle steps only the areas flagged active in
+ * bsm_batch_cycle_masked (continuous batching for serving); every area keeps its
+ * own state parity, rnd stream and input frame. */
+enum bsm_batch_flags { BSM_BATCH_DYNAMIC = 1u };
+int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, uint32_t flags,
+                        bsm_batch** out);
+/* Dynamic batches: axon range each area's frame is copied to. */
+int bsm_batch_set_input_ranges(bsm_batch* b, const uint32_t* axn_offset, const uint32_t* len);
+/* Dynamic batches: one shared cycle of the areas with active[i] != 0.  frames is a
+ * host buffer of frame_bytes; area i's frame starts at frame_ofs[i] (UINT64_MAX:
+ * no intake).  Returns after enqueueing (asynchronous); bsm_batch_query polls. */
+int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* frames,
+                           const uint64_t* frame_ofs, uint64_t frame_bytes, int use_graph);
+int bsm_batch_query(bsm_batch* b, int* done);
+/* Inter-area tracts (thalamic routing, src/subcortex/thalamus/thalamus.rs:71-285,
+ * src/cortex/axon_space.rs:509-629): at the start of every batch cycle, before
+ * any area cycles, each tract copies len axon states of area src_area (as the
+ * previous cycle left them) to area dst_area.  Indices refer to the batch's
+ * area order.  Static batches only. */
+typedef struct bsm_tract {
+    uint32_t src_area, src_offset, dst_area, dst_offset, len;
+} bsm_tract;
+int bsm_batch_set_tracts(bsm_batch* b, const bsm_tract* tracts, uint32_t n);
+/* Plain form of one tract: copy src's axon range into dst's, after everything
+ * enqueued on src; complete when the call returns. */
+int bsm_area_copy_axons(bsm_area* dst, uint32_t dst_offset, bsm_area* src, uint32_t src_offset,
+                        uint32_t len);
 int bsm_batch_destroy(bsm_batch* b);
 /* use_graph != 0: each cycle replays a captured hipGraph. */
 int bsm_batch_cycle(bsm_batch* b, uint32_t n_cycles, int use_graph);
