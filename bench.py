@@ -85,6 +85,83 @@ def cpu_baseline(scheme, n_cycles, frames):
                       f"oracle/bsm_oracle.c with OpenMP over {threads} threads, {dt:.1f} s"}
 
 
+def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
+    """Config 3: 64 uneven areas in one persistent launch per stage with a
+    hipGraph-captured cycle, against the same areas cycled one by one."""
+    import numpy as np
+    from bismit_amd import inputs, recipes
+    from bismit_amd.autotune import candidates
+    from bismit_amd.cortex import Batch, CorticalArea
+    schemes = recipes.config3_areas(64, seed=0)
+    areas = [CorticalArea(s, ctx, rnd_seed=i, layout_seed=i) for i, s in enumerate(schemes)]
+    n_syn = sum(s.n_synapses() for s in schemes)
+    nf = 64
+    devs, lens = [], []
+    for i, s in enumerate(schemes):
+        f = inputs.SdrStream(s.input_range()[1], 0.02, 0.5, seed=i).frames(nf)
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        devs.append(d)
+        lens.append(f.shape[1])
+    # plain per-area form: every area launches its own stages on its own stream
+    # (the reference gives each area its own queue); all areas' cycles are
+    # enqueued up front so host overhead stays out and streams may overlap
+    for i, a in enumerate(areas):
+        a.run(devs[i], lens[i], lens[i], 3)
+    ctx.sync()
+    n_plain = max(20, n_cycles // 4)
+    t0 = time.perf_counter()
+    for i, a in enumerate(areas):
+        a.run(devs[i], lens[i], lens[i], min(n_plain, nf))
+        if n_plain > nf:
+            a.run(devs[i], lens[i], lens[i], n_plain - nf)
+    for a in areas:
+        a.sync()
+    ctx.sync()
+    plain_s = time.perf_counter() - t0
+    plain = n_syn * n_plain / plain_s
+    b = Batch(areas, ctx)
+    b.set_inputs(devs, lens, lens, [s.input_range()[0] for s in schemes], nf)
+    best, table = None, []
+    if tune:
+        for t in candidates():
+            b.set_tuning(**t)
+            b.cycle(2)
+            ms = b.time_cycles(10) / 10
+            table.append((ms, t))
+        table.sort(key=lambda x: x[0])
+        best = table[0][1]
+        b.set_tuning(**best)
+    b.cycle(5)
+    times = [b.time_cycles(n_cycles) for _ in range(runs)]
+    ms = statistics.median(times)
+    batched = n_syn * n_cycles / (ms / 1000.0)
+    b.close()
+    for a in areas:
+        a.close()
+    for d in devs:
+        ctx.free(d)
+    return {"workload": "config3: 64 areas, sides log-uniform 32..192 (multiples of 8), 1-3 slices "
+                        "per layer, 16-64 synapses per dendrite, own 2% SDR input each",
+            "synapses": n_syn, "batched_synapse_updates_per_s": batched,
+            "plain_per_area_synapse_updates_per_s": plain, "speedup_vs_plain": batched / plain,
+            "batched_ms_per_cycle": ms / n_cycles, "plain_ms_per_cycle": 1000 * plain_s / n_plain,
+            "tuning": best, "hbm_frac_at_5B_per_synapse": batched * 5 / (HBM_PEAK_GBS * 1e9)}
+
+
+def serving_leg(ctx, wall_s=20.0, rate=200.0, budget_gb=96, max_sessions=64):
+    """Config 5: Poisson session arrivals, HBM-footprint admission, continuous
+    batching of frames into shared cycles; p50/p99 per-frame latency."""
+    from bismit_amd.serve import Server, session_specs
+    specs = session_specs(rate=rate, horizon_s=wall_s, seed=0)
+    srv = Server(ctx, hbm_budget_bytes=budget_gb << 30, max_sessions=max_sessions)
+    st = srv.run(specs, wall_limit_s=wall_s)
+    out = st.summary()
+    out.update({"arrival_rate_sessions_per_s": rate, "hbm_budget_gb": budget_gb,
+                "max_sessions": max_sessions})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +170,9 @@ def main():
     ap.add_argument("--runs", type=int, default=5, help="timed runs (median reported)")
     ap.add_argument("--cpu-cycles", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="profiling helper: one copy-kernel calibration launch, then 3 cycles")
     args = ap.parse_args()
@@ -171,6 +251,12 @@ def main():
         "roofline": roofline,
         "stages": stages,
     }
+    if not args.no_batched:
+        out["batched"] = batched_leg(ctx)
+        print("# batched leg done", file=sys.stderr, flush=True)
+    if not args.no_serve:
+        out["serving"] = serving_leg(ctx, wall_s=args.serve_seconds)
+        print("# serving leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scheme, args.cpu_cycles,
                                            inputs.SdrStream(in_len, 0.02, 0.5, seed=0).frames(8))

@@ -35,6 +35,8 @@ class SdrStream:
         self.n = n_axons
         self.k = max(1, int(round(density * n_axons)))
         self.keep = int(round(overlap * self.k))
+        if self.n - self.k < self.k - self.keep:
+            raise ValueError("density too high for the requested overlap")
         self.rng = np.random.default_rng(seed)
         self.active = self.rng.choice(self.n, self.k, replace=False)
 
