@@ -179,6 +179,10 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
             T.spt = T.d.dens_per_tft * T.d.syns_per_den;
             T.syn_idz = ns;
             T.den_idz = nd;
+            // zero-filled buffers: slice 0, offsets 0
+            T.reach_max = 0;
+            T.slc_bits[0] = 1;
+            T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
             ns += (uint64_t)L.cells * T.spt;
             nd += (uint64_t)L.cells * T.d.dens_per_tft;
         }
@@ -222,6 +226,43 @@ int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, u
     k_layout<<<grid, 256, 0, s>>>(key, n, n_src, src, reach, slc, v, u);
     BSM_HIP(hipGetLastError());
     return 0;
+}
+
+void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
+                        uint32_t n_ids) {
+    TuftRt& T = a->L[l].t[t];
+    T.reach_max = reach;
+    T.slc_bits[0] = T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
+    for (uint32_t k = 0; k < n_ids; k++) T.slc_bits[ids[k] >> 6] |= 1ull << (ids[k] & 63);
+    a->plan_dirty = true;
+    a->layout_epoch++;
+}
+
+// Conservative update of the per-tuft source summary after a write (data) or a
+// fill (fill_val >= 0) of a synapse source buffer range.
+void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len,
+                         const uint8_t* data, int fill_val) {
+    if (buf != BSM_SYN_SRC_SLC_IDS && buf != BSM_SYN_SRC_COL_V_OFFS &&
+        buf != BSM_SYN_SRC_COL_U_OFFS)
+        return;
+    LayerRt& L = a->L[layer];
+    for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+        TuftRt& T = L.t[t];
+        const size_t z = T.syn_idz, n = (size_t)L.cells * T.spt;
+        const size_t lo = std::max(offset, z), hi = std::min(offset + len, z + n);
+        if (lo >= hi) continue;
+        for (size_t i = lo; i < hi; i++) {
+            const uint8_t b = data ? data[i - offset] : (uint8_t)fill_val;
+            if (buf == BSM_SYN_SRC_SLC_IDS) {
+                T.slc_bits[b >> 6] |= 1ull << (b & 63);
+            } else {
+                const int32_t v = (int8_t)b;
+                T.reach_max = std::max<uint32_t>(T.reach_max, (uint32_t)(v < 0 ? -v : v));
+            }
+        }
+        a->plan_dirty = true;
+        a->layout_epoch++;
+    }
 }
 
 }  // namespace bsm
@@ -368,6 +409,7 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->items_d = nullptr;
     a->plan_dirty = true;
     a->batch_stream = nullptr;
+    a->layout_epoch = 0;
     *out = a;
     return 0;
 }
@@ -521,6 +563,7 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
         return 0;
     }
     BSM_HIP(hipMemcpy(p + offset, src, len, hipMemcpyHostToDevice));
+    layout_stats_update(a, buf, layer, offset, len, (const uint8_t*)src, -1);
     return 0;
 }
 
@@ -538,6 +581,13 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
                                a->stream));
     } else {
         BSM_HIP(hipMemsetAsync(p, val, n, a->stream));
+        if (buf != BSM_AXN_STATES) {
+            // a whole-buffer fill replaces the summary of every tuft of the layer
+            for (uint32_t t = 0; t < a->L[layer].d.n_tufts; t++) {
+                if (buf == BSM_SYN_SRC_SLC_IDS) layout_stats_reset(a, layer, t, a->L[layer].t[t].reach_max, &val, 1);
+            }
+            layout_stats_update(a, buf, layer, 0, n, nullptr, val);
+        }
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;
@@ -675,6 +725,7 @@ int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
             if (launch_layout(a->stream, seed, l, t, n, T.d.n_src_slcs, T.d.src_slc_ids, reach,
                               L.syn_slc + T.syn_idz, L.syn_v + T.syn_idz, L.syn_u + T.syn_idz))
                 return 1;
+            layout_stats_reset(a, l, t, (uint32_t)reach, T.d.src_slc_ids, T.d.n_src_slcs);
         }
         BSM_HIP(hipMemsetAsync(L.syn_str, 0, L.n_syn, a->stream));
     }

@@ -130,6 +130,7 @@ struct bsm_batch {
     int xfer_next = 0;
     bsm::TractRt* tracts_d = nullptr;
     uint32_t n_tracts = 0;
+    std::vector<uint32_t> epochs;   // areas' layout epochs the plan was made for
 };
 
 namespace bsm {
@@ -151,6 +152,8 @@ static int batch_plan(bsm_batch* b) {
     b->sjobs.clear();
     b->pjobs.clear();
     b->stages.clear();
+    b->epochs.clear();
+    for (bsm_area* a : b->areas) b->epochs.push_back(a->layout_epoch);
     std::vector<WorkItem> items;
     const uint32_t na = (uint32_t)b->areas.size();
     // rank of each layer among the area's spatial / pyramidal layers
@@ -278,7 +281,20 @@ static int capture(bsm_batch* b) {
     return 0;
 }
 
+static bool layouts_changed(const bsm_batch* b) {
+    for (size_t i = 0; i < b->areas.size(); i++)
+        if (i >= b->epochs.size() || b->areas[i]->layout_epoch != b->epochs[i]) return true;
+    return false;
+}
+
 static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
+    if (!b->dirty && layouts_changed(b)) {
+        // a layout write may have changed which gather path is exact: re-plan
+        // against the areas' current parities
+        BSM_HIP(hipStreamSynchronize(b->stream));
+        if (bsm_batch_sync(b)) return 1;
+        b->dirty = true;
+    }
     if (b->dirty && batch_plan(b)) return 1;
     if (use_graph && !b->exec[0] && capture(b)) return 1;
     for (uint32_t c = 0; c < n; c++) {

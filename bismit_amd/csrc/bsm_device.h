@@ -29,6 +29,12 @@ struct SliceRt {
     uint32_t _pad;
 };
 
+// Global-address-space views of device pointers.  Job tables carry plain
+// pointers, which the compiler would otherwise access with flat instructions
+// (these also count against the LDS wait counter and serialise LDS waits
+// behind outstanding HBM loads).
+#define BSM_G(T, p) ((__attribute__((address_space(1))) T*)(p))
+
 // xorshift mix: cl/bismit.cl:426-431
 __device__ __forceinline__ int32_t rnd_mix(int32_t rnd, int32_t seed) {
     uint32_t s = (uint32_t)seed;
@@ -72,7 +78,7 @@ __device__ __forceinline__ uint32_t axn_gather(const uint8_t* __restrict__ axn,
                                                int32_t vo, int32_t u, int32_t uo,
                                                bool scalar_quirk) {
     if (slc >= n_slices) return 0;
-    SliceRt s = tab[slc];
+    const SliceRt s = tab[slc];
     int32_t vm = s.v_mid, um = s.u_mid;
     int32_t vof, uof;
     if (scalar_quirk) {
@@ -85,7 +91,7 @@ __device__ __forceinline__ uint32_t axn_gather(const uint8_t* __restrict__ axn,
     int32_t vv = (int32_t)(((uint32_t)v * s.v_scale) >> 4) + vof;
     int32_t uu = (int32_t)(((uint32_t)u * s.u_scale) >> 4) + uof;
     if (vv < 0 || vv >= (int32_t)s.v_size || uu < 0 || uu >= (int32_t)s.u_size) return 0;
-    return axn[s.idz + (uint32_t)(vv * (int32_t)s.u_size + uu)];
+    return BSM_G(const uint8_t, axn)[s.idz + (uint32_t)(vv * (int32_t)s.u_size + uu)];
 }
 
 // The cell's own axon index (scalar axn_idx_3d_unsafe with zero offsets,

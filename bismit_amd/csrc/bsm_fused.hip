@@ -35,40 +35,127 @@ template <>
 struct Chunk<16> {
     static constexpr int NW = 4;
     __device__ static inline void load(const void* p, uint32_t* o) {
-        u32x4 x = __builtin_nontemporal_load((const u32x4*)p);
+        u32x4 x = __builtin_nontemporal_load(BSM_G(const u32x4, p));
         o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w;
     }
     __device__ static inline void store(void* p, const uint32_t* o) {
-        *(uint4*)p = make_uint4(o[0], o[1], o[2], o[3]);
+        u32x4 v = {o[0], o[1], o[2], o[3]};
+        *BSM_G(u32x4, p) = v;
     }
 };
 template <>
 struct Chunk<8> {
     static constexpr int NW = 2;
     __device__ static inline void load(const void* p, uint32_t* o) {
-        u32x2 x = __builtin_nontemporal_load((const u32x2*)p);
+        u32x2 x = __builtin_nontemporal_load(BSM_G(const u32x2, p));
         o[0] = x.x; o[1] = x.y;
     }
     __device__ static inline void store(void* p, const uint32_t* o) {
-        *(uint2*)p = make_uint2(o[0], o[1]);
+        u32x2 v = {o[0], o[1]};
+        *BSM_G(u32x2, p) = v;
     }
 };
 template <>
 struct Chunk<4> {
     static constexpr int NW = 1;
     __device__ static inline void load(const void* p, uint32_t* o) {
-        o[0] = __builtin_nontemporal_load((const uint32_t*)p);
+        o[0] = __builtin_nontemporal_load(BSM_G(const uint32_t, p));
     }
-    __device__ static inline void store(void* p, const uint32_t* o) { *(uint32_t*)p = o[0]; }
+    __device__ static inline void store(void* p, const uint32_t* o) { *BSM_G(uint32_t, p) = o[0]; }
 };
 template <>
 struct Chunk<1> {
     static constexpr int NW = 1;
-    __device__ static inline void load(const void* p, uint32_t* o) { o[0] = *(const uint8_t*)p; }
-    __device__ static inline void store(void* p, const uint32_t* o) { *(uint8_t*)p = (uint8_t)o[0]; }
+    __device__ static inline void load(const void* p, uint32_t* o) { o[0] = *BSM_G(const uint8_t, p); }
+    __device__ static inline void store(void* p, const uint32_t* o) { *BSM_G(uint8_t, p) = (uint8_t)o[0]; }
 };
 
-template <int W, int NST>
+// Dendrites (den_cycle_tft, ssc_cycle on the SSC soma) and tufts (tft_cycle) of
+// one tile from the per-chunk (sum, raw) partials in LDS; ends with a barrier.
+template <int W>
+__device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* part,
+                                 uint32_t stage_cap, uint32_t n_chunks, uint32_t depth,
+                                 uint32_t tvn, uint32_t tun, uint32_t v0, uint32_t u0,
+                                 uint32_t VU, uint32_t U, uint32_t CPT, int32_t rnd, uint32_t tid,
+                                 uint32_t nth) {
+    // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
+    const uint32_t dens = J.dens, CPD = J.spd / W;
+    const uint32_t cells_t = depth * tvn * tun;
+    const uint32_t nd_t = cells_t * dens;
+    const bool do_tft = J.tft[0] != nullptr;
+    uint8_t* denc = lds + stage_cap + n_chunks * 4;
+    auto den_act = BSM_G(uint8_t, J.den_act);
+    auto den_raw = BSM_G(uint8_t, J.den_raw);
+    auto den_st = BSM_G(uint8_t, J.den_st);
+    auto energy = BSM_G(uint8_t, J.ssc_energy);
+    for (uint32_t x = tid; x < nd_t; x += nth) {
+        const uint32_t ct = x / dens, dj = x - ct * dens;
+        const uint32_t q0 = ct * CPT + dj * CPD;
+        uint32_t sum = 0, raw = 0;
+        for (uint32_t m = 0; m < CPD; m++) {
+            uint32_t pv = part[q0 + m];
+            sum += pv & 0xFFFFu;
+            raw += pv >> 16;
+        }
+        const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
+        const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+        const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+        const uint32_t dl = cel * dens + dj;
+        const uint32_t dg = J.den_idz + dl;
+        const int32_t active = sum > J.thresh;
+        sum = active ? sum : 0u;
+        const uint32_t raw_o = min(raw >> 4, 255u);
+        uint32_t st_o = min(sum >> 4, 255u);
+        const int32_t seed = (int32_t)((raw + J.thresh) | dg);
+        den_act[dl] = (uint8_t)activity(den_act[dl], active, rnd, seed, DEN_ACT_DECAY);
+        den_raw[dl] = (uint8_t)raw_o;
+        if (J.ssc_energy != nullptr && dg < J.C) {
+            // ssc_cycle (cl/bismit.cl:849-884) on soma index dg == dendrite dg
+            const uint32_t e = energy[dg];
+            const uint32_t restless = (e > 191) & (st_o != 0);
+            const uint32_t dark = e == 255;
+            if (dark | restless) energy[dg] = (uint8_t)(e - 64);
+            st_o = min(st_o + 255u * restless + 255u * dark, 255u);
+        }
+        den_st[dl] = (uint8_t)st_o;
+        if (do_tft) {
+            denc[x] = (uint8_t)raw_o;
+            denc[nd_t + x] = (uint8_t)st_o;
+        }
+    }
+
+    // ---- tufts (tft_cycle)
+    if (do_tft) {
+        __syncthreads();
+        const uint32_t l2 = J.l2;
+        for (uint32_t ct = tid; ct < cells_t; ct += nth) {
+            uint32_t best_id = 0, best_raw = 0, best = 0, asum = 0;
+            for (uint32_t j = 0; j < dens; j++) {
+                const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
+                if (sr > best_raw) { best_id = j; best_raw = sr; }
+                best = max(best, s);
+                asum += s;
+            }
+            const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
+            const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
+            const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+            const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+            auto t0 = BSM_G(uint8_t, J.tft[0]), t1 = BSM_G(uint8_t, J.tft[1]);
+            auto t2 = BSM_G(uint8_t, J.tft[2]), t3 = BSM_G(uint8_t, J.tft[3]);
+            BSM_G(uint8_t, J.tft[4])[cel] = t0[cel];
+            BSM_G(uint8_t, J.tft[5])[cel] = t1[cel];
+            BSM_G(uint8_t, J.tft[6])[cel] = t2[cel];
+            BSM_G(uint8_t, J.tft[7])[cel] = t3[cel];
+            t0[cel] = (uint8_t)best_id;
+            t1[cel] = (uint8_t)best_raw;
+            t2[cel] = (uint8_t)best;
+            t3[cel] = (uint8_t)tstate;
+        }
+    }
+    __syncthreads();
+}
+
+template <int W, int NST, bool FAST>
 __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                                                const WorkItem* __restrict__ items, uint32_t n_items,
                                                uint32_t par, uint32_t rnd_ofs, uint32_t TV,
@@ -93,6 +180,121 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         const bool quirk = J.scalar_quirk != 0;
 
+        if constexpr (FAST) {
+            // Every synapse of the job sources a staged slice with |offsets| <= halo
+            // (proved on the host), and the staged slices share one geometry: stage
+            // the full tile + halo window of each (zeros off the slice) and gather
+            // without any per-synapse test.
+            const SliceRt s0 = J.slices[J.staged[0]];
+            const int32_t ih = (int32_t)halo;
+            const int32_t a0 = (int32_t)((v0 * s0.v_scale) >> 4) + (int32_t)s0.v_mid - ih;
+            const int32_t b0 = (int32_t)((u0 * s0.u_scale) >> 4) + (int32_t)s0.u_mid - ih;
+            const int32_t nr = (int32_t)(((v0 + tvn - 1) * s0.v_scale) >> 4) -
+                               (int32_t)((v0 * s0.v_scale) >> 4) + 1 + 2 * ih;
+            const int32_t nc = (int32_t)(((u0 + tun - 1) * s0.u_scale) >> 4) -
+                               (int32_t)((u0 * s0.u_scale) >> 4) + 1 + 2 * ih;
+            const uint32_t WS = (uint32_t)(nr * nc);
+            for (uint32_t j = 0; j < J.n_staged; j++) {
+                const SliceRt s = J.slices[J.staged[j]];
+                auto src = BSM_G(const uint8_t, J.axn + s.idz);
+                for (uint32_t x = tid; x < WS; x += nth) {
+                    const int32_t rr = (int32_t)(x / (uint32_t)nc), cc = (int32_t)x - rr * nc;
+                    const int32_t gv = a0 + rr, gu = b0 + cc;
+                    uint8_t val = 0;
+                    if (gv >= 0 && gv < (int32_t)s.v_size && gu >= 0 && gu < (int32_t)s.u_size)
+                        val = src[(uint32_t)gv * s.u_size + (uint32_t)gu];
+                    lds[j * WS + x] = val;
+                }
+            }
+            __syncthreads();
+            const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
+            const uint32_t CPT = spt / W;
+            const uint32_t row_chunks = tun * CPT;
+            const uint32_t n_chunks = depth * tvn * row_chunks;
+            const uint32_t VU = V * U;
+            uint8_t* __restrict__ st_out = J.st[p & 1];
+            const int32_t coefA = (nc & 0xFF) | (1 << 8);
+            const int32_t coefB = ((nc & 0xFF) << 16) | (1 << 24);
+            const int32_t sid0ws = (int32_t)(J.staged[0] * WS);
+            auto pos = [&](uint32_t q, size_t& so, int32_t& lb) {
+                const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
+                const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+                const uint32_t cc = rem / CPT, k = rem - cc * CPT;
+                const uint32_t v = v0 + r, u = u0 + cc;
+                so = (size_t)(sl * VU + v * U + u) * spt + (size_t)k * W;
+                lb = ((int32_t)((v * s0.v_scale) >> 4) + (int32_t)s0.v_mid - a0) * nc +
+                     (int32_t)((u * s0.u_scale) >> 4) + (int32_t)s0.u_mid - b0;
+                if (NST > 1) lb -= sid0ws;
+            };
+            uint32_t ms[NW], mv[NW], mu[NW], mt[NW];
+            uint32_t q = tid;
+            size_t so = 0;
+            int32_t lb = 0;
+            if (q < n_chunks) {
+                pos(q, so, lb);
+                Chunk<W>::load(J.slc + so, ms);
+                Chunk<W>::load(J.vofs + so, mv);
+                Chunk<W>::load(J.uofs + so, mu);
+                Chunk<W>::load(J.str + so, mt);
+            }
+            for (; q < n_chunks; q += nth) {
+                // prefetch the next chunk's metadata before working on this one
+                const uint32_t qn = q + nth;
+                size_t so_n = 0;
+                int32_t lb_n = 0;
+                uint32_t ns[NW], nv[NW], nu[NW], nt[NW];
+                if (qn < n_chunks) {
+                    pos(qn, so_n, lb_n);
+                    Chunk<W>::load(J.slc + so_n, ns);
+                    Chunk<W>::load(J.vofs + so_n, nv);
+                    Chunk<W>::load(J.uofs + so_n, nu);
+                    Chunk<W>::load(J.str + so_n, nt);
+                }
+                uint32_t out[NW];
+                uint32_t psum = 0, praw = 0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    // [vo_b, uo_b, vo_b+1, uo_b+1] byte pairs -> dot4 gives lb + vo*cols + uo
+                    const uint32_t p01 = __builtin_amdgcn_perm(mu[w], mv[w], 0x05010400u);
+                    const uint32_t p23 = __builtin_amdgcn_perm(mu[w], mv[w], 0x07030602u);
+                    int32_t ad0 = __builtin_amdgcn_sdot4((int32_t)p01, coefA, lb, false);
+                    int32_t ad1 = __builtin_amdgcn_sdot4((int32_t)p01, coefB, lb, false);
+                    int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lb, false);
+                    int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lb, false);
+                    if (NST > 1) {
+                        ad0 += (int32_t)(byte_of(ms[w], 0) * WS);
+                        ad1 += (int32_t)(byte_of(ms[w], 1) * WS);
+                        ad2 += (int32_t)(byte_of(ms[w], 2) * WS);
+                        ad3 += (int32_t)(byte_of(ms[w], 3) * WS);
+                    }
+                    const uint32_t x = (uint32_t)lds[ad0] | ((uint32_t)lds[ad1] << 8) |
+                                       ((uint32_t)lds[ad2] << 16) | ((uint32_t)lds[ad3] << 24);
+                    // syn_fire on four bytes: bit 7 = (axon != 0), low 7 bits = axon >> 1
+                    const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+                    const uint32_t stw = ((t | x) & 0x80808080u) | ((x >> 1) & 0x7F7F7F7Fu);
+                    out[w] = stw;
+                    praw = __builtin_amdgcn_sad_u8(stw, 0u, praw);
+                    const uint32_t neg = ((mt[w] & 0x80808080u) >> 7) * 0xFFu;
+                    psum = __builtin_amdgcn_sad_u8(stw & ~neg, 0u, psum);
+                }
+                Chunk<W>::store(st_out + so, out);
+                part[q] = psum | (praw << 16);
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    ms[w] = ns[w];
+                    mv[w] = nv[w];
+                    mu[w] = nu[w];
+                    mt[w] = nt[w];
+                }
+                so = so_n;
+                lb = lb_n;
+            }
+            __syncthreads();
+            tuft_tail<W>(J, lds, part, stage_cap, n_chunks, depth, tvn, tun, v0, u0, VU, U, CPT,
+                      rnd, tid, nth);
+            continue;
+        }
+
         // ---- stage the tile's hex neighbourhood of each staged source slice
         int32_t vlo[NST], ulo[NST], rows[NST], cols[NST], lbase[NST];
         uint32_t sid[NST], vsc[NST], usc[NST], vmd[NST], umd[NST];
@@ -114,7 +316,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 int32_t nr = a1 >= a0 ? a1 - a0 + 1 : 0, nc = b1 >= b0 ? b1 - b0 + 1 : 0;
                 if (off + (uint32_t)(nr * nc) > stage_cap) { nr = 0; nc = 0; }
                 vlo[j] = a0; ulo[j] = b0; rows[j] = nr; cols[j] = nc; lbase[j] = (int32_t)off;
-                const uint8_t* src = J.axn + s.idz;
+                auto src = BSM_G(const uint8_t, J.axn + s.idz);
                 const uint32_t n = (uint32_t)(nr * nc);
                 for (uint32_t x = tid; x < n; x += nth) {
                     uint32_t rr = x / (uint32_t)nc, cc = x - rr * (uint32_t)nc;
@@ -184,75 +386,8 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         __syncthreads();
 
-        // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
-        const uint32_t dens = J.dens, CPD = J.spd / W;
-        const uint32_t cells_t = depth * tvn * tun;
-        const uint32_t nd_t = cells_t * dens;
-        const bool do_tft = J.tft[0] != nullptr;
-        uint8_t* denc = lds + stage_cap + n_chunks * 4;
-        for (uint32_t x = tid; x < nd_t; x += nth) {
-            const uint32_t ct = x / dens, dj = x - ct * dens;
-            const uint32_t q0 = ct * CPT + dj * CPD;
-            uint32_t sum = 0, raw = 0;
-            for (uint32_t m = 0; m < CPD; m++) {
-                uint32_t pv = part[q0 + m];
-                sum += pv & 0xFFFFu;
-                raw += pv >> 16;
-            }
-            const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
-            const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
-            const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
-            const uint32_t dl = cel * dens + dj;
-            const uint32_t dg = J.den_idz + dl;
-            const int32_t active = sum > J.thresh;
-            sum = active ? sum : 0u;
-            const uint32_t raw_o = min(raw >> 4, 255u);
-            uint32_t st_o = min(sum >> 4, 255u);
-            const int32_t seed = (int32_t)((raw + J.thresh) | dg);
-            J.den_act[dl] = (uint8_t)activity(J.den_act[dl], active, rnd, seed, DEN_ACT_DECAY);
-            J.den_raw[dl] = (uint8_t)raw_o;
-            if (J.ssc_energy != nullptr && dg < J.C) {
-                // ssc_cycle (cl/bismit.cl:849-884) on soma index dg == dendrite dg
-                const uint32_t e = J.ssc_energy[dg];
-                const uint32_t restless = (e > 191) & (st_o != 0);
-                const uint32_t dark = e == 255;
-                if (dark | restless) J.ssc_energy[dg] = (uint8_t)(e - 64);
-                st_o = min(st_o + 255u * restless + 255u * dark, 255u);
-            }
-            J.den_st[dl] = (uint8_t)st_o;
-            if (do_tft) {
-                denc[x] = (uint8_t)raw_o;
-                denc[nd_t + x] = (uint8_t)st_o;
-            }
-        }
-
-        // ---- tufts (tft_cycle)
-        if (do_tft) {
-            __syncthreads();
-            const uint32_t l2 = J.l2;
-            for (uint32_t ct = tid; ct < cells_t; ct += nth) {
-                uint32_t best_id = 0, best_raw = 0, best = 0, asum = 0;
-                for (uint32_t j = 0; j < dens; j++) {
-                    const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
-                    if (sr > best_raw) { best_id = j; best_raw = sr; }
-                    best = max(best, s);
-                    asum += s;
-                }
-                const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
-                const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
-                const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
-                const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
-                J.tft[4][cel] = J.tft[0][cel];
-                J.tft[5][cel] = J.tft[1][cel];
-                J.tft[6][cel] = J.tft[2][cel];
-                J.tft[7][cel] = J.tft[3][cel];
-                J.tft[0][cel] = (uint8_t)best_id;
-                J.tft[1][cel] = (uint8_t)best_raw;
-                J.tft[2][cel] = (uint8_t)best;
-                J.tft[3][cel] = (uint8_t)tstate;
-            }
-        }
-        __syncthreads();
+        tuft_tail<W>(J, lds, part, stage_cap, n_chunks, depth, tvn, tun, v0, u0, VU, U, CPT, rnd,
+                  tid, nth);
     }
 }
 
@@ -556,13 +691,44 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
         const SliceRt& s = a->slices_h[sid];
         if (j.scalar_quirk && (s.v_mid || s.u_mid)) continue;
         j.staged[n] = sid;
-        j.st_vsize[n] = s.v_size;
-        j.st_usize[n] = s.u_size;
-        j.st_vsc[n] = s.v_scale;
-        j.st_usc[n] = s.u_scale;
         n++;
     }
     j.n_staged = n;
+    std::sort(j.staged, j.staged + n);
+    for (uint32_t k = 0; k < n; k++) {
+        const SliceRt& s = a->slices_h[j.staged[k]];
+        j.st_vsize[k] = s.v_size;
+        j.st_usize[k] = s.u_size;
+        j.st_vsc[k] = s.v_scale;
+        j.st_usc[k] = s.u_scale;
+    }
+    // Check-free gather: every source slice present in the tuft is staged, the
+    // staged slices are consecutive ids of one geometry, and the window of the
+    // largest offset stays small enough for 8-bit dot-product coefficients.
+    bool fast = n > 0 && T.reach_max <= 40;
+    for (uint32_t w = 0; w < 4 && fast; w++) {
+        uint64_t bits = T.slc_bits[w];
+        while (bits && fast) {
+            const uint32_t id = w * 64 + (uint32_t)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            bool staged = false;
+            for (uint32_t k = 0; k < n; k++) staged |= j.staged[k] == id;
+            fast = staged;
+        }
+    }
+    for (uint32_t k = 1; k < n && fast; k++) {
+        const SliceRt& s0 = a->slices_h[j.staged[0]];
+        const SliceRt& s = a->slices_h[j.staged[k]];
+        fast = j.staged[k] == j.staged[0] + k && s.v_size == s0.v_size && s.u_size == s0.u_size &&
+               s.v_scale == s0.v_scale && s.u_scale == s0.u_scale && s.v_mid == s0.v_mid &&
+               s.u_mid == s0.u_mid;
+    }
+    if (fast) {
+        const uint32_t cmax = (((tn.tile_u - 1) * j.st_usc[0]) >> 4) + 2 + 2 * T.reach_max;
+        fast = cmax <= 127;
+    }
+    j.fast = fast ? 1u : 0u;
+    if (fast) j.halo = T.reach_max;
     return j;
 }
 
@@ -648,7 +814,12 @@ PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3) {
     return j;
 }
 
-static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU) {
+static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fast) {
+    if (fast) {
+        const uint32_t r = (((TV - 1) * j.st_vsc[0]) >> 4) + 2 + 2 * j.halo;
+        const uint32_t c = (((TU - 1) * j.st_usc[0]) >> 4) + 2 + 2 * j.halo;
+        return j.n_staged * r * c;
+    }
     uint32_t tot = 0;
     for (uint32_t k = 0; k < j.n_staged; k++) {
         uint32_t r = std::min<uint32_t>(j.st_vsize[k], (((TV - 1) * j.st_vsc[k]) >> 4) + 2 + 2 * j.halo);
@@ -662,10 +833,13 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st) {
     uint32_t W = 16, nst = 0;
+    bool fast = true;
     for (uint32_t k = 0; k < job_ids.size(); k++) {
         W = std::min(W, job_w[k]);
         nst = std::max(nst, jobs[job_ids[k]].n_staged);
+        fast &= jobs[job_ids[k]].fast != 0;
     }
+    fast &= W >= 4;
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
     uint32_t cap = 0, lds = 0;
     for (;;) {
@@ -673,7 +847,7 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
         uint32_t chunks = 0, denc = 0;
         for (uint32_t id : job_ids) {
             const TuftJob& j = jobs[id];
-            cap = std::max(cap, stage_bytes(j, TV, TU));
+            cap = std::max(cap, stage_bytes(j, TV, TU, fast));
             chunks = std::max(chunks, j.depth * TV * TU * (j.spt / W));
             if (j.tft[0]) denc = std::max(denc, 2 * j.depth * TV * TU * j.dens);
         }
@@ -698,6 +872,8 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     st.lds_bytes = lds;
     st.W = W;
     st.nst = nst <= 1 ? 1 : (nst <= 2 ? 2 : 4);
+    st.fast = fast ? 1u : 0u;
+    if (fast) st.nst = nst <= 1 ? 1 : 2;
     st.stage_cap = cap;
     st.tv = TV;
     st.tu = TU;
@@ -826,17 +1002,31 @@ int build_plan(bsm_area* a) {
     return 0;
 }
 
-#define F1_CASE(w, n)                                                                       \
-    case (w * 8 + n):                                                                       \
-        kf_tuft<w, n><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par, rnd_ofs, \
-                                                         st.tv, st.tu, st.stage_cap);       \
+#define F1_CASE(w, n)                                                                        \
+    case (w * 8 + n):                                                                        \
+        kf_tuft<w, n, false><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par,    \
+                                                                rnd_ofs, st.tv, st.tu,      \
+                                                                st.stage_cap);              \
+        break;
+#define F1_FAST_CASE(w, n)                                                                   \
+    case (w * 8 + n):                                                                        \
+        kf_tuft<w, n, true><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par,     \
+                                                               rnd_ofs, st.tv, st.tu,       \
+                                                               st.stage_cap);               \
         break;
 
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs) {
     if (st.n_items == 0) return 0;
     const WorkItem* it = items + st.item0;
-    if (st.kind == 1) {
+    if (st.kind == 1 && st.fast) {
+        switch (st.W * 8 + st.nst) {
+            F1_FAST_CASE(16, 1) F1_FAST_CASE(16, 2)
+            F1_FAST_CASE(8, 1) F1_FAST_CASE(8, 2)
+            F1_FAST_CASE(4, 1) F1_FAST_CASE(4, 2)
+        default: return fail("bad fast F1 variant");
+        }
+    } else if (st.kind == 1) {
         switch (st.W * 8 + st.nst) {
             F1_CASE(16, 1) F1_CASE(16, 2) F1_CASE(16, 4)
             F1_CASE(8, 1) F1_CASE(8, 2) F1_CASE(8, 4)

@@ -49,7 +49,7 @@ struct TuftJob {
     uint32_t scalar_quirk; // syns_per_tft % 4 != 0: char-wrapped mids (scalar kernel)
     uint32_t halo;         // staged reach around the tile, in source-slice columns
     uint32_t par_xor;      // state buffer written = st[(par ^ par_xor) & 1]
-    uint32_t _pad;
+    uint32_t fast;         // every synapse sources a staged slice within `halo`
     const uint32_t* dyn;   // dynamic batches: per (area, layer) word, bit0 parity, bit1 active
     // host-side planning copies of the staged slices' geometry
     uint16_t st_vsize[4], st_usize[4];
@@ -112,6 +112,12 @@ struct TuftRt {
     bsm_tuft_desc d;
     uint32_t spt;
     uint64_t syn_idz, den_idz;
+    // Host-side summary of the tuft's synapse sources, kept conservative on
+    // every write (never shrinks): the largest |v/u offset| and the set of
+    // source slice ids present.  Lets the planner prove that a staged LDS
+    // window covers every synapse of the tuft.
+    uint32_t reach_max;
+    uint64_t slc_bits[4];
 };
 
 struct LayerRt {
@@ -145,6 +151,7 @@ struct Stage {
     uint32_t layer;      // plain form: layer whose state parity flips at F1
     uint32_t W;          // synapse chunk width (bytes per lane)
     uint32_t nst;        // F1: staged-slice template (1, 2 or 4)
+    uint32_t fast;       // F1: all jobs take the check-free gather path
     uint32_t stage_cap;  // F1: LDS bytes reserved for staged windows
     uint32_t tv, tu;     // tile extents in columns
     uint32_t grid;
@@ -190,6 +197,7 @@ struct bsm_area {
     bool plan_dirty;
     hipEvent_t ev0, ev1;
     hipStream_t batch_stream;  // non-null while the area belongs to a batch
+    uint32_t layout_epoch;     // bumped whenever the synapse source summary changes
 };
 
 namespace bsm {
@@ -213,6 +221,10 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs);
 int ensure_rnd(bsm_area* a, size_t n_ints);
+void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
+                        uint32_t n_ids);
+void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len,
+                         const uint8_t* data, int fill_val);
 int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n,
                   uint32_t n_src, const uint8_t* src_ids, int32_t reach, uint8_t* slc, int8_t* v,
                   int8_t* u);

@@ -210,3 +210,22 @@ def test_buffer_io_roundtrip(ctx):
     with pytest.raises(L.BsmError):
         g.read("SYN_STATES", 1, offset=10**9, length=1)
     g.close()
+
+
+def test_cycle_generic_gather_paths(ctx):
+    """Tufts the host cannot prove covered by the staged LDS window take the
+    checked gather: a synapse sourcing a slice outside its tuft's sources, and
+    offsets beyond the check-free reach."""
+    scheme = recipes.cortical_area("g", 48, pyr_depth=2, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                                   pyr_spd=16, inhib_radius=3, thresholds=(200, 0, 300))
+    g, o = make_pair(ctx, scheme, reach=6)
+    # SSC synapse 5 sources a pyramidal slice; pyramidal distal offsets reach 45
+    for side in (g, o):
+        side.write("SYN_SRC_SLC_IDS", 0, np.array([3], dtype=np.uint8), offset=5)
+    n1 = scheme.cells(1) * 64
+    far = np.random.default_rng(2).integers(-45, 46, n1).astype(np.int8)
+    for side in (g, o):
+        side.write("SYN_SRC_COL_V_OFFS", 1, far, offset=n1)
+    frames = inputs.SdrStream(48 * 48, 0.04, 0.5, seed=5).frames(20)
+    run_cycles(g, o, scheme, frames, 20, tag="generic")
+    g.close()
