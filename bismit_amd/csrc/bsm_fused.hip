@@ -73,30 +73,25 @@ struct Chunk<1> {
 // Dendrites (den_cycle_tft, ssc_cycle on the SSC soma) and tufts (tft_cycle) of
 // one tile from the per-chunk (sum, raw) partials in LDS; ends with a barrier.
 template <int W>
-__device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* part,
-                                 uint32_t stage_cap, uint32_t n_chunks, uint32_t depth,
-                                 uint32_t tvn, uint32_t tun, uint32_t v0, uint32_t u0,
-                                 uint32_t VU, uint32_t U, uint32_t CPT, int32_t rnd, uint32_t tid,
-                                 uint32_t nth) {
+__device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* acc,
+                                 uint32_t stage_cap, uint32_t depth, uint32_t tvn, uint32_t tun,
+                                 uint32_t v0, uint32_t u0, uint32_t VU, uint32_t U, int32_t rnd,
+                                 uint32_t tid, uint32_t nth) {
     // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
-    const uint32_t dens = J.dens, CPD = J.spd / W;
+    const uint32_t dens = J.dens;
     const uint32_t cells_t = depth * tvn * tun;
     const uint32_t nd_t = cells_t * dens;
     const bool do_tft = J.tft[0] != nullptr;
-    uint8_t* denc = lds + stage_cap + n_chunks * 4;
+    uint8_t* denc = lds + stage_cap + nd_t * 4;
     auto den_act = BSM_G(uint8_t, J.den_act);
     auto den_raw = BSM_G(uint8_t, J.den_raw);
     auto den_st = BSM_G(uint8_t, J.den_st);
     auto energy = BSM_G(uint8_t, J.ssc_energy);
     for (uint32_t x = tid; x < nd_t; x += nth) {
         const uint32_t ct = x / dens, dj = x - ct * dens;
-        const uint32_t q0 = ct * CPT + dj * CPD;
-        uint32_t sum = 0, raw = 0;
-        for (uint32_t m = 0; m < CPD; m++) {
-            uint32_t pv = part[q0 + m];
-            sum += pv & 0xFFFFu;
-            raw += pv >> 16;
-        }
+        const uint32_t pv = acc[x];
+        uint32_t sum = pv & 0xFFFFu;
+        const uint32_t raw = pv >> 16;
         const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
         const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
         const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
@@ -155,6 +150,27 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
     __syncthreads();
 }
 
+// Adds one chunk's per-word (sum, raw) partials to the per-dendrite packed
+// accumulators (sum in bits 0-15, raw in bits 16-31; a dendrite's raw sum is at
+// most 256*255 < 2^16, so the halves never carry into each other).  A chunk
+// inside one dendrite takes one atomic; otherwise each 4-synapse word (never
+// split across dendrites) takes its own.  dj = off / spd via a multiply-high
+// with magic = ceil(2^32 / spd), exact since off * spd < 2^32.
+template <int NW, int NB>
+__device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, const uint32_t* pw,
+                                 bool whole, uint32_t magic) {
+    if (whole) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) tot += pw[w];
+        atomicAdd(&acc[den0 + __umulhi(off0, magic)], tot);
+    } else {
+#pragma unroll
+        for (int w = 0; w < NW; w++)
+            atomicAdd(&acc[den0 + __umulhi(off0 + (uint32_t)(w * NB), magic)], pw[w]);
+    }
+}
+
 template <int W, int NST, bool FAST>
 __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                                                const WorkItem* __restrict__ items, uint32_t n_items,
@@ -164,7 +180,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
     constexpr int NB = W < 4 ? W : 4;               // bytes per word
     constexpr int NW = Chunk<W>::NW;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
-    uint32_t* part = (uint32_t*)(lds + stage_cap);
+    uint32_t* acc = (uint32_t*)(lds + stage_cap);
 
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem wi = items[it];
@@ -179,6 +195,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             p = d & 1u;
         }
         const bool quirk = J.scalar_quirk != 0;
+        const uint32_t nd_t = depth * tvn * tun * J.dens;
+        for (uint32_t x = tid; x < nd_t; x += nth) acc[x] = 0u;
+        const bool whole = (J.spd % W) == 0;
+        const uint32_t magic = 0xFFFFFFFFu / J.spd + 1u;
 
         if constexpr (FAST) {
             // Every synapse of the job sources a staged slice with |offsets| <= halo
@@ -216,11 +236,13 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             const int32_t coefA = (nc & 0xFF) | (1 << 8);
             const int32_t coefB = ((nc & 0xFF) << 16) | (1 << 24);
             const int32_t sid0ws = (int32_t)(J.staged[0] * WS);
-            auto pos = [&](uint32_t q, size_t& so, int32_t& lb) {
+            auto pos = [&](uint32_t q, size_t& so, int32_t& lb, uint32_t& den0, uint32_t& off0) {
                 const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
                 const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
                 const uint32_t cc = rem / CPT, k = rem - cc * CPT;
                 const uint32_t v = v0 + r, u = u0 + cc;
+                den0 = (rowq * tun + cc) * J.dens;
+                off0 = k * W;
                 so = (size_t)(sl * VU + v * U + u) * spt + (size_t)k * W;
                 lb = ((int32_t)((v * s0.v_scale) >> 4) + (int32_t)s0.v_mid - a0) * nc +
                      (int32_t)((u * s0.u_scale) >> 4) + (int32_t)s0.u_mid - b0;
@@ -230,8 +252,9 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             uint32_t q = tid;
             size_t so = 0;
             int32_t lb = 0;
+            uint32_t den0 = 0, off0 = 0;
             if (q < n_chunks) {
-                pos(q, so, lb);
+                pos(q, so, lb, den0, off0);
                 Chunk<W>::load(J.slc + so, ms);
                 Chunk<W>::load(J.vofs + so, mv);
                 Chunk<W>::load(J.uofs + so, mu);
@@ -242,16 +265,16 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 const uint32_t qn = q + nth;
                 size_t so_n = 0;
                 int32_t lb_n = 0;
+                uint32_t den0_n = 0, off0_n = 0;
                 uint32_t ns[NW], nv[NW], nu[NW], nt[NW];
                 if (qn < n_chunks) {
-                    pos(qn, so_n, lb_n);
+                    pos(qn, so_n, lb_n, den0_n, off0_n);
                     Chunk<W>::load(J.slc + so_n, ns);
                     Chunk<W>::load(J.vofs + so_n, nv);
                     Chunk<W>::load(J.uofs + so_n, nu);
                     Chunk<W>::load(J.str + so_n, nt);
                 }
-                uint32_t out[NW];
-                uint32_t psum = 0, praw = 0;
+                uint32_t out[NW], pw[NW];
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
                     // [vo_b, uo_b, vo_b+1, uo_b+1] byte pairs -> dot4 gives lb + vo*cols + uo
@@ -273,12 +296,12 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
                     const uint32_t stw = ((t | x) & 0x80808080u) | ((x >> 1) & 0x7F7F7F7Fu);
                     out[w] = stw;
-                    praw = __builtin_amdgcn_sad_u8(stw, 0u, praw);
                     const uint32_t neg = ((mt[w] & 0x80808080u) >> 7) * 0xFFu;
-                    psum = __builtin_amdgcn_sad_u8(stw & ~neg, 0u, psum);
+                    pw[w] = __builtin_amdgcn_sad_u8(stw & ~neg, 0u, 0u) |
+                            (__builtin_amdgcn_sad_u8(stw, 0u, 0u) << 16);
                 }
                 Chunk<W>::store(st_out + so, out);
-                part[q] = psum | (praw << 16);
+                acc_chunk<NW, 4>(acc, den0, off0, pw, whole, magic);
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
                     ms[w] = ns[w];
@@ -288,10 +311,11 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 }
                 so = so_n;
                 lb = lb_n;
+                den0 = den0_n;
+                off0 = off0_n;
             }
             __syncthreads();
-            tuft_tail<W>(J, lds, part, stage_cap, n_chunks, depth, tvn, tun, v0, u0, VU, U, CPT,
-                      rnd, tid, nth);
+            tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth);
             continue;
         }
 
@@ -352,10 +376,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             Chunk<W>::load(J.vofs + so, mv);
             Chunk<W>::load(J.uofs + so, mu);
             Chunk<W>::load(J.str + so, mt);
-            uint32_t psum = 0, praw = 0;
+            uint32_t pw[NW];
 #pragma unroll
             for (int w = 0; w < NW; w++) {
-                uint32_t ow = 0;
+                uint32_t ow = 0, psum = 0, praw = 0;
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
                     const uint32_t sl_id = byte_of(ms[w], b);
@@ -380,14 +404,14 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     psum += (sbyte_of(mt[w], b) >= 0) ? stv : 0u;
                 }
                 out[w] = ow;
+                pw[w] = psum | (praw << 16);
             }
             Chunk<W>::store(st_out + so, out);
-            part[q] = psum | (praw << 16);
+            acc_chunk<NW, NB>(acc, (rowq * tun + cc) * J.dens, k * W, pw, whole, magic);
         }
         __syncthreads();
 
-        tuft_tail<W>(J, lds, part, stage_cap, n_chunks, depth, tvn, tun, v0, u0, VU, U, CPT, rnd,
-                  tid, nth);
+        tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth);
     }
 }
 
@@ -639,9 +663,12 @@ static uint32_t pick_w(uint64_t a, uint64_t b, uint64_t c, uint32_t max_w) {
     return 1;
 }
 
+// Chunk width of a tuft: aligned to the tuft base and the cell stride; a
+// 4-byte word must not straddle two dendrites.
 uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w) {
     const TuftRt& T = a->L[l].t[t];
-    return pick_w(T.d.syns_per_den, T.syn_idz, T.spt, max_w);
+    const uint32_t w = pick_w(T.syn_idz, T.spt, 1, max_w);
+    return (T.d.syns_per_den % 4 == 0) ? w : 1u;
 }
 
 uint32_t ssc_chunk(const bsm_area* a, uint32_t l) { return pick_w(a->L[l].t[0].spt, 0, 0, 16); }
@@ -844,15 +871,15 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     uint32_t cap = 0, lds = 0;
     for (;;) {
         cap = 0;
-        uint32_t chunks = 0, denc = 0;
+        uint32_t accb = 0, denc = 0;
         for (uint32_t id : job_ids) {
             const TuftJob& j = jobs[id];
             cap = std::max(cap, stage_bytes(j, TV, TU, fast));
-            chunks = std::max(chunks, j.depth * TV * TU * (j.spt / W));
+            accb = std::max(accb, 4 * j.depth * TV * TU * j.dens);
             if (j.tft[0]) denc = std::max(denc, 2 * j.depth * TV * TU * j.dens);
         }
         cap = (cap + 15) & ~15u;
-        lds = cap + chunks * 4 + denc;
+        lds = cap + accb + denc;
         if (lds <= LDS_LIMIT) break;
         if (TV == 1 && TU == 1) return fail("fused tuft kernel does not fit in LDS");
         if (TV >= TU && TV > 1) TV = (TV + 1) / 2;
