@@ -43,6 +43,20 @@ def stage_bytes(scheme, li, kind):
     return None
 
 
+def batch_stages(schemes):
+    """(kind, [(area, layer)]) of every persistent launch of a static batch in
+    launch order: spatial layers then pyramidal ones, by rank within the area;
+    per rank one F1 launch, then F2 (spatial) or F3 (pyramidal)."""
+    out = []
+    for spatial in (True, False):
+        per = [[li for li, l in enumerate(s.layers) if l.spatial == spatial] for s in schemes]
+        for k in range(max(len(p) for p in per)):
+            mem = [(ai, p[k]) for ai, p in enumerate(per) if k < len(p)]
+            out.append((1, mem))
+            out.append((2 if spatial else 3, mem))
+    return out
+
+
 def read_traffic():
     """HBM bytes per dominant-kernel launch from the committed rocprof PMC summary."""
     try:
@@ -136,6 +150,23 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
     times = [b.time_cycles(n_cycles) for _ in range(runs)]
     ms = statistics.median(times)
     batched = n_syn * n_cycles / (ms / 1000.0)
+    # per-stage device time (HIP events around each persistent launch)
+    n_prof = 30
+    prof = b.profile_stages(n_prof)
+    kinds = {1: "kf_tuft", 2: "kf_ssc_post", 3: "kf_pyr_post"}
+    stages = []
+    plan = b.stage_plan()
+    for (k, t), (k2, mem), p in zip(prof, batch_stages(schemes), plan):
+        assert k == k2 == p["kind"]
+        st = {"kernel": kinds[k], "layers": len(mem), "avg_us": round(1000.0 * t / n_prof, 2),
+              "chunk": p["chunk"], "fast": p["fast"], "tile": [p["tile_v"], p["tile_u"]],
+              "items": p["n_items"]}
+        if k == 1:
+            alg = sum(stage_bytes(schemes[ai], li, 1) for ai, li in mem)
+            st["alg_bytes_per_launch"] = alg
+            st["hbm_frac"] = round(alg / (t / n_prof / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+        stages.append(st)
+    cyc_alg = sum(st.get("alg_bytes_per_launch", 0) for st in stages)
     b.close()
     for a in areas:
         a.close()
@@ -146,7 +177,9 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
             "synapses": n_syn, "batched_synapse_updates_per_s": batched,
             "plain_per_area_synapse_updates_per_s": plain, "speedup_vs_plain": batched / plain,
             "batched_ms_per_cycle": ms / n_cycles, "plain_ms_per_cycle": 1000 * plain_s / n_plain,
-            "tuning": best, "hbm_frac_at_5B_per_synapse": batched * 5 / (HBM_PEAK_GBS * 1e9)}
+            "tuning": best, "stages": stages,
+            "hbm_frac_cycle": cyc_alg * n_cycles / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
+            "hbm_frac_at_5B_per_synapse": batched * 5 / (HBM_PEAK_GBS * 1e9)}
 
 
 def serving_leg(ctx, wall_s=20.0, rate=200.0, budget_gb=96, max_sessions=64):
@@ -173,6 +206,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling helper: run only the 64-area batched leg and print it")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="profiling helper: one copy-kernel calibration launch, then 3 cycles")
     args = ap.parse_args()
@@ -186,6 +221,10 @@ def main():
     from bismit_amd.cortex import Context, CorticalArea
 
     ctx = Context(0)
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(ctx, n_cycles=args.steps, runs=args.runs)}),
+              flush=True)
+        return
     scheme = recipes.config2_area()
     area = CorticalArea(scheme, ctx, rnd_seed=0, layout_seed=0, reach=6)
     in_ofs, in_len = scheme.input_range()
@@ -218,8 +257,11 @@ def main():
     # roofline of the dominant kernel: HIP events around every launch
     prof = area.profile_stages(50)
     kinds = {1: "kf_tuft", 2: "kf_ssc_post", 3: "kf_pyr_post"}
+    plan = area.stage_plan()
     stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name,
-               "avg_us": 1000.0 * t / 50} for k, li, t in prof]
+               "avg_us": round(1000.0 * t / 50, 2), "chunk": p["chunk"], "fast": p["fast"],
+               "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"]}
+              for (k, li, t), p in zip(prof, plan)]
     dom_i = max(range(len(prof)), key=lambda i: prof[i][2])
     dk, dl, dt = prof[dom_i]
     avg_s = dt / 50 / 1000.0

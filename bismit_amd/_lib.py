@@ -72,6 +72,17 @@ class Tuning(C.Structure):
                 ("chunk", C.c_uint32)]
 
 
+class StageInfo(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("layer", C.c_uint32), ("chunk", C.c_uint32),
+                ("staged", C.c_uint32), ("fast", C.c_uint32), ("tile_v", C.c_uint32),
+                ("tile_u", C.c_uint32), ("n_items", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("grid", C.c_uint32)]
+
+
+def stage_dicts(arr, n):
+    return [{f: getattr(arr[i], f) for f, _ in StageInfo._fields_} for i in range(n)]
+
+
 _P = C.c_void_p
 _U8P = C.POINTER(C.c_uint8)
 _I32P = C.POINTER(C.c_int32)
@@ -145,6 +156,12 @@ SIGNATURES = [
                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32]),
     ("bsm_batch_sync", C.c_int, [_P]),
     ("bsm_batch_time_cycles", C.c_int, [_P, C.c_uint32, C.c_int, C.POINTER(C.c_float)]),
+    ("bsm_area_stage_plan", C.c_int, [_P, C.c_uint32, C.POINTER(StageInfo),
+                                      C.POINTER(C.c_uint32)]),
+    ("bsm_batch_stage_plan", C.c_int, [_P, C.c_uint32, C.POINTER(StageInfo),
+                                       C.POINTER(C.c_uint32)]),
+    ("bsm_batch_profile_stages", C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
     ("bsm_batch_set_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
 ]
 

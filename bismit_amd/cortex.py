@@ -186,6 +186,13 @@ class CorticalArea:
         L.check(self.lib.bsm_area_profile_stages(self.h, n_cycles, m, ms, kind, lay, C.byref(n)))
         return [(kind[i], lay[i], ms[i]) for i in range(n.value)]
 
+    def stage_plan(self):
+        """The fused launches of one cycle: kernel variant and tiling of each."""
+        arr = (L.StageInfo * 64)()
+        n = C.c_uint32()
+        L.check(self.lib.bsm_area_stage_plan(self.h, 64, arr, C.byref(n)))
+        return L.stage_dicts(arr, n.value)
+
     def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0):
         t = L.Tuning(tile_v, tile_u, halo, chunk)
         L.check(self.lib.bsm_area_set_tuning(self.h, C.byref(t)))
@@ -300,6 +307,23 @@ class Batch:
         ms = C.c_float()
         L.check(self.lib.bsm_batch_time_cycles(self.h, n, 1 if graph else 0, C.byref(ms)))
         return ms.value
+
+    def stage_plan(self):
+        """The persistent launches of one batched cycle: variant and tiling of each."""
+        arr = (L.StageInfo * 64)()
+        n = C.c_uint32()
+        L.check(self.lib.bsm_batch_stage_plan(self.h, 64, arr, C.byref(n)))
+        return L.stage_dicts(arr, n.value)
+
+    def profile_stages(self, n_cycles):
+        """[(kind, total_ms)] of every persistent stage launch of a static batch
+        over n_cycles ungraphed cycles (which advance the areas), launch order."""
+        m = 64
+        ms = (C.c_float * m)()
+        kind = (C.c_int * m)()
+        n = C.c_uint32()
+        L.check(self.lib.bsm_batch_profile_stages(self.h, n_cycles, m, ms, kind, C.byref(n)))
+        return [(kind[i], ms[i]) for i in range(n.value)]
 
     def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0):
         t = L.Tuning(tile_v, tile_u, halo, chunk)

@@ -247,7 +247,9 @@ static int batch_plan(bsm_batch* b) {
     return 0;
 }
 
-static int enqueue_cycle(bsm_batch* b, uint32_t par) {
+// One batched cycle on the batch stream.  With ev, events ev[2i], ev[2i+1]
+// bracket stage i (never while capturing).
+static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
     const uint32_t na = (uint32_t)b->areas.size();
     if (b->n_tracts) {
         kb_tracts<<<b->n_tracts, 256, 0, b->stream>>>(b->tracts_d, b->n_tracts);
@@ -255,9 +257,13 @@ static int enqueue_cycle(bsm_batch* b, uint32_t par) {
     }
     kb_prologue<<<na, 256, 0, b->stream>>>(b->io_d, na, b->counter_d);
     BSM_HIP(hipGetLastError());
-    for (const Stage& st : b->stages)
-        if (launch_stage(b->stream, st, b->tjobs_d, b->sjobs_d, b->pjobs_d, b->items_d, par, 0))
+    for (size_t i = 0; i < b->stages.size(); i++) {
+        if (ev) BSM_HIP(hipEventRecord(ev[2 * i], b->stream));
+        if (launch_stage(b->stream, b->stages[i], b->tjobs_d, b->sjobs_d, b->pjobs_d, b->items_d,
+                         par, 0))
             return 1;
+        if (ev) BSM_HIP(hipEventRecord(ev[2 * i + 1], b->stream));
+    }
     kb_tick<<<1, 1, 0, b->stream>>>(b->counter_d);
     BSM_HIP(hipGetLastError());
     return 0;
@@ -287,7 +293,7 @@ static bool layouts_changed(const bsm_batch* b) {
     return false;
 }
 
-static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
+static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = nullptr) {
     if (!b->dirty && layouts_changed(b)) {
         // a layout write may have changed which gather path is exact: re-plan
         // against the areas' current parities
@@ -301,7 +307,7 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph) {
         b->bpar ^= 1u;
         if (use_graph) {
             BSM_HIP(hipGraphLaunch(b->exec[b->bpar], b->stream));
-        } else if (enqueue_cycle(b, b->bpar)) {
+        } else if (enqueue_cycle(b, b->bpar, ev ? ev + (size_t)2 * c * b->stages.size() : nullptr)) {
             return 1;
         }
     }
@@ -512,6 +518,53 @@ int bsm_batch_time_cycles(bsm_batch* b, uint32_t n_cycles, int use_graph, float*
     BSM_HIP(hipEventSynchronize(b->ev1));
     BSM_HIP(hipEventElapsedTime(ms_total, b->ev0, b->ev1));
     return 0;
+}
+
+int bsm_batch_profile_stages(bsm_batch* b, uint32_t n_cycles, uint32_t max_stages,
+                             float* stage_ms, int* stage_kind, uint32_t* n_stages) {
+    if (!b || !n_stages) return fail("null arg");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    if (b->dynamic) return fail("stage profiling needs a static batch");
+    if (!b->dirty && layouts_changed(b)) {
+        BSM_HIP(hipStreamSynchronize(b->stream));
+        if (bsm_batch_sync(b)) return 1;
+        b->dirty = true;
+    }
+    if (b->dirty && batch_plan(b)) return 1;
+    const uint32_t ns = (uint32_t)b->stages.size();
+    *n_stages = ns;
+    if (ns > max_stages) return fail("max_stages too small");
+    std::vector<hipEvent_t> ev((size_t)2 * ns * n_cycles);
+    for (auto& e : ev) BSM_HIP(hipEventCreate(&e));
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    int rc = run_cycles(b, n_cycles, 0, ev.data());
+    if (rc == 0) BSM_HIP(hipStreamSynchronize(b->stream));
+    for (uint32_t i = 0; i < ns && rc == 0; i++) {
+        float tot = 0;
+        for (uint32_t c = 0; c < n_cycles; c++) {
+            float ms = 0;
+            BSM_HIP(hipEventElapsedTime(&ms, ev[2 * ((size_t)c * ns + i)],
+                                        ev[2 * ((size_t)c * ns + i) + 1]));
+            tot += ms;
+        }
+        if (stage_ms) stage_ms[i] = tot;
+        if (stage_kind) stage_kind[i] = b->stages[i].kind;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int bsm_batch_stage_plan(bsm_batch* b, uint32_t max_stages, bsm_stage_info* out,
+                         uint32_t* n_stages) {
+    if (!b || !n_stages) return fail("null arg");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    if (!b->dirty && layouts_changed(b)) {
+        BSM_HIP(hipStreamSynchronize(b->stream));
+        if (bsm_batch_sync(b)) return 1;
+        b->dirty = true;
+    }
+    if (b->dirty && batch_plan(b)) return 1;
+    return stage_info(b->stages, max_stages, out, n_stages);
 }
 
 int bsm_batch_set_tuning(bsm_batch* b, const bsm_tuning* t) {

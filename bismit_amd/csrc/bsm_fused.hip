@@ -667,7 +667,7 @@ static uint32_t pick_w(uint64_t a, uint64_t b, uint64_t c, uint32_t max_w) {
 // 4-byte word must not straddle two dendrites.
 uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w) {
     const TuftRt& T = a->L[l].t[t];
-    const uint32_t w = pick_w(T.syn_idz, T.spt, 1, max_w);
+    const uint32_t w = pick_w(T.syn_idz, T.spt, 0, max_w);
     return (T.d.syns_per_den % 4 == 0) ? w : 1u;
 }
 
@@ -1042,6 +1042,27 @@ int build_plan(bsm_area* a) {
                                                                st.stage_cap);               \
         break;
 
+int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
+               uint32_t* n_stages) {
+    *n_stages = (uint32_t)stages.size();
+    if (stages.size() > max_stages) return fail("max_stages too small");
+    for (size_t i = 0; i < stages.size() && out; i++) {
+        const Stage& st = stages[i];
+        bsm_stage_info& o = out[i];
+        o.kind = st.kind;
+        o.layer = st.layer;
+        o.chunk = st.W;
+        o.staged = st.kind == 1 ? st.nst : 0;
+        o.fast = st.fast;
+        o.tile_v = st.tv;
+        o.tile_u = st.tu;
+        o.n_items = st.n_items;
+        o.lds_bytes = st.lds_bytes;
+        o.grid = st.grid;
+    }
+    return 0;
+}
+
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs) {
     if (st.n_items == 0) return 0;
@@ -1192,6 +1213,7 @@ int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages,
     if (!a || !n_stages) return fail("null arg");
     if (a->batch_stream) return fail("area belongs to a batch");
     BSM_HIP(hipSetDevice(a->ctx->device));
+    if (a->plan_dirty && build_plan(a)) return 1;
     if (upload_rnd(a, nullptr, n_cycles)) return 1;
     const uint32_t ns = (uint32_t)a->stages.size();
     *n_stages = ns;
@@ -1226,6 +1248,15 @@ int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages,
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return 0;
+}
+
+int bsm_area_stage_plan(bsm_area* a, uint32_t max_stages, bsm_stage_info* out,
+                        uint32_t* n_stages) {
+    if (!a || !n_stages) return fail("null arg");
+    if (a->batch_stream) return fail("area belongs to a batch");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    if (a->plan_dirty && build_plan(a)) return 1;
+    return stage_info(a->stages, max_stages, out, n_stages);
 }
 
 int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {

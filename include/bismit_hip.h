@@ -240,6 +240,30 @@ int bsm_area_time_cycles(bsm_area* area, uint32_t axn_offset, const uint8_t* dev
 int bsm_area_profile_stages(bsm_area* area, uint32_t n_cycles, uint32_t max_stages,
                             float* stage_ms, int* stage_kind, uint32_t* stage_layer,
                             uint32_t* n_stages);
+/* Per-stage device time of n_cycles cycles of a static batch, enqueued without
+ * graph capture (HIP events around every persistent launch on the batch
+ * stream).  stage_ms[i] is the total over the cycles, stage_kind[i] as for
+ * bsm_area_profile_stages.  The cycles advance the areas like bsm_batch_cycle. */
+int bsm_batch_profile_stages(bsm_batch* batch, uint32_t n_cycles, uint32_t max_stages,
+                             float* stage_ms, int* stage_kind, uint32_t* n_stages);
+/* The fused launch plan (planned now if stale): one entry per persistent launch
+ * of a cycle, in launch order.  Lets callers and tests see which kernel
+ * variant runs (chunk width, check-free gather) and its tiling. */
+typedef struct bsm_stage_info {
+    int32_t kind;        /* 1 gather+dendrites+tufts, 2 SSC post, 3 pyramidal post */
+    uint32_t layer;      /* plain form: the layer; batches: 0 */
+    uint32_t chunk;      /* synapses per lane chunk (W) */
+    uint32_t staged;     /* F1: staged-slice template */
+    uint32_t fast;       /* F1: check-free LDS gather */
+    uint32_t tile_v, tile_u;
+    uint32_t n_items;    /* work items (tiles) of the launch */
+    uint32_t lds_bytes;
+    uint32_t grid;       /* workgroups launched */
+} bsm_stage_info;
+int bsm_area_stage_plan(bsm_area* area, uint32_t max_stages, bsm_stage_info* out,
+                        uint32_t* n_stages);
+int bsm_batch_stage_plan(bsm_batch* batch, uint32_t max_stages, bsm_stage_info* out,
+                         uint32_t* n_stages);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 16): the known-size
  * reference for scaling HBM traffic counters. */
 int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);

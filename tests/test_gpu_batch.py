@@ -83,6 +83,58 @@ def test_static_batch_matches_plain(ctx, graph):
         ctx.free(d)
 
 
+def test_batch_profile_stages_advances_like_cycles(ctx):
+    """Profiled (event-bracketed, ungraphed) cycles are ordinary batch cycles."""
+    schemes = uneven_schemes()[:3]
+    n = 12
+    batched = make_areas(ctx, schemes, 700)
+    plain = make_areas(ctx, schemes, 700)
+    frames = [frames_for(s, n, 70 + i) for i, s in enumerate(schemes)]
+    devs = []
+    for f in frames:
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        devs.append(d)
+    b = Batch(batched, ctx)
+    b.set_inputs(devs, [f.shape[1] for f in frames], [f.shape[1] for f in frames],
+                 [s.input_range()[0] for s in schemes], n)
+    b.cycle(5, graph=True)
+    prof = b.profile_stages(4)
+    assert [k for k, _ in prof] == [1, 2, 1, 3]
+    assert all(t > 0 for _, t in prof)
+    b.cycle(3, graph=True)
+    b.sync()
+    b.close()
+    for i, a in enumerate(plain):
+        for c in range(n):
+            a.intake(frames[i][c])
+            a.cycle(1)
+    for i, s in enumerate(schemes):
+        assert_same(batched[i], plain[i], s, f"profiled area {i}")
+    for a in batched + plain:
+        a.close()
+    for d in devs:
+        ctx.free(d)
+
+
+def test_config3_plan_uses_fast_wide_variant(ctx):
+    """Every C3 tuft has synapses per dendrite a multiple of 4 and offsets within
+    6 columns: both persistent tuft launches must take the check-free gather at
+    16 synapses per lane."""
+    schemes = recipes.config3_areas(64, seed=0)
+    areas = make_areas(ctx, schemes, 1000)
+    b = Batch(areas, ctx)
+    plan = b.stage_plan()
+    print(plan, flush=True)
+    f1 = [p for p in plan if p["kind"] == 1]
+    assert len(f1) == 2
+    for p in f1:
+        assert p["chunk"] == 16 and p["fast"] == 1, p
+    b.close()
+    for a in areas:
+        a.close()
+
+
 def test_config3_batch_matches_plain(ctx):
     """The 64-area mix: 12 graph-captured batched cycles vs each area alone."""
     schemes = recipes.config3_areas(64, seed=0)

@@ -186,6 +186,12 @@ def test_cycle_config2_full_size(ctx):
     g, o = make_pair(ctx, scheme)
     frames = inputs.SdrStream(256 * 256, 0.02, 0.5, seed=0).frames(3)
     run_cycles(g, o, scheme, frames, 3, tag="c2")
+    # the path measured at this size: both tuft launches check-free, 16 synapses per lane
+    plan = g.stage_plan()
+    assert [p["kind"] for p in plan] == [1, 2, 1, 3]
+    for p in plan:
+        if p["kind"] == 1:
+            assert p["chunk"] == 16 and p["fast"] == 1, p
     g.close()
     o.close()
 

@@ -36,11 +36,12 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "bismit_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(bsm_slice_desc), sizeof(bsm_tuft_desc), sizeof(bsm_layer_desc),'
                    'sizeof(bsm_area_desc), offsetof(bsm_layer_desc, tufts),'
                    'offsetof(bsm_area_desc, layers), offsetof(bsm_tuft_desc, threshold),'
-                   'sizeof(bsm_tuning));return 0;}\n')
+                   'sizeof(bsm_tuning), sizeof(bsm_stage_info),'
+                   'offsetof(bsm_stage_info, grid));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -48,7 +49,8 @@ def test_struct_layouts_match_header(tmp_path):
                                           check=True).stdout.split()]
     want = [C.sizeof(L.SliceDesc), C.sizeof(L.TuftDesc), C.sizeof(L.LayerDesc),
             C.sizeof(L.AreaDesc), L.LayerDesc.tufts.offset, L.AreaDesc.layers.offset,
-            L.TuftDesc.threshold.offset, C.sizeof(L.Tuning)]
+            L.TuftDesc.threshold.offset, C.sizeof(L.Tuning), C.sizeof(L.StageInfo),
+            L.StageInfo.grid.offset]
     assert got == want
 
 
