@@ -180,7 +180,7 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
             T.syn_idz = ns;
             T.den_idz = nd;
             // zero-filled buffers: slice 0, offsets 0
-            T.reach_max = 0;
+            T.reach_v = T.reach_u = T.reach_max = 0;
             T.slc_bits[0] = 1;
             T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
             ns += (uint64_t)L.cells * T.spt;
@@ -231,15 +231,16 @@ int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, u
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids) {
     TuftRt& T = a->L[l].t[t];
-    T.reach_max = reach;
+    T.reach_v = T.reach_u = T.reach_max = reach;
     T.slc_bits[0] = T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
     for (uint32_t k = 0; k < n_ids; k++) T.slc_bits[ids[k] >> 6] |= 1ull << (ids[k] & 63);
     a->plan_dirty = true;
     a->layout_epoch++;
 }
 
-// Conservative update of the per-tuft source summary after a write (data) or a
-// fill (fill_val >= 0) of a synapse source buffer range.
+// Update of the per-tuft source summary after a write (data) or a fill
+// (fill_val >= 0) of a synapse source buffer range: a range covering the whole
+// tuft replaces that buffer's component, a partial range widens it.
 void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len,
                          const uint8_t* data, int fill_val) {
     if (buf != BSM_SYN_SRC_SLC_IDS && buf != BSM_SYN_SRC_COL_V_OFFS &&
@@ -251,15 +252,26 @@ void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, si
         const size_t z = T.syn_idz, n = (size_t)L.cells * T.spt;
         const size_t lo = std::max(offset, z), hi = std::min(offset + len, z + n);
         if (lo >= hi) continue;
-        for (size_t i = lo; i < hi; i++) {
+        const bool whole = lo == z && hi == z + n;
+        uint32_t& reach = buf == BSM_SYN_SRC_COL_V_OFFS ? T.reach_v : T.reach_u;
+        if (whole) {
+            if (buf == BSM_SYN_SRC_SLC_IDS)
+                T.slc_bits[0] = T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
+            else
+                reach = 0;
+        }
+        // a fill contributes its one value
+        const size_t end = data ? hi : lo + 1;
+        for (size_t i = lo; i < end; i++) {
             const uint8_t b = data ? data[i - offset] : (uint8_t)fill_val;
             if (buf == BSM_SYN_SRC_SLC_IDS) {
                 T.slc_bits[b >> 6] |= 1ull << (b & 63);
             } else {
                 const int32_t v = (int8_t)b;
-                T.reach_max = std::max<uint32_t>(T.reach_max, (uint32_t)(v < 0 ? -v : v));
+                reach = std::max<uint32_t>(reach, (uint32_t)(v < 0 ? -v : v));
             }
         }
+        T.reach_max = std::max(T.reach_v, T.reach_u);
         a->plan_dirty = true;
         a->layout_epoch++;
     }
@@ -581,13 +593,8 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
                                a->stream));
     } else {
         BSM_HIP(hipMemsetAsync(p, val, n, a->stream));
-        if (buf != BSM_AXN_STATES) {
-            // a whole-buffer fill replaces the summary of every tuft of the layer
-            for (uint32_t t = 0; t < a->L[layer].d.n_tufts; t++) {
-                if (buf == BSM_SYN_SRC_SLC_IDS) layout_stats_reset(a, layer, t, a->L[layer].t[t].reach_max, &val, 1);
-            }
-            layout_stats_update(a, buf, layer, 0, n, nullptr, val);
-        }
+        // a whole-buffer fill replaces the summary of every tuft of the layer
+        if (buf != BSM_AXN_STATES) layout_stats_update(a, buf, layer, 0, n, nullptr, val);
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;

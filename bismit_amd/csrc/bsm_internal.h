@@ -116,8 +116,11 @@ struct TuftRt {
     // every write (never shrinks): the largest |v/u offset| and the set of
     // source slice ids present.  Lets the planner prove that a staged LDS
     // window covers every synapse of the tuft.
-    uint32_t reach_max;
-    uint64_t slc_bits[4];
+    // Source summary of the tuft's synapses (bounds that only widen on partial
+    // writes, exact after a write or fill of the whole tuft range):
+    uint32_t reach_v, reach_u;  // max |v offset|, max |u offset|
+    uint32_t reach_max;         // max(reach_v, reach_u)
+    uint64_t slc_bits[4];       // set of source slice ids
 };
 
 struct LayerRt {

@@ -123,6 +123,8 @@ def test_cycle_config1_1000_cycles(ctx):
     learning on, 1000 cycles, every buffer compared after every cycle."""
     scheme = recipes.config1_area()
     g, o = make_pair(ctx, scheme, layout_seed=0)
+    # host-written layouts within 6 columns: both tuft launches check-free
+    assert [(p["kind"], p["fast"]) for p in g.stage_plan()] == [(1, 1), (2, 0), (1, 1), (3, 0)]
     frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=0).frames(1000)
     run_cycles(g, o, scheme, frames, 1000, tag="c1")
     # the run did learn and fire
@@ -139,8 +141,8 @@ def test_cycle_config1_1000_cycles(ctx):
     dict(tile_v=3, tile_u=5, halo=2, chunk=1),
 ])
 def test_cycle_tunings(ctx, tuning):
-    """Any tile/halo/chunk choice gives identical results (small halos force
-    the out-of-window global gather path)."""
+    """Any tile/halo/chunk choice gives identical results (chunk 1 forces the
+    checked gather, whose small halos take the out-of-window global path)."""
     print("tuning", tuning, flush=True)
     scheme = recipes.cortical_area("t", 40, input_side=32, pyr_depth=3, ssc_dens=4, ssc_spd=16,
                                    pyr_dens=4, pyr_spd=24, inhib_radius=4)
