@@ -268,10 +268,12 @@ def main():
     alg = stage_bytes(scheme, dl, dk)
     achieved = alg / avg_s / 1e9
     traffic = read_traffic()
+    ceil = stream_ceiling(ctx)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
-                "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2)}
+                "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
+                "stream_ceiling": ceil, "frac_of_stream_ceiling": round(achieved / ceil["gbs"], 4)}
 
     out = {
         "metric": "synapse-updates/s (cycle+LTP), batched vs plain on 1 GPU; p50/p99 frame latency",
@@ -304,6 +306,23 @@ def main():
                                            inputs.SdrStream(in_len, 0.02, 0.5, seed=0).frames(8))
     area.close()
     print(json.dumps(out), flush=True)
+
+
+def stream_ceiling(ctx, n=128 << 20, reps=5):
+    """Best of `reps` launches of the stream calibration kernel (F1's four read
+    streams and one write stream, no gather): a practical HBM ceiling."""
+    import ctypes as C
+    from bismit_amd import _lib as L
+    src, dst = ctx.alloc(4 * n), ctx.alloc(n)
+    best = None
+    for _ in range(reps):
+        ms = C.c_float()
+        L.check(ctx.lib.bsm_calib_stream(ctx.h, dst, src, n, C.byref(ms)))
+        best = ms.value if best is None else min(best, ms.value)
+    ctx.free(src)
+    ctx.free(dst)
+    return {"gbs": round(5 * n / (best / 1000.0) / 1e9, 1),
+            "pattern": f"4 non-temporal 16-B read streams + 1 16-B write stream, {n >> 20} MiB each"}
 
 
 def C_float_copy(ctx, dst, src, n):

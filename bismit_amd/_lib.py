@@ -137,6 +137,7 @@ SIGNATURES = [
                                           C.POINTER(C.c_int), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint32)]),
     ("bsm_calib_copy", C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_float)]),
+    ("bsm_calib_stream", C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_float)]),
     ("bsm_gen_tuft_layout", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
                                       C.c_uint32, _U8P, C.c_int32, _P, _P, _P]),
     ("bsm_area_init_layout", C.c_int, [_P, C.c_uint64, C.c_int32]),

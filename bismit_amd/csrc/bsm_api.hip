@@ -608,7 +608,7 @@ int bsm_area_intake(bsm_area* a, uint32_t axn_offset, const uint8_t* src, uint32
     if (!len) return 0;
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (src_is_device) {
-        BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, src, len, hipMemcpyDeviceToDevice, a->stream));
+        if (launch_intake(a->stream, a->axn + axn_offset, src, len)) return 1;
     } else {
         BSM_HIP(hipStreamSynchronize(a->stream));
         BSM_HIP(hipMemcpy(a->axn + axn_offset, src, len, hipMemcpyHostToDevice));

@@ -34,48 +34,51 @@ struct AreaIo {
     const uint8_t* stage;   // dynamic batches: staged frames of this cycle
 };
 
-// One block per area: thread 0 draws the rnd slots (xorshift128, Marsaglia
-// 2003), all threads copy the frame.
-__global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io, uint32_t n_areas,
+// Frame bytes copied by one prologue / tract block.
+constexpr uint32_t COPY_SEG = 16384;
+
+// Blocks (area, part): block (a, 0) draws area a's rnd slots (thread 0,
+// xorshift128, Marsaglia 2003) and updates its dynamic words; every block
+// copies part `part` (COPY_SEG bytes) of the area's frame.
+__global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io,
                                                    const uint32_t* __restrict__ counter) {
-    for (uint32_t a = blockIdx.x; a < n_areas; a += gridDim.x) {
-        const AreaIo& A = io[a];
-        if (A.dyn) {
-            // dynamic batch: only areas with a frame this cycle step; their layers
-            // flip parity and are marked active, the others are marked idle
-            const uint32_t active = *A.act;
-            if (threadIdx.x < BSM_MAX_LAYERS && ((A.layer_mask >> threadIdx.x) & 1u)) {
-                const uint32_t d = A.dyn[threadIdx.x];
-                A.dyn[threadIdx.x] = active ? (((d & 1u) ^ 1u) | 2u) : (d & 1u);
-            }
-            if (!active) continue;
+    const AreaIo A = io[blockIdx.x];
+    const uint32_t part = blockIdx.y;
+    if (A.dyn) {
+        // dynamic batch: only areas with a frame this cycle step; their layers
+        // flip parity and are marked active, the others are marked idle
+        const uint32_t active = *A.act;
+        if (part == 0 && threadIdx.x < BSM_MAX_LAYERS && ((A.layer_mask >> threadIdx.x) & 1u)) {
+            const uint32_t d = A.dyn[threadIdx.x];
+            A.dyn[threadIdx.x] = active ? (((d & 1u) ^ 1u) | 2u) : (d & 1u);
         }
-        if (threadIdx.x == 0) {
-            uint32_t x = A.rng[0], y = A.rng[1], z = A.rng[2], w = A.rng[3];
-            for (uint32_t k = 0; k < A.n_slots; k++) {
-                uint32_t t = x ^ (x << 11);
-                x = y;
-                y = z;
-                z = w;
-                w = w ^ (w >> 19) ^ (t ^ (t >> 8));
-                A.rnd[k] = (int32_t)w;
-            }
-            A.rng[0] = x;
-            A.rng[1] = y;
-            A.rng[2] = z;
-            A.rng[3] = w;
-        }
-        if (A.dyn) {
-            const uint64_t fo = *A.frame_ofs;
-            if (A.len && fo != ~0ull) {
-                const uint8_t* src = A.stage + fo;
-                for (uint32_t i = threadIdx.x; i < A.len; i += blockDim.x) A.dst[i] = src[i];
-            }
-        } else if (A.len && A.n_frames) {
-            const uint8_t* src = A.frames + (size_t)(*counter % A.n_frames) * A.stride;
-            for (uint32_t i = threadIdx.x; i < A.len; i += blockDim.x) A.dst[i] = src[i];
-        }
+        if (!active) return;
     }
+    if (part == 0 && threadIdx.x == 0) {
+        uint32_t x = A.rng[0], y = A.rng[1], z = A.rng[2], w = A.rng[3];
+        for (uint32_t k = 0; k < A.n_slots; k++) {
+            uint32_t t = x ^ (x << 11);
+            x = y;
+            y = z;
+            z = w;
+            w = w ^ (w >> 19) ^ (t ^ (t >> 8));
+            A.rnd[k] = (int32_t)w;
+        }
+        A.rng[0] = x;
+        A.rng[1] = y;
+        A.rng[2] = z;
+        A.rng[3] = w;
+    }
+    const uint8_t* src = nullptr;
+    if (A.dyn) {
+        const uint64_t fo = *A.frame_ofs;
+        if (A.len && fo != ~0ull) src = A.stage + fo;
+    } else if (A.len && A.n_frames) {
+        src = A.frames + (size_t)(*counter % A.n_frames) * A.stride;
+    }
+    const uint32_t lo = part * COPY_SEG;
+    if (src && lo < A.len)
+        block_copy(A.dst + lo, src + lo, min(COPY_SEG, A.len - lo), threadIdx.x, blockDim.x);
 }
 
 __global__ void kb_tick(uint32_t* counter) { *counter += 1; }
@@ -87,12 +90,13 @@ struct TractRt {
     uint32_t _pad;
 };
 
-// One block per tract: copy the source area's axon range into the target's.
-__global__ __launch_bounds__(256) void kb_tracts(const TractRt* __restrict__ tr, uint32_t n) {
-    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
-        const TractRt T = tr[t];
-        for (uint32_t i = threadIdx.x; i < T.len; i += blockDim.x) T.dst[i] = T.src[i];
-    }
+// Blocks (tract, part): copy part `part` (COPY_SEG bytes) of the source
+// area's axon range into the target's.
+__global__ __launch_bounds__(256) void kb_tracts(const TractRt* __restrict__ tr) {
+    const TractRt T = tr[blockIdx.x];
+    const uint32_t lo = blockIdx.y * COPY_SEG;
+    if (lo < T.len)
+        block_copy(T.dst + lo, T.src + lo, min(COPY_SEG, T.len - lo), threadIdx.x, blockDim.x);
 }
 
 }  // namespace bsm
@@ -130,6 +134,8 @@ struct bsm_batch {
     int xfer_next = 0;
     bsm::TractRt* tracts_d = nullptr;
     uint32_t n_tracts = 0;
+    uint32_t io_parts = 1;      // prologue blocks per area (largest axon space / COPY_SEG)
+    uint32_t tract_parts = 1;   // tract blocks per tract (longest tract / COPY_SEG)
     std::vector<uint32_t> epochs;   // areas' layout epochs the plan was made for
 };
 
@@ -252,10 +258,10 @@ static int batch_plan(bsm_batch* b) {
 static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
     const uint32_t na = (uint32_t)b->areas.size();
     if (b->n_tracts) {
-        kb_tracts<<<b->n_tracts, 256, 0, b->stream>>>(b->tracts_d, b->n_tracts);
+        kb_tracts<<<dim3(b->n_tracts, b->tract_parts), 256, 0, b->stream>>>(b->tracts_d);
         BSM_HIP(hipGetLastError());
     }
-    kb_prologue<<<na, 256, 0, b->stream>>>(b->io_d, na, b->counter_d);
+    kb_prologue<<<dim3(na, b->io_parts), 256, 0, b->stream>>>(b->io_d, b->counter_d);
     BSM_HIP(hipGetLastError());
     for (size_t i = 0; i < b->stages.size(); i++) {
         if (ev) BSM_HIP(hipEventRecord(ev[2 * i], b->stream));
@@ -370,6 +376,8 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
         if (areas[i]->batch_stream) return fail("area already belongs to a batch");
         for (uint32_t k = 0; k < i; k++)
             if (areas[k] == areas[i]) return fail("duplicate area in batch");
+        if ((areas[i]->axn_len + COPY_SEG - 1) / COPY_SEG > 65535)
+            return fail("axon space too large for the batch prologue");
     }
     BSM_HIP(hipSetDevice(ctx->device));
     bsm_batch* b = new bsm_batch();
@@ -394,6 +402,7 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
         io.rnd = a->rnd_d;
         io.n_slots = a->n_slots;
         a->batch_stream = b->stream;
+        b->io_parts = std::max(b->io_parts, (uint32_t)((a->axn_len + COPY_SEG - 1) / COPY_SEG));
     }
     BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     for (int i = 0; i < 2; i++) BSM_HIP(hipEventCreateWithFlags(&b->xfer_ev[i], hipEventDisableTiming));
@@ -663,6 +672,7 @@ int bsm_batch_set_tracts(bsm_batch* b, const bsm_tract* tracts, uint32_t n) {
             t.dst_offset < t.src_offset + t.len)
             return fail("tract " + std::to_string(i) + ": overlapping ranges");
         rt[i] = TractRt{sa->axn + t.src_offset, da->axn + t.dst_offset, t.len, 0};
+        b->tract_parts = std::max(b->tract_parts, (t.len + COPY_SEG - 1) / COPY_SEG);
     }
     (void)hipFree(b->tracts_d);
     b->tracts_d = nullptr;
@@ -671,6 +681,7 @@ int bsm_batch_set_tracts(bsm_batch* b, const bsm_tract* tracts, uint32_t n) {
         BSM_HIP(hipMemcpy(b->tracts_d, rt.data(), sizeof(TractRt) * n, hipMemcpyHostToDevice));
     }
     b->n_tracts = n;
+    if (!n) b->tract_parts = 1;
     free_graphs(b);
     return 0;
 }

@@ -9,6 +9,9 @@
 
 namespace bsm {
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 // constants: src/cmn/mod.rs:323-336, cl/bismit.cl:16,22, cl/control.cl:6-13
 constexpr uint8_t CEL_PREV_ACTIVE = 0x08;
 constexpr uint8_t SYN_PREV_ACTIVE = 0x10;
@@ -103,6 +106,39 @@ __device__ __forceinline__ uint32_t cel_axn_idx(const SliceRt& s, uint32_t v, ui
     int32_t uu = (int32_t)((u * s.u_scale) >> 4) + uo;
     *safe = (vv >= 0) & (vv < (int32_t)s.v_size) & (uu >= 0) & (uu < (int32_t)s.u_size);
     return s.idz + (uint32_t)(vv * (int32_t)s.u_size + uu);
+}
+
+// Block-cooperative copy of len bytes: 16-byte vectors when both ends are
+// 16-byte aligned, four loads in flight per thread ahead of their stores.
+__device__ inline void block_copy(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src,
+                                  uint32_t len, uint32_t tid, uint32_t nth) {
+    auto d8 = BSM_G(uint8_t, dst);
+    auto s8 = BSM_G(const uint8_t, src);
+    uint32_t done = 0;
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15u) == 0) {
+        const uint32_t nv = len >> 4;
+        auto d = BSM_G(u32x4, dst);
+        auto s = BSM_G(const u32x4, src);
+        uint32_t i = tid;
+        for (; i + 3 * nth < nv; i += 4 * nth) {
+            const u32x4 a = s[i], b = s[i + nth], c = s[i + 2 * nth], e = s[i + 3 * nth];
+            d[i] = a;
+            d[i + nth] = b;
+            d[i + 2 * nth] = c;
+            d[i + 3 * nth] = e;
+        }
+        for (; i < nv; i += nth) d[i] = s[i];
+        done = nv << 4;
+    }
+    uint32_t i = done + tid;
+    for (; i + 3 * nth < len; i += 4 * nth) {
+        const uint8_t a = s8[i], b = s8[i + nth], c = s8[i + 2 * nth], e = s8[i + 3 * nth];
+        d8[i] = a;
+        d8[i + nth] = b;
+        d8[i + 2 * nth] = c;
+        d8[i + 3 * nth] = e;
+    }
+    for (; i < len; i += nth) d8[i] = s8[i];
 }
 
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int b) { return (w >> (8 * b)) & 0xFFu; }

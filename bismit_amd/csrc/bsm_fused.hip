@@ -26,8 +26,6 @@
 namespace bsm {
 
 // ------------------------------------------------------------------ F1
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 template <int W>
 struct Chunk;
@@ -465,6 +463,16 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         }
         uint32_t* list = (uint32_t*)(lds + ((WV * WU + 15) & ~15u));
         const uint8_t* soma = J.soma + (size_t)slc * VU;
+        // loads independent of the window, issued with it ahead of the barrier:
+        // rnd values, the own slice entry, this thread's first cell's rating
+        const int32_t rnd_c = J.mode == 0 ? J.rnd[rnd_ofs + J.slot_ctrl] : 0;
+        const int32_t rnd_m = J.learn ? J.rnd[rnd_ofs + J.slot_mtp] : 0;
+        const SliceRt own = J.slices[J.base_slc + slc];
+        uint32_t act0 = 0;
+        if (J.mode != 2 && tid < tvn * tun) {
+            const uint32_t tr = tid / tun, tc = tid - tr * tun;
+            act0 = BSM_G(const uint8_t, J.cel_act)[slc * VU + (v0 + tr) * U + (u0 + tc)];
+        }
         for (uint32_t x = tid; x < WV * WU; x += nth) {
             const uint32_t wr = x / WU, wc = x - wr * WU;
             const int32_t nv = (int32_t)v0 - r + (int32_t)wr, nu = (int32_t)u0 - r + (int32_t)wc;
@@ -475,9 +483,7 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         if (tid == 0) n_act = 0;
         __syncthreads();
 
-        const int32_t rnd_c = J.mode == 0 ? J.rnd[rnd_ofs + J.slot_ctrl] : 0;
         const int32_t imax = (r + INHIB_HORIZ_OFS) * (r + INHIB_HORIZ_OFS);
-        const SliceRt own = J.slices[J.base_slc + slc];
         for (uint32_t x = tid; x < tvn * tun; x += nth) {
             const uint32_t tr = x / tun, tc = x - tr * tun;
             const uint32_t v = v0 + tr, u = u0 + tc;
@@ -513,8 +519,9 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     is_act = cs != 0;
                 }
                 J.axn[axi] = (uint8_t)out;
-                J.cel_act[c] = (uint8_t)activity(J.cel_act[c], (int32_t)is_act, rnd_c,
-                                                 (int32_t)(cs | c), CEL_ACT_DECAY);
+                const uint32_t a_in = x == tid ? act0 : (uint32_t)J.cel_act[c];
+                J.cel_act[c] = (uint8_t)activity(a_in, (int32_t)is_act, rnd_c, (int32_t)(cs | c),
+                                                 CEL_ACT_DECAY);
             }
             if (J.learn && out) list[atomicAdd(&n_act, 1u)] = c;
         }
@@ -522,7 +529,6 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
 
         if (J.learn) {
             const uint32_t na = n_act, spt = J.spt, CPC = spt / W;
-            const int32_t rnd_m = J.rnd[rnd_ofs + J.slot_mtp];
             const uint8_t* __restrict__ st = J.st[p & 1];
             for (uint32_t y = tid; y < na * CPC; y += nth) {
                 const uint32_t ci = y / CPC, k = y - ci * CPC;
@@ -561,18 +567,25 @@ __device__ inline void dst_ltp_chunk(const uint8_t* __restrict__ prev, int8_t* _
 }
 
 constexpr uint32_t F3_COLS = 256;
+constexpr uint32_t F3_G = 4;  // slices whose per-column inputs are loaded ahead of any store
 
+// One work item = F3_COLS columns of one layer, one thread per column.  Pass 1:
+// pyr_cycle of every slice and the intra-column running best; pass 2, F3_G
+// slices at a time: column output (inhibition / passthrough), the distal
+// learning decisions (queued in LDS), then the queued dendrites' LTP by the
+// whole block.  Every load of a group is issued before its stores.
 template <int W>
 __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
                                                    uint32_t n_items, uint32_t par, uint32_t rnd_ofs) {
-    __shared__ uint32_t list[F3_COLS * BSM_MAX_TUFTS];
+    __shared__ uint32_t lz[F3_COLS * F3_G * BSM_MAX_TUFTS];  // queued dendrite: first synapse
+    __shared__ uint8_t lt[F3_COLS * F3_G * BSM_MAX_TUFTS];   // queued dendrite: its tuft
     __shared__ uint32_t n_l;
     const uint32_t tid = threadIdx.x;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem wi = items[it];
         const PyrJob& J = jobs[wi.job];
-        const uint32_t VU = J.V * J.U, C = J.C, depth = J.depth;
+        const uint32_t U = J.U, VU = J.V * U, C = J.C, depth = J.depth;
         const uint32_t col = (uint32_t)wi.tv * F3_COLS + tid;
         const bool valid = col < VU;
         uint32_t p = par ^ J.par_xor;
@@ -581,67 +594,116 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
             if (!(d & 2u)) continue;
             p = d & 1u;
         }
-        // pass 1: pyr_cycle for every slice of the column, running intra-column best
+        auto tft_st = BSM_G(const uint8_t, J.tft_states);
+        auto best_raw = BSM_G(const uint8_t, J.tft_best_raw);
+        auto pb_raw = BSM_G(const uint8_t, J.prev_best_raw);
+        auto pb_ids = BSM_G(const uint8_t, J.prev_best_ids);
+        auto soma = BSM_G(uint8_t, J.soma);
+        auto flags = BSM_G(uint8_t, J.cel_flags);
+        auto axn = BSM_G(uint8_t, J.axn);
+        const bool prx_on = (J.en & FLAG_BASAL_PROXIMAL) != 0;
+        const bool dst_on = (J.en & FLAG_BASAL_DISTAL) != 0;
+        const bool out_ctl = J.control == BSM_CTRL_INTRA_COLUMN || J.control == BSM_CTRL_PYR_OUTPUT;
+        const bool passthru = J.control == BSM_CTRL_PYR_OUTPUT;
+        const bool learn = J.learn && J.distal_mask;
+        const uint32_t dmask = J.distal_mask, n_tufts = J.n_tufts;
+        const uint32_t v = col / U, u = col - v * U;
+
+        // pass 1: pyr_cycle, running intra-column best (Q3 reset quirk)
+        uint32_t cs_r[F3_G] = {0, 0, 0, 0};
         uint32_t best = 0;
         if (valid) {
-            for (uint32_t s = 0; s < depth; s++) {
-                const uint32_t c = s * VU + col;
-                const uint32_t pp = (J.en & FLAG_BASAL_PROXIMAL) ? J.tft_states[J.prx * C + c] : 0u;
-                const uint32_t dd = (J.en & FLAG_BASAL_DISTAL) ? J.tft_best_raw[J.bdst * C + c] : 0u;
-                const uint32_t act = pp != 0;
-                const uint32_t cs = (pp >> 2) + ((pp <= 3) && act) + act * (dd >> 1);
-                J.soma[c] = (uint8_t)cs;
-                best = (cs >= best) ? cs : 0u;
+            for (uint32_t s0 = 0; s0 < depth; s0 += F3_G) {
+                uint32_t pp[F3_G], dd[F3_G];
+#pragma unroll
+                for (uint32_t g = 0; g < F3_G; g++) {
+                    const uint32_t c = (s0 + g) * VU + col;
+                    const bool in = s0 + g < depth;
+                    pp[g] = (in && prx_on) ? tft_st[J.prx * C + c] : 0u;
+                    dd[g] = (in && dst_on) ? best_raw[J.bdst * C + c] : 0u;
+                }
+#pragma unroll
+                for (uint32_t g = 0; g < F3_G; g++) {
+                    if (s0 + g >= depth) break;
+                    const uint32_t act = pp[g] != 0;
+                    const uint32_t cs = (pp[g] >> 2) + ((pp[g] <= 3) && act) + act * (dd[g] >> 1);
+                    soma[(s0 + g) * VU + col] = (uint8_t)cs;
+                    best = (cs >= best) ? cs : 0u;
+                    if (s0 == 0) cs_r[g] = cs;
+                }
             }
         }
-        // pass 2, per slice: axon, learning decisions, cooperative distal LTP
-        for (uint32_t s = 0; s < depth; s++) {
+
+        // pass 2: outputs and learning, F3_G slices at a time
+        for (uint32_t s0 = 0; s0 < depth; s0 += F3_G) {
             if (tid == 0) n_l = 0;
             __syncthreads();
             if (valid) {
-                const uint32_t c = s * VU + col;
-                const uint32_t cs = J.soma[c];
-                uint32_t act;
-                const uint32_t v = col / J.U, u = col - (col / J.U) * J.U;
-                if (J.control == BSM_CTRL_INTRA_COLUMN || J.control == BSM_CTRL_PYR_OUTPUT) {
-                    int safe;
-                    const uint32_t axi = cel_axn_idx(J.slices[J.base_slc + s], v, u, &safe);
-                    const uint32_t out =
-                        (J.control == BSM_CTRL_PYR_OUTPUT) ? cs : ((cs >= best) ? cs : 0u);
-                    J.axn[axi] = (uint8_t)out;
-                    act = out != 0;
-                } else {
-                    act = J.axn[J.slices[J.base_slc].idz + c] != 0;
-                }
-                if (J.learn && J.distal_mask) {
-                    uint32_t f = J.cel_flags[c];
-                    for (uint32_t t = 0; t < J.n_tufts; t++) {
-                        if (!((J.distal_mask >> t) & 1u)) continue;
-                        const uint32_t prev = (f & CEL_PREV_ACTIVE) != 0;
-                        const uint32_t tpa = J.prev_best_raw[t * C + c] != 0;
-                        const uint32_t bid = J.prev_best_ids[t * C + c];
-                        if (!prev && act && tpa && bid * J.t_spd[t] < J.t_spt[t])
-                            list[atomicAdd(&n_l, 1u)] = (c << 2) | t;
-                        f = (f & ~(uint32_t)CEL_PREV_ACTIVE) | (act ? CEL_PREV_ACTIVE : 0u);
+                uint32_t cs[F3_G], fl[F3_G], ain[F3_G], pbr[F3_G][BSM_MAX_TUFTS],
+                    pbi[F3_G][BSM_MAX_TUFTS];
+#pragma unroll
+                for (uint32_t g = 0; g < F3_G; g++) {
+                    const uint32_t c = (s0 + g) * VU + col;
+                    const bool in = s0 + g < depth;
+                    cs[g] = s0 == 0 ? cs_r[g] : (in ? (uint32_t)soma[c] : 0u);
+                    fl[g] = (in && learn) ? flags[c] : 0u;
+                    ain[g] = (in && !out_ctl) ? axn[J.slices[J.base_slc].idz + c] : 0u;
+#pragma unroll
+                    for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
+                        const bool on = in && learn && t < n_tufts && ((dmask >> t) & 1u);
+                        pbr[g][t] = on ? pb_raw[t * C + c] : 0u;
+                        pbi[g][t] = on ? pb_ids[t * C + c] : 0u;
                     }
-                    J.cel_flags[c] = (uint8_t)f;
+                }
+#pragma unroll
+                for (uint32_t g = 0; g < F3_G; g++) {
+                    const uint32_t s = s0 + g;
+                    if (s >= depth) break;
+                    const uint32_t c = s * VU + col;
+                    uint32_t act;
+                    if (out_ctl) {
+                        int safe;
+                        const uint32_t axi = cel_axn_idx(J.slices[J.base_slc + s], v, u, &safe);
+                        const uint32_t out = passthru ? cs[g] : ((cs[g] >= best) ? cs[g] : 0u);
+                        axn[axi] = (uint8_t)out;
+                        act = out != 0;
+                    } else {
+                        act = ain[g] != 0;
+                    }
+                    if (learn) {
+                        // tft_dst_mtp per distal tuft in tuft order; the cell flag is
+                        // shared by the tufts (Q9)
+                        uint32_t f = fl[g];
+#pragma unroll
+                        for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
+                            if (t >= n_tufts || !((dmask >> t) & 1u)) continue;
+                            const uint32_t prev = (f & CEL_PREV_ACTIVE) != 0;
+                            const uint32_t spd = J.t_spd[t];
+                            if (!prev && act && pbr[g][t] != 0 && pbi[g][t] * spd < J.t_spt[t]) {
+                                const uint32_t q = atomicAdd(&n_l, 1u);
+                                lz[q] = c * J.t_spt[t] + J.t_syn_idz[t] + pbi[g][t] * spd;
+                                lt[q] = (uint8_t)t;
+                            }
+                            f = (f & ~(uint32_t)CEL_PREV_ACTIVE) | (act ? CEL_PREV_ACTIVE : 0u);
+                        }
+                        flags[c] = (uint8_t)f;
+                    }
                 }
             }
             __syncthreads();
-            if (J.learn && J.distal_mask) {
+            if (learn) {
                 const uint32_t nl = n_l;
                 const uint8_t* __restrict__ prev_st = J.st[(p & 1) ^ 1];
                 const uint32_t maxch = J.max_chunks;
+                const int32_t r0 = J.rnd[rnd_ofs + J.slot_mtp0];
+                const int32_t r1 = n_tufts > 1 ? J.rnd[rnd_ofs + J.slot_mtp0 + 1] : 0;
+                const int32_t r2 = n_tufts > 2 ? J.rnd[rnd_ofs + J.slot_mtp0 + 2] : 0;
                 for (uint32_t y = tid; y < nl * maxch; y += blockDim.x) {
                     const uint32_t li = y / maxch, k = y - li * maxch;
-                    const uint32_t e = list[li];
-                    const uint32_t c = e >> 2, t = e & 3u;
-                    const uint32_t spd = J.t_spd[t];
-                    if (k * W >= spd) continue;
-                    const uint32_t z = c * J.t_spt[t] + J.t_syn_idz[t] +
-                                       (uint32_t)J.prev_best_ids[t * C + c] * spd;
-                    const int32_t rnd = J.rnd[rnd_ofs + J.slot_mtp0 + t];
-                    dst_ltp_chunk<W>(prev_st, J.str, z, z + k * W, rnd);
+                    const uint32_t t = lt[li];
+                    if (k * W >= J.t_spd[t]) continue;
+                    const uint32_t z = lz[li];
+                    dst_ltp_chunk<W>(prev_st, J.str, z, z + k * W, t == 0 ? r0 : (t == 1 ? r1 : r2));
                 }
             }
             __syncthreads();
@@ -1101,6 +1163,22 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
     return 0;
 }
 
+// Device-to-device frame copy into the axon space (AxonSpace::intake,
+// src/cortex/axon_space.rs:509-629): one block per INTAKE_SEG bytes.
+constexpr uint32_t INTAKE_SEG = 16384;
+__global__ __launch_bounds__(256) void k_intake(uint8_t* __restrict__ dst,
+                                                const uint8_t* __restrict__ src, uint32_t len) {
+    const uint32_t lo = blockIdx.x * INTAKE_SEG;
+    if (lo < len) block_copy(dst + lo, src + lo, min(INTAKE_SEG, len - lo), threadIdx.x, blockDim.x);
+}
+
+int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len) {
+    if (!len) return 0;
+    k_intake<<<(len + INTAKE_SEG - 1) / INTAKE_SEG, 256, 0, s>>>(dst, src, len);
+    BSM_HIP(hipGetLastError());
+    return 0;
+}
+
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs) {
     for (const Stage& st : a->stages) {
         LayerRt& L = a->L[st.layer];
@@ -1165,9 +1243,10 @@ int bsm_area_run(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (upload_rnd(a, nullptr, n_cycles)) return 1;
     for (uint32_t c = 0; c < n_cycles; c++) {
-        if (dev_frames && frame_len)
-            BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
-                                   frame_len, hipMemcpyDeviceToDevice, a->stream));
+        if (dev_frames && frame_len &&
+            launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
+                          frame_len))
+            return 1;
         if (cycle_fused(a, c * a->n_slots)) return 1;
     }
     return 0;
@@ -1184,9 +1263,10 @@ int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_fr
     BSM_HIP(hipStreamSynchronize(a->stream));
     BSM_HIP(hipEventRecord(a->ev0, a->stream));
     for (uint32_t c = 0; c < n_cycles; c++) {
-        if (dev_frames && frame_len)
-            BSM_HIP(hipMemcpyAsync(a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
-                                   frame_len, hipMemcpyDeviceToDevice, a->stream));
+        if (dev_frames && frame_len &&
+            launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
+                          frame_len))
+            return 1;
         if (cycle_fused(a, c * a->n_slots)) return 1;
     }
     BSM_HIP(hipEventRecord(a->ev1, a->stream));
@@ -1203,6 +1283,21 @@ __global__ __launch_bounds__(256) void k_calib_copy(const uint4* __restrict__ sr
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+// The synapse streams of F1 without the gather: four 16-byte non-temporal
+// read streams (n vectors each, back to back in src) and one 16-byte write
+// stream, the load/store forms F1 uses.
+__global__ __launch_bounds__(256) void k_calib_stream(const u32x4* __restrict__ src,
+                                                      u32x4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const u32x4 a = __builtin_nontemporal_load(BSM_G(const u32x4, src + i));
+        const u32x4 b = __builtin_nontemporal_load(BSM_G(const u32x4, src + n + i));
+        const u32x4 c = __builtin_nontemporal_load(BSM_G(const u32x4, src + 2 * n + i));
+        const u32x4 d = __builtin_nontemporal_load(BSM_G(const u32x4, src + 3 * n + i));
+        *BSM_G(u32x4, dst + i) = a ^ b ^ c ^ d;
+    }
 }
 }  // namespace bsm
 
@@ -1257,6 +1352,28 @@ int bsm_area_stage_plan(bsm_area* a, uint32_t max_stages, bsm_stage_info* out,
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (a->plan_dirty && build_plan(a)) return 1;
     return stage_info(a->stages, max_stages, out, n_stages);
+}
+
+int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+    if (!ctx || !dst || !src) return fail("null arg");
+    if (bytes % 16) return fail("bytes must be a multiple of 16");
+    BSM_HIP(hipSetDevice(ctx->device));
+    hipEvent_t e0, e1;
+    BSM_HIP(hipEventCreate(&e0));
+    BSM_HIP(hipEventCreate(&e1));
+    const size_t n = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
+    BSM_HIP(hipEventRecord(e0, 0));
+    k_calib_stream<<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n);
+    BSM_HIP(hipGetLastError());
+    BSM_HIP(hipEventRecord(e1, 0));
+    BSM_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    BSM_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
 }
 
 int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {

@@ -220,6 +220,7 @@ int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_id
 int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_ids,
             std::vector<WorkItem>& items, Stage& st);
 int build_plan(bsm_area* a);
+int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len);
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
                uint32_t* n_stages);
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,

@@ -267,6 +267,11 @@ int bsm_batch_stage_plan(bsm_batch* batch, uint32_t max_stages, bsm_stage_info* 
 /* Plain device-to-device copy kernel of `bytes` (multiple of 16): the known-size
  * reference for scaling HBM traffic counters. */
 int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
+/* The fused synapse stage's HBM pattern without its gather: src holds four
+ * streams of `bytes` (multiple of 16) back to back, read with non-temporal
+ * 16-byte loads; dst receives one stream of `bytes`.  The practical bandwidth
+ * ceiling for that stage's 4:1 read/write mix. */
+int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
 
 /* ---- host-side synapse layout generator (replaces Synapses::grow(init=true),
  *      src/cortex/synapses.rs:454-501, entropy-seeded upstream): src slice
