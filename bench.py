@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"),
+                    help="fixed column tile of the plain form instead of the autotuner's")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling helper: run only the 64-area batched leg and print it")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -233,6 +235,9 @@ def main():
     dev = ctx.alloc(frames.nbytes)
     ctx.upload(dev, frames)
 
+    if args.tile:
+        tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
+        area.set_tuning(**tuning)
     if args.pmc_probe:
         n = 256 << 20
         a, b = ctx.alloc(n), ctx.alloc(n)
@@ -243,6 +248,14 @@ def main():
         print("pmc probe done", flush=True)
         return
 
+    if not args.tile:
+        # tile geometry by the project autotuner (timed on a scratch copy of the area)
+        from bismit_amd.autotune import tune_area
+        tiles = [(8, 8), (4, 8), (2, 8), (2, 16), (1, 32), (4, 16)]
+        tuning, _ = tune_area(scheme, ctx, cycles=10,
+                              cands=[dict(tile_v=tv, tile_u=tu, chunk=16, halo=6)
+                                     for tv, tu in tiles])
+        area.set_tuning(**tuning)
     area.run(dev, in_len, in_len, args.warmup)
     area.sync()
     times = []
@@ -291,7 +304,8 @@ def main():
         "config": {"workload": "config2: one 256x256-column area, plain form, 142,606,336 synapses "
                                "(SSC 1x4x32, pyramidal 2 slices x 2 tufts x 8x64), 2% SDR input "
                                "with 50% overlap, learning on",
-                   "synapses": n_syn, "runs": args.runs, "device": ctx.info()["name"]},
+                   "synapses": n_syn, "runs": args.runs, "tuning": tuning,
+                   "device": ctx.info()["name"]},
         "roofline": roofline,
         "stages": stages,
     }

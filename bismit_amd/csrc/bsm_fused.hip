@@ -27,11 +27,27 @@ namespace bsm {
 
 // ------------------------------------------------------------------ F1
 
+// Buffer resource over `bytes` bytes at base (raw, dword format): loads and
+// stores through it take one 32-bit offset VGPR and return 0 beyond the range.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ inline Rsrc make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+constexpr int BUF_NT = 2;  // non-temporal (streaming) cache policy
+
 template <int W>
 struct Chunk;
 template <>
 struct Chunk<16> {
     static constexpr int NW = 4;
+    __device__ static inline void load_b(Rsrc r, uint32_t off, uint32_t* o) {
+        u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, BUF_NT);
+        o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w;
+    }
+    __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
+        u32x4 v = {o[0], o[1], o[2], o[3]};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+    }
     __device__ static inline void load(const void* p, uint32_t* o) {
         u32x4 x = __builtin_nontemporal_load(BSM_G(const u32x4, p));
         o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w;
@@ -44,6 +60,14 @@ struct Chunk<16> {
 template <>
 struct Chunk<8> {
     static constexpr int NW = 2;
+    __device__ static inline void load_b(Rsrc r, uint32_t off, uint32_t* o) {
+        u32x2 x = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, BUF_NT);
+        o[0] = x.x; o[1] = x.y;
+    }
+    __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
+        u32x2 v = {o[0], o[1]};
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
+    }
     __device__ static inline void load(const void* p, uint32_t* o) {
         u32x2 x = __builtin_nontemporal_load(BSM_G(const u32x2, p));
         o[0] = x.x; o[1] = x.y;
@@ -56,6 +80,12 @@ struct Chunk<8> {
 template <>
 struct Chunk<4> {
     static constexpr int NW = 1;
+    __device__ static inline void load_b(Rsrc r, uint32_t off, uint32_t* o) {
+        o[0] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, BUF_NT);
+    }
+    __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
+        __builtin_amdgcn_raw_buffer_store_b32(o[0], r, (int)off, 0, 0);
+    }
     __device__ static inline void load(const void* p, uint32_t* o) {
         o[0] = __builtin_nontemporal_load(BSM_G(const uint32_t, p));
     }
@@ -148,6 +178,14 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
     __syncthreads();
 }
 
+// n / d as a multiply-high with magic = ceil(2^32 / d) (d > 1), exact whenever
+// n * d < 2^32 (the rounding error n * (magic * d - 2^32) / 2^32 stays below
+// the gap to the next multiple of d); d == 1 passes n through.
+__device__ __forceinline__ uint32_t magic_u32(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
+__device__ __forceinline__ uint32_t quot_u32(uint32_t n, uint32_t d, uint32_t magic) {
+    return d > 1 ? __umulhi(n, magic) : n;
+}
+
 // Adds one chunk's per-word (sum, raw) partials to the per-dendrite packed
 // accumulators (sum in bits 0-15, raw in bits 16-31; a dendrite's raw sum is at
 // most 256*255 < 2^16, so the halves never carry into each other).  A chunk
@@ -204,14 +242,24 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             // the full tile + halo window of each (zeros off the slice) and gather
             // without any per-synapse test.
             const SliceRt s0 = J.slices[J.staged[0]];
+            // uniform per item: keep the geometry (and the parity) in scalar registers
+            const uint32_t vsc = __builtin_amdgcn_readfirstlane(s0.v_scale);
+            const uint32_t usc = __builtin_amdgcn_readfirstlane(s0.u_scale);
+            const int32_t vmid = (int32_t)__builtin_amdgcn_readfirstlane(s0.v_mid);
+            const int32_t umid = (int32_t)__builtin_amdgcn_readfirstlane(s0.u_mid);
+            const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
             const int32_t ih = (int32_t)halo;
-            const int32_t a0 = (int32_t)((v0 * s0.v_scale) >> 4) + (int32_t)s0.v_mid - ih;
-            const int32_t b0 = (int32_t)((u0 * s0.u_scale) >> 4) + (int32_t)s0.u_mid - ih;
-            const int32_t nr = (int32_t)(((v0 + tvn - 1) * s0.v_scale) >> 4) -
-                               (int32_t)((v0 * s0.v_scale) >> 4) + 1 + 2 * ih;
-            const int32_t nc = (int32_t)(((u0 + tun - 1) * s0.u_scale) >> 4) -
-                               (int32_t)((u0 * s0.u_scale) >> 4) + 1 + 2 * ih;
+            const int32_t a0 = (int32_t)((v0 * vsc) >> 4) + vmid - ih;
+            const int32_t b0 = (int32_t)((u0 * usc) >> 4) + umid - ih;
+            const int32_t nr = (int32_t)(((v0 + tvn - 1) * vsc) >> 4) -
+                               (int32_t)((v0 * vsc) >> 4) + 1 + 2 * ih;
+            const int32_t nc = (int32_t)(((u0 + tun - 1) * usc) >> 4) -
+                               (int32_t)((u0 * usc) >> 4) + 1 + 2 * ih;
             const uint32_t WS = (uint32_t)(nr * nc);
+            const uint32_t CPT = spt / W;
+            const uint32_t rows = depth * tvn, row_chunks = tun * CPT;
+            const uint32_t n_chunks = rows * row_chunks;
+            const uint32_t VU = V * U;
             for (uint32_t j = 0; j < J.n_staged; j++) {
                 const SliceRt s = J.slices[J.staged[j]];
                 auto src = BSM_G(const uint8_t, J.axn + s.idz);
@@ -224,54 +272,45 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     lds[j * WS + x] = val;
                 }
             }
+            // per tile row (slice, v): byte offset of its first cell's synapses, and
+            // the window offset of that row's centre minus the first column's
+            uint32_t* rowt = (uint32_t*)(lds + ((J.n_staged * WS + 15) & ~15u));
+            const int32_t sid0ws = NST > 1 ? (int32_t)(J.staged[0] * WS) : 0;
+            for (uint32_t rq = tid; rq < rows; rq += nth) {
+                const uint32_t sl = rq / tvn, v = v0 + (rq - sl * tvn);
+                rowt[2 * rq] = (sl * VU + v * U + u0) * spt;
+                rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * nc +
+                                              umid - b0 - sid0ws);
+            }
             __syncthreads();
             const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
-            const uint32_t CPT = spt / W;
-            const uint32_t row_chunks = tun * CPT;
-            const uint32_t n_chunks = depth * tvn * row_chunks;
-            const uint32_t VU = V * U;
-            uint8_t* __restrict__ st_out = J.st[p & 1];
+            // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
+            const uint32_t nrec = J.C * spt;
+            const Rsrc r_slc = make_rsrc(J.slc, nrec), r_v = make_rsrc(J.vofs, nrec);
+            const Rsrc r_u = make_rsrc(J.uofs, nrec), r_str = make_rsrc(J.str, nrec);
+            const Rsrc r_out = make_rsrc(J.st[pu & 1], nrec);
             const int32_t coefA = (nc & 0xFF) | (1 << 8);
             const int32_t coefB = ((nc & 0xFF) << 16) | (1 << 24);
-            const int32_t sid0ws = (int32_t)(J.staged[0] * WS);
-            auto pos = [&](uint32_t q, size_t& so, int32_t& lb, uint32_t& den0, uint32_t& off0) {
-                const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
-                const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
-                const uint32_t cc = rem / CPT, k = rem - cc * CPT;
-                const uint32_t v = v0 + r, u = u0 + cc;
-                den0 = (rowq * tun + cc) * J.dens;
-                off0 = k * W;
-                so = (size_t)(sl * VU + v * U + u) * spt + (size_t)k * W;
-                lb = ((int32_t)((v * s0.v_scale) >> 4) + (int32_t)s0.v_mid - a0) * nc +
-                     (int32_t)((u * s0.u_scale) >> 4) + (int32_t)s0.u_mid - b0;
-                if (NST > 1) lb -= sid0ws;
+            // chunk index -> (row, chunk in row) -> (column, chunk in cell) by
+            // multiply-high quotients (exact: the planner keeps n_chunks < 2^16)
+            const uint32_t m_rc = magic_u32(row_chunks), m_cpt = magic_u32(CPT);
+            auto fetch = [&](uint32_t q, uint32_t* ms, uint32_t* mv, uint32_t* mu, uint32_t* mt,
+                             uint32_t& so, int32_t& lb, uint32_t& den0, uint32_t& off0) {
+                const uint32_t rowq = quot_u32(q, row_chunks, m_rc);
+                const uint32_t rem = q - __umul24(rowq, row_chunks);
+                const uint32_t cc = quot_u32(rem, CPT, m_cpt);
+                den0 = __umul24(__umul24(rowq, tun) + cc, J.dens);
+                off0 = (rem - __umul24(cc, CPT)) * W;
+                so = rowt[2 * rowq] + rem * W;
+                lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
+                Chunk<W>::load_b(r_slc, so, ms);
+                Chunk<W>::load_b(r_v, so, mv);
+                Chunk<W>::load_b(r_u, so, mu);
+                Chunk<W>::load_b(r_str, so, mt);
             };
-            uint32_t ms[NW], mv[NW], mu[NW], mt[NW];
-            uint32_t q = tid;
-            size_t so = 0;
-            int32_t lb = 0;
-            uint32_t den0 = 0, off0 = 0;
-            if (q < n_chunks) {
-                pos(q, so, lb, den0, off0);
-                Chunk<W>::load(J.slc + so, ms);
-                Chunk<W>::load(J.vofs + so, mv);
-                Chunk<W>::load(J.uofs + so, mu);
-                Chunk<W>::load(J.str + so, mt);
-            }
-            for (; q < n_chunks; q += nth) {
-                // prefetch the next chunk's metadata before working on this one
-                const uint32_t qn = q + nth;
-                size_t so_n = 0;
-                int32_t lb_n = 0;
-                uint32_t den0_n = 0, off0_n = 0;
-                uint32_t ns[NW], nv[NW], nu[NW], nt[NW];
-                if (qn < n_chunks) {
-                    pos(qn, so_n, lb_n, den0_n, off0_n);
-                    Chunk<W>::load(J.slc + so_n, ns);
-                    Chunk<W>::load(J.vofs + so_n, nv);
-                    Chunk<W>::load(J.uofs + so_n, nu);
-                    Chunk<W>::load(J.str + so_n, nt);
-                }
+            auto work = [&](const uint32_t* ms, const uint32_t* mv, const uint32_t* mu,
+                            const uint32_t* mt, uint32_t so, int32_t lb, uint32_t den0,
+                            uint32_t off0) {
                 uint32_t out[NW], pw[NW];
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
@@ -283,10 +322,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lb, false);
                     int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lb, false);
                     if (NST > 1) {
-                        ad0 += (int32_t)(byte_of(ms[w], 0) * WS);
-                        ad1 += (int32_t)(byte_of(ms[w], 1) * WS);
-                        ad2 += (int32_t)(byte_of(ms[w], 2) * WS);
-                        ad3 += (int32_t)(byte_of(ms[w], 3) * WS);
+                        ad0 += (int32_t)__umul24(byte_of(ms[w], 0), WS);
+                        ad1 += (int32_t)__umul24(byte_of(ms[w], 1), WS);
+                        ad2 += (int32_t)__umul24(byte_of(ms[w], 2), WS);
+                        ad3 += (int32_t)__umul24(byte_of(ms[w], 3), WS);
                     }
                     const uint32_t x = (uint32_t)lds[ad0] | ((uint32_t)lds[ad1] << 8) |
                                        ((uint32_t)lds[ad2] << 16) | ((uint32_t)lds[ad3] << 24);
@@ -298,19 +337,25 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     pw[w] = __builtin_amdgcn_sad_u8(stw & ~neg, 0u, 0u) |
                             (__builtin_amdgcn_sad_u8(stw, 0u, 0u) << 16);
                 }
-                Chunk<W>::store(st_out + so, out);
+                Chunk<W>::store_b(r_out, so, out);
                 acc_chunk<NW, 4>(acc, den0, off0, pw, whole, magic);
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    ms[w] = ns[w];
-                    mv[w] = nv[w];
-                    mu[w] = nu[w];
-                    mt[w] = nt[w];
-                }
-                so = so_n;
-                lb = lb_n;
-                den0 = den0_n;
-                off0 = off0_n;
+            };
+            // two register sets in turn: the next chunk's metadata is in flight
+            // while the current one is gathered
+            uint32_t aS[NW], aV[NW], aU[NW], aT[NW], bS[NW], bV[NW], bU[NW], bT[NW];
+            uint32_t soA = 0, denA = 0, offA = 0, soB = 0, denB = 0, offB = 0;
+            int32_t lbA = 0, lbB = 0;
+            uint32_t q = tid;
+            if (q < n_chunks) fetch(q, aS, aV, aU, aT, soA, lbA, denA, offA);
+            while (q < n_chunks) {
+                const uint32_t qb = q + nth;
+                if (qb < n_chunks) fetch(qb, bS, bV, bU, bT, soB, lbB, denB, offB);
+                work(aS, aV, aU, aT, soA, lbA, denA, offA);
+                if (qb >= n_chunks) break;
+                const uint32_t qa = qb + nth;
+                if (qa < n_chunks) fetch(qa, aS, aV, aU, aT, soA, lbA, denA, offA);
+                work(bS, bV, bU, bT, soB, lbB, denB, offB);
+                q = qa;
             }
             __syncthreads();
             tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth);
@@ -445,7 +490,7 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                                                    uint32_t n_items, uint32_t par, uint32_t rnd_ofs,
                                                    uint32_t TV, uint32_t TU) {
     extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t n_act;
+    __shared__ uint32_t n_act, n_hex;
     const uint32_t tid = threadIdx.x, nth = blockDim.x;
     for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
         const WorkItem wi = items[it];
@@ -461,7 +506,9 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             if (!(d & 2u)) continue;
             p = d & 1u;
         }
-        uint32_t* list = (uint32_t*)(lds + ((WV * WU + 15) & ~15u));
+        uint32_t* list = (uint32_t*)(lds + ((WV * WU + 15) & ~15u));  // learning cells
+        uint16_t* hexl = (uint16_t*)(list + TV * TU);  // tile cells that need the hex test
+        uint8_t* hexo = (uint8_t*)(hexl + TV * TU);    // their uninhibited flags
         const uint8_t* soma = J.soma + (size_t)slc * VU;
         // loads independent of the window, issued with it ahead of the barrier:
         // rnd values, the own slice entry, this thread's first cell's rating
@@ -480,10 +527,43 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             if (nv >= 0 && nv < (int32_t)V && nu >= 0 && nu < (int32_t)U) val = soma[nv * U + nu];
             lds[x] = val;
         }
-        if (tid == 0) n_act = 0;
+        if (tid == 0) {
+            n_act = 0;
+            n_hex = 0;
+        }
         __syncthreads();
 
         const int32_t imax = (r + INHIB_HORIZ_OFS) * (r + INHIB_HORIZ_OFS);
+        if (J.mode == 0) {
+            // The hex test decides only cells with a non-zero soma: pack those
+            // densely so that lanes of silent cells do not run the loop.
+            for (uint32_t x = tid; x < tvn * tun; x += nth) {
+                const uint32_t tr = x / tun, tc = x - tr * tun;
+                int safe;
+                (void)cel_axn_idx(own, v0 + tr, u0 + tc, &safe);
+                if (safe && lds[(tr + r) * WU + tc + r]) hexl[atomicAdd(&n_hex, 1u)] = (uint16_t)x;
+            }
+            __syncthreads();
+            const uint32_t nh = n_hex;
+            for (uint32_t y = tid; y < nh; y += nth) {
+                const uint32_t x = hexl[y];
+                const uint32_t tr = x / tun, tc = x - tr * tun;
+                const int32_t ci = (int32_t)lds[(tr + r) * WU + tc + r] * imax;
+                uint32_t unin = 1;
+                for (int32_t vo = -r; vo <= r && unin; vo++) {
+                    const int32_t uz = max(-r, -vo - r), um = min(r, -vo + r);
+                    const uint8_t* row = lds + (uint32_t)((int32_t)tr + r + vo) * WU + tc + r;
+                    for (int32_t uo = uz; uo <= um; uo++) {
+                        const int32_t w = -vo - uo;
+                        const int32_t dist = (abs(vo) + abs(uo) + abs(w)) >> 1;
+                        const int32_t dc = dist - INHIB_CENTER_OFS;
+                        unin &= (uint32_t)(ci >= (int32_t)row[uo] * (imax - dc * dc));
+                    }
+                }
+                hexo[x] = (uint8_t)unin;
+            }
+            __syncthreads();
+        }
         for (uint32_t x = tid; x < tvn * tun; x += nth) {
             const uint32_t tr = x / tun, tc = x - tr * tun;
             const uint32_t v = v0 + tr, u = u0 + tc;
@@ -498,20 +578,7 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                 uint32_t is_act;
                 if (J.mode == 0) {
                     cs = safe ? cs : 0u;
-                    int unin = 1;
-                    if (cs) {
-                        const int32_t ci = (int32_t)cs * imax;
-                        for (int32_t vo = -r; vo <= r && unin; vo++) {
-                            const int32_t uz = max(-r, -vo - r), um = min(r, -vo + r);
-                            const uint8_t* row = lds + (uint32_t)((int32_t)tr + r + vo) * WU + tc + r;
-                            for (int32_t uo = uz; uo <= um; uo++) {
-                                const int32_t w = -vo - uo;
-                                const int32_t dist = (abs(vo) + abs(uo) + abs(w)) >> 1;
-                                const int32_t dc = dist - INHIB_CENTER_OFS;
-                                unin &= ci >= (int32_t)row[uo] * (imax - dc * dc);
-                            }
-                        }
-                    }
+                    const uint32_t unin = cs ? (uint32_t)hexo[x] : 1u;
                     out = unin ? cs : 0u;
                     is_act = unin & (cs != 0);
                 } else {
@@ -905,9 +972,10 @@ PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3) {
 
 static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fast) {
     if (fast) {
+        // windows, then 8 bytes per tile row (slice, v)
         const uint32_t r = (((TV - 1) * j.st_vsc[0]) >> 4) + 2 + 2 * j.halo;
         const uint32_t c = (((TU - 1) * j.st_usc[0]) >> 4) + 2 + 2 * j.halo;
-        return j.n_staged * r * c;
+        return ((j.n_staged * r * c + 15) & ~15u) + 8 * j.depth * TV;
     }
     uint32_t tot = 0;
     for (uint32_t k = 0; k < j.n_staged; k++) {
@@ -942,7 +1010,11 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
         }
         cap = (cap + 15) & ~15u;
         lds = cap + accb + denc;
-        if (lds <= LDS_LIMIT) break;
+        // the check-free path divides chunk indices by multiply-high (exact below 2^16 chunks)
+        uint32_t chunks = 0;
+        for (uint32_t id : job_ids)
+            chunks = std::max(chunks, jobs[id].depth * TV * TU * (jobs[id].spt / std::max(W, 1u)));
+        if (lds <= LDS_LIMIT && (!fast || chunks < 65536u)) break;
         if (TV == 1 && TU == 1) return fail("fused tuft kernel does not fit in LDS");
         if (TV >= TU && TV > 1) TV = (TV + 1) / 2;
         else TU = (TU + 1) / 2;
@@ -990,7 +1062,7 @@ int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_id
     }
     st.n_items = (uint32_t)items.size() - st.item0;
     uint32_t win = (TV + 2 * rmax) * (TU + 2 * rmax);
-    st.lds_bytes = ((win + 15) & ~15u) + 4 * TV * TU;
+    st.lds_bytes = ((win + 15) & ~15u) + 7 * TV * TU;  // window | learn list | hex list | flags
     if (st.lds_bytes > LDS_LIMIT) return fail("inhibition window does not fit in LDS");
     st.W = W;
     st.tv = TV;
