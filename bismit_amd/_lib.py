@@ -69,7 +69,7 @@ class Tract(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_v", C.c_uint32), ("tile_u", C.c_uint32), ("halo", C.c_uint32),
-                ("chunk", C.c_uint32)]
+                ("chunk", C.c_uint32), ("inhib_tile", C.c_uint32)]
 
 
 class StageInfo(C.Structure):

@@ -277,6 +277,26 @@ void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, si
     }
 }
 
+// A tuning update: zero fields keep the current value; inhib_tile 1 returns
+// the inhibition tile to the planner's choice.
+int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
+    Tuning nt = cur;
+    if (t->tile_v) nt.tile_v = t->tile_v;
+    if (t->tile_u) nt.tile_u = t->tile_u;
+    if (t->halo) nt.halo = t->halo;
+    if (t->chunk) nt.chunk = t->chunk;
+    if (t->inhib_tile) nt.inhib_tile = t->inhib_tile == 1 ? 0u : t->inhib_tile;
+    if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
+        return fail("tile too large");
+    if (nt.halo > 64) return fail("halo too large");
+    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
+        return fail("chunk must be 1, 4, 8 or 16");
+    if (nt.inhib_tile != 0 && nt.inhib_tile != 8 && nt.inhib_tile != 16 && nt.inhib_tile != 32)
+        return fail("inhib_tile must be 1 (planner's choice), 8, 16 or 32");
+    *out = nt;
+    return 0;
+}
+
 }  // namespace bsm
 
 using namespace bsm;
@@ -654,16 +674,8 @@ int bsm_area_draw_rnd(bsm_area* a, int32_t* out, uint32_t n_cycles) {
 int bsm_area_set_tuning(bsm_area* a, const bsm_tuning* t) {
     if (!a || !t) return fail("null arg");
     if (a->batch_stream) return fail("area belongs to a batch");
-    Tuning nt = a->tune;
-    if (t->tile_v) nt.tile_v = t->tile_v;
-    if (t->tile_u) nt.tile_u = t->tile_u;
-    if (t->halo) nt.halo = t->halo;
-    if (t->chunk) nt.chunk = t->chunk;
-    if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
-        return fail("tile too large");
-    if (nt.halo > 64) return fail("halo too large");
-    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
-        return fail("chunk must be 1, 4, 8 or 16");
+    Tuning nt;
+    if (merge_tuning(a->tune, t, &nt)) return 1;
     a->tune = nt;
     a->plan_dirty = true;
     return 0;
@@ -675,6 +687,7 @@ int bsm_area_get_tuning(bsm_area* a, bsm_tuning* t) {
     t->tile_u = a->tune.tile_u;
     t->halo = a->tune.halo;
     t->chunk = a->tune.chunk;
+    t->inhib_tile = a->tune.inhib_tile ? a->tune.inhib_tile : 1u;
     return 0;
 }
 

@@ -578,16 +578,8 @@ int bsm_batch_stage_plan(bsm_batch* b, uint32_t max_stages, bsm_stage_info* out,
 
 int bsm_batch_set_tuning(bsm_batch* b, const bsm_tuning* t) {
     if (!b || !t) return fail("null arg");
-    Tuning nt = b->tune;
-    if (t->tile_v) nt.tile_v = t->tile_v;
-    if (t->tile_u) nt.tile_u = t->tile_u;
-    if (t->halo) nt.halo = t->halo;
-    if (t->chunk) nt.chunk = t->chunk;
-    if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
-        return fail("tile too large");
-    if (nt.halo > 64) return fail("halo too large");
-    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
-        return fail("chunk must be 1, 4, 8 or 16");
+    Tuning nt;
+    if (merge_tuning(b->tune, t, &nt)) return 1;
     BSM_HIP(hipStreamSynchronize(b->stream));
     b->tune = nt;
     b->dirty = true;

@@ -484,6 +484,8 @@ __device__ inline void ssc_ltp_chunk(const uint8_t* __restrict__ st, int8_t* __r
     Chunk<W>::store(str + i0, out);
 }
 
+constexpr uint32_t F2_MAXC = 4;  // tile cells per thread (tiles up to 4 x 256 cells)
+
 template <int W>
 __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
@@ -515,10 +517,15 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         const int32_t rnd_c = J.mode == 0 ? J.rnd[rnd_ofs + J.slot_ctrl] : 0;
         const int32_t rnd_m = J.learn ? J.rnd[rnd_ofs + J.slot_mtp] : 0;
         const SliceRt own = J.slices[J.base_slc + slc];
-        uint32_t act0 = 0;
-        if (J.mode != 2 && tid < tvn * tun) {
-            const uint32_t tr = tid / tun, tc = tid - tr * tun;
-            act0 = BSM_G(const uint8_t, J.cel_act)[slc * VU + (v0 + tr) * U + (u0 + tc)];
+        uint32_t actp[F2_MAXC];  // activity ratings of this thread's cells
+#pragma unroll
+        for (uint32_t k = 0; k < F2_MAXC; k++) {
+            const uint32_t x = tid + k * nth;
+            actp[k] = 0;
+            if (J.mode != 2 && x < tvn * tun) {
+                const uint32_t tr = x / tun, tc = x - tr * tun;
+                actp[k] = BSM_G(const uint8_t, J.cel_act)[slc * VU + (v0 + tr) * U + (u0 + tc)];
+            }
         }
         for (uint32_t x = tid; x < WV * WU; x += nth) {
             const uint32_t wr = x / WU, wc = x - wr * WU;
@@ -564,7 +571,10 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             }
             __syncthreads();
         }
-        for (uint32_t x = tid; x < tvn * tun; x += nth) {
+#pragma unroll
+        for (uint32_t k = 0; k < F2_MAXC; k++) {
+            const uint32_t x = tid + k * nth;
+            if (x >= tvn * tun) break;
             const uint32_t tr = x / tun, tc = x - tr * tun;
             const uint32_t v = v0 + tr, u = u0 + tc;
             const uint32_t c = slc * VU + v * U + u;
@@ -586,9 +596,8 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     is_act = cs != 0;
                 }
                 J.axn[axi] = (uint8_t)out;
-                const uint32_t a_in = x == tid ? act0 : (uint32_t)J.cel_act[c];
-                J.cel_act[c] = (uint8_t)activity(a_in, (int32_t)is_act, rnd_c, (int32_t)(cs | c),
-                                                 CEL_ACT_DECAY);
+                J.cel_act[c] = (uint8_t)activity(actp[k], (int32_t)is_act, rnd_c,
+                                                 (int32_t)(cs | c), CEL_ACT_DECAY);
             }
             if (J.learn && out) list[atomicAdd(&n_act, 1u)] = c;
         }
@@ -1044,12 +1053,24 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
 
 int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_ids, const Tuning& tn,
             std::vector<WorkItem>& items, Stage& st) {
-    uint32_t TV = tn.inhib_tile_v, TU = tn.inhib_tile_u, rmax = 0, W = 16;
+    uint32_t rmax = 0, W = 16;
     for (uint32_t id : job_ids) {
         const SscJob& j = jobs[id];
         if (j.mode == 0) rmax = std::max(rmax, j.radius);
         W = std::min(W, pick_w(j.spt, 0, 0, 16));
     }
+    // Inhibition tile: the tuning's, or by default 16 x 16 unless that gives more
+    // tiles than about one round of resident blocks, then 32 x 32 (each item is a
+    // chain of dependent memory round trips; fewer, larger items overlap them).
+    auto n_tiles = [&](uint32_t t) {
+        uint64_t n = 0;
+        for (uint32_t id : job_ids)
+            n += (uint64_t)jobs[id].depth * ((jobs[id].V + t - 1) / t) * ((jobs[id].U + t - 1) / t);
+        return n;
+    };
+    uint32_t side = tn.inhib_tile ? tn.inhib_tile : (n_tiles(16) > 2048 ? 32u : 16u);
+    const uint32_t TV = side, TU = side;
+    if (TV * TU > F2_MAXC * 256) return fail("inhibition tile larger than 4 x 256 cells");
     st.kind = 2;
     st.item0 = (uint32_t)items.size();
     for (uint32_t id : job_ids) {

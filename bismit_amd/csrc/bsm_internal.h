@@ -143,7 +143,7 @@ struct LayerRt {
 
 struct Tuning {
     uint32_t tile_v = 8, tile_u = 8, halo = 8, chunk = 16;
-    uint32_t inhib_tile_v = 16, inhib_tile_u = 16;
+    uint32_t inhib_tile = 0;  // side of the square inhibition tile; 0 = chosen by the planner
 };
 
 // One launch of a fused stage over a contiguous range of work items.
@@ -220,6 +220,7 @@ int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_id
 int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_ids,
             std::vector<WorkItem>& items, Stage& st);
 int build_plan(bsm_area* a);
+int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out);
 int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len);
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
                uint32_t* n_stages);

@@ -219,6 +219,8 @@ typedef struct bsm_tuning {
     uint32_t tile_u;     /* columns per workgroup tile, u */
     uint32_t halo;       /* LDS halo (columns of source reach staged) */
     uint32_t chunk;      /* synapses per lane chunk: 4, 8 or 16 */
+    uint32_t inhib_tile; /* side of the square inhibition tile: 8, 16 or 32;
+                            1 = the planner's choice (the default) */
 } bsm_tuning;
 int bsm_area_set_tuning(bsm_area* area, const bsm_tuning* t);
 int bsm_area_get_tuning(bsm_area* area, bsm_tuning* t);

@@ -136,13 +136,14 @@ def test_cycle_config1_1000_cycles(ctx):
 
 @pytest.mark.parametrize("tuning", [
     dict(tile_v=4, tile_u=4, halo=1, chunk=4),
-    dict(tile_v=16, tile_u=8, halo=8, chunk=8),
+    dict(tile_v=16, tile_u=8, halo=8, chunk=8, inhib_tile=32),
     dict(tile_v=8, tile_u=16, halo=3, chunk=16),
-    dict(tile_v=3, tile_u=5, halo=2, chunk=1),
+    dict(tile_v=3, tile_u=5, halo=2, chunk=1, inhib_tile=8),
 ])
 def test_cycle_tunings(ctx, tuning):
-    """Any tile/halo/chunk choice gives identical results (chunk 1 forces the
-    checked gather, whose small halos take the out-of-window global path)."""
+    """Any tile/halo/chunk/inhibition-tile choice gives identical results
+    (chunk 1 forces the checked gather, whose small halos take the out-of-window
+    global path; 32 x 32 inhibition tiles put four cells on each thread)."""
     print("tuning", tuning, flush=True)
     scheme = recipes.cortical_area("t", 40, input_side=32, pyr_depth=3, ssc_dens=4, ssc_spd=16,
                                    pyr_dens=4, pyr_spd=24, inhib_radius=4)
