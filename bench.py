@@ -134,6 +134,18 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
     ctx.sync()
     plain_s = time.perf_counter() - t0
     plain = n_syn * n_plain / plain_s
+    # the reference's driving pattern (Cortex::cycle): every cycle, each area's
+    # intake and cycle enqueued in turn by one host thread (C loop, no Python)
+    from bismit_amd.cortex import cycle_areas
+    n_loop = max(10, n_cycles // 10)
+    ctx.sync()
+    t0 = time.perf_counter()
+    cycle_areas(areas, devs, lens, nf, n_loop)
+    for a in areas:
+        a.sync()
+    ctx.sync()
+    loop_s = time.perf_counter() - t0
+    loop = n_syn * n_loop / loop_s
     b = Batch(areas, ctx)
     b.set_inputs(devs, lens, lens, [s.input_range()[0] for s in schemes], nf)
     best, table = None, []
@@ -176,6 +188,9 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
                         "per layer, 16-64 synapses per dendrite, own 2% SDR input each",
             "synapses": n_syn, "batched_synapse_updates_per_s": batched,
             "plain_per_area_synapse_updates_per_s": plain, "speedup_vs_plain": batched / plain,
+            "plain_cortex_loop_synapse_updates_per_s": loop,
+            "speedup_vs_cortex_loop": batched / loop,
+            "cortex_loop_ms_per_cycle": 1000 * loop_s / n_loop,
             "batched_ms_per_cycle": ms / n_cycles, "plain_ms_per_cycle": 1000 * plain_s / n_plain,
             "tuning": best, "stages": stages,
             "hbm_frac_cycle": cyc_alg * n_cycles / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,

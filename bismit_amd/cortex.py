@@ -365,6 +365,24 @@ class Cortex:
                 area.intake(inputs[name])
             area.cycle(1)
 
+    def cycle_device(self, dev_frames, lens, n_frames, n_cycles, offsets=None):
+        """n_cycles host-driven cycles in C (bsm_cortex_cycle): each cycle, each
+        area takes frame (c % n_frames) of its device frame ring and one cycle."""
+        cycle_areas(list(self.areas.values()), dev_frames, lens, n_frames, n_cycles, offsets)
+
     def close(self):
         for a in self.areas.values():
             a.close()
+
+
+def cycle_areas(areas, dev_frames, lens, n_frames, n_cycles, offsets=None):
+    """bsm_cortex_cycle over CorticalArea objects (device frame rings, one per area)."""
+    n = len(areas)
+    if offsets is None:
+        offsets = [a.scheme.input_range()[0] for a in areas]
+    ah = (C.c_void_p * n)(*[a.h.value for a in areas])
+    fr = (C.c_void_p * n)(*[p.value if p is not None else None for p in dev_frames])
+    st = (C.c_uint64 * n)(*lens)
+    ln = (C.c_uint32 * n)(*lens)
+    of = (C.c_uint32 * n)(*offsets)
+    L.check(areas[0].lib.bsm_cortex_cycle(ah, n, fr, st, ln, of, n_frames, n_cycles))

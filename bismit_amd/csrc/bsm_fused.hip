@@ -1327,6 +1327,35 @@ int bsm_area_cycle(bsm_area* a, const int32_t* rnd, uint32_t n_cycles) {
     return 0;
 }
 
+int bsm_cortex_cycle(bsm_area* const* areas, uint32_t n_areas, const uint8_t* const* dev_frames,
+                     const uint64_t* stride, const uint32_t* len, const uint32_t* axn_offset,
+                     uint32_t n_frames, uint32_t n_cycles) {
+    if (!areas && n_areas) return fail("null areas");
+    for (uint32_t i = 0; i < n_areas; i++) {
+        bsm_area* a = areas[i];
+        if (!a) return fail("null area");
+        if (a->batch_stream) return fail("area belongs to a batch; cycle it through the batch");
+        if (dev_frames && dev_frames[i] && len && len[i] &&
+            ((axn_offset ? axn_offset[i] : 0) > a->axn_len ||
+             len[i] > a->axn_len - (axn_offset ? axn_offset[i] : 0)))
+            return fail("frame range out of the axon space of area " + std::to_string(i));
+    }
+    for (uint32_t c = 0; c < n_cycles; c++) {
+        for (uint32_t i = 0; i < n_areas; i++) {
+            bsm_area* a = areas[i];
+            BSM_HIP(hipSetDevice(a->ctx->device));
+            if (dev_frames && dev_frames[i] && len && len[i] && n_frames) {
+                const uint64_t st = stride ? stride[i] : len[i];
+                if (launch_intake(a->stream, a->axn + (axn_offset ? axn_offset[i] : 0),
+                                  dev_frames[i] + (size_t)(c % n_frames) * st, len[i]))
+                    return 1;
+            }
+            if (bsm_area_cycle(a, nullptr, 1)) return 1;
+        }
+    }
+    return 0;
+}
+
 int bsm_area_run(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
                  uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles) {
     if (!a) return fail("null area");

@@ -209,6 +209,14 @@ int bsm_area_cycle(bsm_area* area, const int32_t* rnd, uint32_t n_cycles);
  * a device pointer) is copied to axn_offset before cycle i. */
 int bsm_area_run(bsm_area* area, uint32_t axn_offset, const uint8_t* dev_frames,
                  uint64_t frame_stride, uint32_t frame_len, uint32_t n_cycles);
+/* Cortex::cycle (src/cortex/cortex.rs:100-120) over plain-form areas, driven from
+ * the host: for each of n_cycles cycles, each area in turn takes its frame
+ * (len[i] bytes at dev_frames[i] + (c % n_frames) * stride[i], device memory,
+ * copied to axn_offset[i]; no intake where dev_frames or dev_frames[i] is NULL)
+ * and one bsm_area_cycle on its own stream.  Returns once enqueued. */
+int bsm_cortex_cycle(bsm_area* const* areas, uint32_t n_areas, const uint8_t* const* dev_frames,
+                     const uint64_t* stride, const uint32_t* len, const uint32_t* axn_offset,
+                     uint32_t n_frames, uint32_t n_cycles);
 /* Draw n_cycles*n_slots values from the area's own stream (host side) without
  * running anything; advances the stream exactly as bsm_area_cycle(NULL) would. */
 int bsm_area_draw_rnd(bsm_area* area, int32_t* out, uint32_t n_cycles);

@@ -117,6 +117,34 @@ def test_batch_profile_stages_advances_like_cycles(ctx):
         ctx.free(d)
 
 
+def test_cortex_cycle_matches_per_area_calls(ctx):
+    """bsm_cortex_cycle (host-driven cycle of several plain areas) equals each
+    area's own intake + cycle calls."""
+    from bismit_amd.cortex import cycle_areas
+    schemes = uneven_schemes()[:3]
+    n = 7
+    looped = make_areas(ctx, schemes, 900)
+    single = make_areas(ctx, schemes, 900)
+    frames = [frames_for(s, 4, 90 + i) for i, s in enumerate(schemes)]
+    devs = []
+    for f in frames:
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        devs.append(d)
+    lens = [f.shape[1] for f in frames]
+    cycle_areas(looped, devs, lens, 4, n)
+    for i, a in enumerate(single):
+        for c in range(n):
+            a.intake(frames[i][c % 4])
+            a.cycle(1)
+    for i, s in enumerate(schemes):
+        assert_same(looped[i], single[i], s, f"cortex loop area {i}")
+    for a in looped + single:
+        a.close()
+    for d in devs:
+        ctx.free(d)
+
+
 def test_config3_plan_uses_fast_wide_variant(ctx):
     """Every C3 tuft has synapses per dendrite a multiple of 4 and offsets within
     6 columns: both persistent tuft launches must take the check-free gather at
