@@ -39,21 +39,28 @@ class SdrStream:
             raise ValueError("density too high for the requested overlap")
         self.rng = np.random.default_rng(seed)
         self.active = self.rng.choice(self.n, self.k, replace=False)
+        self.member = np.zeros(self.n, dtype=bool)   # membership of self.active
+        self.member[self.active] = True
 
     def next_active(self):
-        kept = self.rng.choice(self.active, self.keep, replace=False)
+        """The current active set; the next one keeps `keep` of it (a uniform
+        random subset) and draws the rest uniformly from outside it."""
+        kept = self.active[self.rng.permutation(self.k)[:self.keep]]
         need = self.k - self.keep
         fresh = np.empty(0, dtype=np.int64)
-        # fresh axons come from outside the previous set (rejection sampling)
+        taken = self.member.copy()
+        # rejection sampling against the previous set and the draws so far
         while fresh.size < need:
-            cand = self.rng.integers(0, self.n, 2 * need + 8)
-            cand = cand[~np.isin(cand, self.active)]
-            cand = np.concatenate([fresh, cand])
+            cand = self.rng.integers(0, self.n, 2 * (need - fresh.size) + 8)
+            cand = cand[~taken[cand]]
             _, first = np.unique(cand, return_index=True)
-            fresh = cand[np.sort(first)]
-        fresh = fresh[:need]
+            cand = cand[np.sort(first)][:need - fresh.size]
+            taken[cand] = True
+            fresh = np.concatenate([fresh, cand])
         cur = self.active
+        self.member[cur] = False
         self.active = np.concatenate([kept, fresh])
+        self.member[self.active] = True
         return cur
 
     def next_frame(self):

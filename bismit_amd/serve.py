@@ -19,7 +19,9 @@ The reference has no serving mode; its single-threaded flywheel event loop
 (src/flywheel.rs:235-377) is the nearest analogue.
 """
 import math
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List
 
@@ -71,7 +73,7 @@ def area_scheme(side, name):
 class Session:
     def __init__(self, spec: SessionSpec, ctx, t_admit, keep_frames=False):
         self.spec = spec
-        self.t0 = t_admit
+        self.t0 = t_admit      # reset when the session joins the batch
         self.schemes = [area_scheme(s, f"s{spec.sid}a{i}") for i, s in enumerate(spec.sides)]
         self.areas = []
         for i, sc in enumerate(self.schemes):
@@ -108,6 +110,8 @@ class ServeStats:
     max_resident: int = 0
     rebuilds: int = 0
     mean_active_areas: float = 0.0
+    rebuild_s: list = field(default_factory=list)
+    setup_s: list = field(default_factory=list)
 
     def summary(self):
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
@@ -124,12 +128,15 @@ class ServeStats:
             "batch_rebuilds": self.rebuilds,
             "mean_active_areas_per_cycle": self.mean_active_areas,
             "wall_s": self.wall_s,
+            "rebuild_ms_max": max(self.rebuild_s, default=0.0) * 1e3,
+            "rebuild_ms_mean": float(np.mean(self.rebuild_s) * 1e3) if self.rebuild_s else 0.0,
+            "session_setup_ms_mean": float(np.mean(self.setup_s) * 1e3) if self.setup_s else 0.0,
         }
 
 
 class Server:
     def __init__(self, ctx, hbm_budget_bytes=96 << 30, max_sessions=64, epoch_s=0.05,
-                 use_graph=True, keep_frames=False, on_retire=None):
+                 use_graph=True, keep_frames=False, on_retire=None, setup_workers=2):
         self.ctx = ctx
         self.budget = hbm_budget_bytes
         self.max_sessions = max_sessions
@@ -140,6 +147,12 @@ class Server:
         self.resident = []
         self.used = 0
         self.batch = None
+        # sessions are set up (areas allocated and laid out on the device) by
+        # worker threads while the loop keeps cycling; the ctypes calls release
+        # the interpreter lock, and each area works on its own stream
+        self.setup_workers = setup_workers
+        self._ready = []
+        self._ready_lock = threading.Lock()
 
     def footprint(self, spec):
         return sum(footprint(area_scheme(s, "f")) for s in spec.sides)
@@ -152,12 +165,25 @@ class Server:
         if areas:
             self.batch = Batch(areas, self.ctx, dynamic=True)
 
+    def _setup(self, spec, fp, st):
+        t0 = time.perf_counter()
+        sess = Session(spec, self.ctx, 0.0, self.keep_frames)
+        sess.fp = fp
+        st.setup_s.append(time.perf_counter() - t0)
+        with self._ready_lock:
+            self._ready.append(sess)
+
     def run(self, specs, wall_limit_s=30.0, time_scale=1.0):
         """Serve `specs` (arrival times in seconds, compressed by time_scale) until
         every admitted session finished or wall_limit_s passed."""
+        with ThreadPoolExecutor(max_workers=self.setup_workers) as pool:
+            return self._run(specs, wall_limit_s, time_scale, pool)
+
+    def _run(self, specs, wall_limit_s, time_scale, pool):
         st = ServeStats()
         pending = sorted(specs, key=lambda s: s.arrival)
         waiting = []
+        setting_up = []           # futures of sessions being set up
         t_start = time.perf_counter()
         clock = lambda: time.perf_counter() - t_start
         last_epoch = -1e9
@@ -186,23 +212,33 @@ class Server:
                         st.completed += 1
                     self.resident = [s for s in self.resident if not s.finished()]
                     changed = True
-                while waiting and len(self.resident) < self.max_sessions:
+                # admission: HBM is reserved now, the set-up runs in the background
+                while waiting and len(self.resident) + len(setting_up) < self.max_sessions:
                     fp = self.footprint(waiting[0])
                     if self.used + fp > self.budget:
                         break
                     spec = waiting.pop(0)
-                    sess = Session(spec, self.ctx, clock(), self.keep_frames)
-                    sess.fp = fp
                     self.used += fp
-                    self.resident.append(sess)
+                    setting_up.append(pool.submit(self._setup, spec, fp, st))
                     st.admitted += 1
+                # sessions whose set-up finished join the batch
+                for f in [f for f in setting_up if f.done()]:
+                    f.result()           # re-raise a set-up error here
+                    setting_up.remove(f)
+                with self._ready_lock:
+                    ready, self._ready = self._ready, []
+                for sess in ready:
+                    sess.t0 = clock()
+                    self.resident.append(sess)
                     changed = True
             if changed:
+                t_rb = time.perf_counter()
                 self._rebuild()
+                st.rebuild_s.append(time.perf_counter() - t_rb)
                 st.rebuilds += 1
                 st.max_resident = max(st.max_resident, len(self.resident))
             if not self.resident:
-                if not pending and not waiting:
+                if not pending and not waiting and not setting_up:
                     break
                 nxt = pending[0].arrival * time_scale if pending else now + 1e-3
                 time.sleep(max(0.0, min(nxt - now, 0.01)))
@@ -250,7 +286,12 @@ class Server:
             st.synapse_updates += syn
         st.wall_s = clock()
         st.mean_active_areas = act_sum / max(st.cycles, 1)
-        # drain
+        # drain: sessions still being set up are closed once ready
+        for f in setting_up:
+            f.result()
+        with self._ready_lock:
+            self.resident.extend(self._ready)
+            self._ready = []
         if self.batch is not None:
             self.batch.close()
             self.batch = None
