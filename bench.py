@@ -43,20 +43,6 @@ def stage_bytes(scheme, li, kind):
     return None
 
 
-def batch_stages(schemes):
-    """(kind, [(area, layer)]) of every persistent launch of a static batch in
-    launch order: spatial layers then pyramidal ones, by rank within the area;
-    per rank one F1 launch, then F2 (spatial) or F3 (pyramidal)."""
-    out = []
-    for spatial in (True, False):
-        per = [[li for li, l in enumerate(s.layers) if l.spatial == spatial] for s in schemes]
-        for k in range(max(len(p) for p in per)):
-            mem = [(ai, p[k]) for ai, p in enumerate(per) if k < len(p)]
-            out.append((1, mem))
-            out.append((2 if spatial else 3, mem))
-    return out
-
-
 def read_traffic():
     """HBM bytes per dominant-kernel launch from the committed rocprof PMC summary."""
     try:
@@ -99,7 +85,7 @@ def cpu_baseline(scheme, n_cycles, frames):
                       f"oracle/bsm_oracle.c with OpenMP over {threads} threads, {dt:.1f} s"}
 
 
-def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
+def batched_leg(ctx, n_cycles=200, runs=3, tune=True, serial=False):
     """Config 3: 64 uneven areas in one persistent launch per stage with a
     hipGraph-captured cycle, against the same areas cycled one by one."""
     import numpy as np
@@ -119,34 +105,48 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
         lens.append(f.shape[1])
     # plain per-area form: every area launches its own stages on its own stream
     # (the reference gives each area its own queue); all areas' cycles are
-    # enqueued up front so host overhead stays out and streams may overlap
+    # enqueued up front so host overhead stays out and streams may overlap.
+    # Both stream modes of the plain form are timed and the faster one is the
+    # baseline (a second stream per area helps one large area, and hurts when
+    # 64 areas' streams share the hardware queues).
+    from bismit_amd.cortex import cycle_areas
     for i, a in enumerate(areas):
         a.run(devs[i], lens[i], lens[i], 3)
     ctx.sync()
     n_plain = max(20, n_cycles // 4)
-    t0 = time.perf_counter()
-    for i, a in enumerate(areas):
-        a.run(devs[i], lens[i], lens[i], min(n_plain, nf))
-        if n_plain > nf:
-            a.run(devs[i], lens[i], lens[i], n_plain - nf)
-    for a in areas:
-        a.sync()
-    ctx.sync()
-    plain_s = time.perf_counter() - t0
-    plain = n_syn * n_plain / plain_s
-    # the reference's driving pattern (Cortex::cycle): every cycle, each area's
-    # intake and cycle enqueued in turn by one host thread (C loop, no Python)
-    from bismit_amd.cortex import cycle_areas
     n_loop = max(10, n_cycles // 10)
-    ctx.sync()
-    t0 = time.perf_counter()
-    cycle_areas(areas, devs, lens, nf, n_loop)
+    plain_t, loop_t = {}, {}
+    for mode in (1, 2):
+        for i, a in enumerate(areas):
+            a.set_tuning(overlap=mode)
+            a.run(devs[i], lens[i], lens[i], 2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for i, a in enumerate(areas):
+            a.run(devs[i], lens[i], lens[i], min(n_plain, nf))
+            if n_plain > nf:
+                a.run(devs[i], lens[i], lens[i], n_plain - nf)
+        for a in areas:
+            a.sync()
+        ctx.sync()
+        plain_t[mode] = (time.perf_counter() - t0) / n_plain
+        # the reference's driving pattern (Cortex::cycle): every cycle, each area's
+        # intake and cycle enqueued in turn by one host thread (C loop, no Python)
+        t0 = time.perf_counter()
+        cycle_areas(areas, devs, lens, nf, n_loop)
+        for a in areas:
+            a.sync()
+        ctx.sync()
+        loop_t[mode] = (time.perf_counter() - t0) / n_loop
     for a in areas:
-        a.sync()
-    ctx.sync()
-    loop_s = time.perf_counter() - t0
-    loop = n_syn * n_loop / loop_s
+        a.set_tuning(overlap=1)
+    plain_s = min(plain_t.values())
+    plain = n_syn / plain_s
+    loop_s = min(loop_t.values())
+    loop = n_syn / loop_s
     b = Batch(areas, ctx)
+    if serial:
+        b.set_tuning(overlap=2)
     b.set_inputs(devs, lens, lens, [s.input_range()[0] for s in schemes], nf)
     best, table = None, []
     if tune:
@@ -168,15 +168,14 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
     kinds = {1: "kf_tuft", 2: "kf_ssc_post", 3: "kf_pyr_post"}
     stages = []
     plan = b.stage_plan()
-    for (k, t), (k2, mem), p in zip(prof, batch_stages(schemes), plan):
-        assert k == k2 == p["kind"]
-        st = {"kernel": kinds[k], "layers": len(mem), "avg_us": round(1000.0 * t / n_prof, 2),
+    for (k, t), p in zip(prof, plan):
+        assert k == p["kind"]
+        st = {"kernel": kinds[k], "stream": p["stream"], "avg_us": round(1000.0 * t / n_prof, 2),
               "chunk": p["chunk"], "fast": p["fast"], "tile": [p["tile_v"], p["tile_u"]],
               "items": p["n_items"]}
         if k == 1:
-            alg = sum(stage_bytes(schemes[ai], li, 1) for ai, li in mem)
-            st["alg_bytes_per_launch"] = alg
-            st["hbm_frac"] = round(alg / (t / n_prof / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+            st["alg_bytes_per_launch"] = p["alg_bytes"]
+            st["hbm_frac"] = round(p["alg_bytes"] / (t / n_prof / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
         stages.append(st)
     cyc_alg = sum(st.get("alg_bytes_per_launch", 0) for st in stages)
     b.close()
@@ -190,8 +189,12 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True):
             "plain_per_area_synapse_updates_per_s": plain, "speedup_vs_plain": batched / plain,
             "plain_cortex_loop_synapse_updates_per_s": loop,
             "speedup_vs_cortex_loop": batched / loop,
-            "cortex_loop_ms_per_cycle": 1000 * loop_s / n_loop,
-            "batched_ms_per_cycle": ms / n_cycles, "plain_ms_per_cycle": 1000 * plain_s / n_plain,
+            "cortex_loop_ms_per_cycle": 1000 * loop_s,
+            "batched_ms_per_cycle": ms / n_cycles, "plain_ms_per_cycle": 1000 * plain_s,
+            "plain_ms_per_cycle_by_mode": {"side_stream": 1000 * plain_t[1],
+                                           "one_stream": 1000 * plain_t[2]},
+            "cortex_loop_ms_per_cycle_by_mode": {"side_stream": 1000 * loop_t[1],
+                                                 "one_stream": 1000 * loop_t[2]},
             "tuning": best, "stages": stages,
             "hbm_frac_cycle": cyc_alg * n_cycles / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
             "hbm_frac_at_5B_per_synapse": batched * 5 / (HBM_PEAK_GBS * 1e9)}
@@ -223,6 +226,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"),
                     help="fixed column tile of the plain form instead of the autotuner's")
+    ap.add_argument("--serial", action="store_true",
+                    help="one stream per area/batch (no side stream); for profiler runs")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling helper: run only the 64-area batched leg and print it")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -239,7 +244,8 @@ def main():
 
     ctx = Context(0)
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(ctx, n_cycles=args.steps, runs=args.runs)}),
+        print(json.dumps({"batched": batched_leg(ctx, n_cycles=args.steps, runs=args.runs,
+                                                 serial=args.serial)}),
               flush=True)
         return
     scheme = recipes.config2_area()
@@ -253,6 +259,8 @@ def main():
     if args.tile:
         tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
         area.set_tuning(**tuning)
+    if args.serial:
+        area.set_tuning(overlap=2)
     if args.pmc_probe:
         n = 256 << 20
         a, b = ctx.alloc(n), ctx.alloc(n)
@@ -282,24 +290,38 @@ def main():
     n_syn = scheme.n_synapses()
     value = n_syn * args.steps / (ms / 1000.0)
 
-    # roofline of the dominant kernel: HIP events around every launch
-    prof = area.profile_stages(50)
+    # roofline of the dominant kernel: HIP events around every launch, launched one
+    # at a time on the one-stream schedule (each layer's tufts in one launch, as
+    # in the committed rocprof summaries); the timed runs above use two streams
+    n_prof = 50
+    schedule = [(p["kind"], p["stream"]) for p in area.stage_plan()]
+    area.set_tuning(overlap=2)
+    prof = area.profile_stages(n_prof)
     kinds = {1: "kf_tuft", 2: "kf_ssc_post", 3: "kf_pyr_post"}
     plan = area.stage_plan()
-    stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name,
-               "avg_us": round(1000.0 * t / 50, 2), "chunk": p["chunk"], "fast": p["fast"],
-               "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"]}
+    stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name, "stream": p["stream"],
+               "avg_us": round(1000.0 * t / n_prof, 2), "chunk": p["chunk"], "fast": p["fast"],
+               "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"],
+               "alg_bytes_per_launch": p["alg_bytes"]}
               for (k, li, t), p in zip(prof, plan)]
-    dom_i = max(range(len(prof)), key=lambda i: prof[i][2])
-    dk, dl, dt = prof[dom_i]
-    avg_s = dt / 50 / 1000.0
-    alg = stage_bytes(scheme, dl, dk)
+    groups = {}
+    for (k, li, t), p in zip(prof, plan):
+        g = groups.setdefault((k, li), [0.0, 0, 0])
+        g[0] += t
+        g[1] += p["alg_bytes"]
+        g[2] += 1
+    area.set_tuning(overlap=2 if args.serial else 1)
+    (dk, dl), (dt, alg_tot, n_launch) = max(groups.items(), key=lambda kv: kv[1][0])
+    assert alg_tot == stage_bytes(scheme, dl, dk)   # the engine's count equals the recipe's
+    avg_s = dt / n_prof / n_launch / 1000.0
+    alg = alg_tot / n_launch
     achieved = alg / avg_s / 1e9
     traffic = read_traffic()
     ceil = stream_ceiling(ctx)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
+                "launches_per_cycle": n_launch,
                 "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
                 "stream_ceiling": ceil, "frac_of_stream_ceiling": round(achieved / ceil["gbs"], 4)}
 
@@ -323,6 +345,7 @@ def main():
                    "device": ctx.info()["name"]},
         "roofline": roofline,
         "stages": stages,
+        "timed_schedule": [{"kernel": kinds[k], "stream": st} for k, st in schedule],
     }
     if not args.no_batched:
         out["batched"] = batched_leg(ctx)

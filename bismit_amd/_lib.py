@@ -69,14 +69,14 @@ class Tract(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_v", C.c_uint32), ("tile_u", C.c_uint32), ("halo", C.c_uint32),
-                ("chunk", C.c_uint32), ("inhib_tile", C.c_uint32)]
+                ("chunk", C.c_uint32), ("inhib_tile", C.c_uint32), ("overlap", C.c_uint32)]
 
 
 class StageInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("layer", C.c_uint32), ("chunk", C.c_uint32),
                 ("staged", C.c_uint32), ("fast", C.c_uint32), ("tile_v", C.c_uint32),
                 ("tile_u", C.c_uint32), ("n_items", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("grid", C.c_uint32)]
+                ("grid", C.c_uint32), ("stream", C.c_uint32), ("alg_bytes", C.c_uint64)]
 
 
 def stage_dicts(arr, n):

@@ -193,16 +193,17 @@ class CorticalArea:
         L.check(self.lib.bsm_area_stage_plan(self.h, 64, arr, C.byref(n)))
         return L.stage_dicts(arr, n.value)
 
-    def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0, inhib_tile=0):
-        """Zero keeps a field; inhib_tile=1 returns it to the planner's choice."""
-        t = L.Tuning(tile_v, tile_u, halo, chunk, inhib_tile)
+    def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0, inhib_tile=0, overlap=0):
+        """Zero keeps a field; inhib_tile=1 returns it to the planner's choice;
+        overlap 1 runs independent pyramidal tufts on a second stream, 2 does not."""
+        t = L.Tuning(tile_v, tile_u, halo, chunk, inhib_tile, overlap)
         L.check(self.lib.bsm_area_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
         t = L.Tuning()
         L.check(self.lib.bsm_area_get_tuning(self.h, C.byref(t)))
         return {"tile_v": t.tile_v, "tile_u": t.tile_u, "halo": t.halo, "chunk": t.chunk,
-                "inhib_tile": t.inhib_tile}
+                "inhib_tile": t.inhib_tile, "overlap": t.overlap}
 
     def sampler(self, kind, layer=0):
         """A callable returning the current contents of one buffer
@@ -327,9 +328,10 @@ class Batch:
         L.check(self.lib.bsm_batch_profile_stages(self.h, n_cycles, m, ms, kind, C.byref(n)))
         return [(kind[i], ms[i]) for i in range(n.value)]
 
-    def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0, inhib_tile=0):
-        """Zero keeps a field; inhib_tile=1 returns it to the planner's choice."""
-        t = L.Tuning(tile_v, tile_u, halo, chunk, inhib_tile)
+    def set_tuning(self, tile_v=0, tile_u=0, halo=0, chunk=0, inhib_tile=0, overlap=0):
+        """Zero keeps a field; inhib_tile=1 returns it to the planner's choice;
+        overlap 1 runs independent pyramidal tufts on a second stream, 2 does not."""
+        t = L.Tuning(tile_v, tile_u, halo, chunk, inhib_tile, overlap)
         L.check(self.lib.bsm_batch_set_tuning(self.h, C.byref(t)))
 
     def sync(self):

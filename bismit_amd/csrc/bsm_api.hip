@@ -277,6 +277,30 @@ void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, si
     }
 }
 
+int side_init(Side& s) {
+    BSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    BSM_HIP(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
+    return 0;
+}
+
+int side_events(Side& s, size_t n_stages) {
+    while (s.join.size() < n_stages) {
+        hipEvent_t e;
+        BSM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s.join.push_back(e);
+    }
+    return 0;
+}
+
+void side_destroy(Side& s) {
+    for (hipEvent_t e : s.join) (void)hipEventDestroy(e);
+    s.join.clear();
+    if (s.fork) (void)hipEventDestroy(s.fork);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.fork = nullptr;
+    s.stream = nullptr;
+}
+
 // A tuning update: zero fields keep the current value; inhib_tile 1 returns
 // the inhibition tile to the planner's choice.
 int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
@@ -286,6 +310,7 @@ int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
     if (t->halo) nt.halo = t->halo;
     if (t->chunk) nt.chunk = t->chunk;
     if (t->inhib_tile) nt.inhib_tile = t->inhib_tile == 1 ? 0u : t->inhib_tile;
+    if (t->overlap) nt.overlap = t->overlap == 1 ? 1u : 0u;
     if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
         return fail("tile too large");
     if (nt.halo > 64) return fail("halo too large");
@@ -293,6 +318,7 @@ int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
         return fail("chunk must be 1, 4, 8 or 16");
     if (nt.inhib_tile != 0 && nt.inhib_tile != 8 && nt.inhib_tile != 16 && nt.inhib_tile != 32)
         return fail("inhib_tile must be 1 (planner's choice), 8, 16 or 32");
+    if (t->overlap > 2) return fail("overlap must be 0 (keep), 1 (on) or 2 (off)");
     *out = nt;
     return 0;
 }
@@ -442,6 +468,7 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->plan_dirty = true;
     a->batch_stream = nullptr;
     a->layout_epoch = 0;
+    if (side_init(a->side)) return 1;
     *out = a;
     return 0;
 }
@@ -463,6 +490,7 @@ int bsm_area_destroy(bsm_area* a) {
     }
     (void)hipEventDestroy(a->ev0);
     (void)hipEventDestroy(a->ev1);
+    side_destroy(a->side);
     (void)hipStreamDestroy(a->stream);
     delete a;
     return 0;
@@ -688,6 +716,7 @@ int bsm_area_get_tuning(bsm_area* a, bsm_tuning* t) {
     t->halo = a->tune.halo;
     t->chunk = a->tune.chunk;
     t->inhib_tile = a->tune.inhib_tile ? a->tune.inhib_tile : 1u;
+    t->overlap = a->tune.overlap ? 1u : 2u;
     return 0;
 }
 

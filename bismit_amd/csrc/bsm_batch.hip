@@ -132,6 +132,7 @@ struct bsm_batch {
     size_t xfer_h_cap[2] = {0, 0};
     hipEvent_t xfer_ev[2];
     int xfer_next = 0;
+    bsm::Side side;             // second stream of the cycle (see enqueue_stages)
     bsm::TractRt* tracts_d = nullptr;
     uint32_t n_tracts = 0;
     uint32_t io_parts = 1;      // prologue blocks per area (largest axon space / COPY_SEG)
@@ -170,9 +171,19 @@ static int batch_plan(bsm_batch* b) {
         max_rank[0] = std::max(max_rank[0], r[0]);
         max_rank[1] = std::max(max_rank[1], r[1]);
     }
+    // per area: slices written in a cycle before its pyramidal tufts of the
+    // current rank run (spatial outputs, then lower-rank pyramidal outputs)
+    std::vector<SliceSet> early(na);
+    for (uint32_t ai = 0; ai < na; ai++) {
+        bsm_area* a = b->areas[ai];
+        if (a->settings & BSM_SET_DISABLE_SSCS) continue;
+        for (uint32_t l = 0; l < a->n_layers; l++)
+            if (a->L[l].d.spatial) early[ai].add_range(a->L[l].d.base_axn_slc, a->L[l].d.depth);
+    }
     for (int pass = 0; pass < 2; pass++) {
         for (uint32_t k = 0; k < max_rank[pass]; k++) {
-            std::vector<uint32_t> t_ids, t_w, s_ids, p_ids;
+            std::vector<uint32_t> t_ids, t_w, s_ids, p_ids, side_ids, side_w;
+            std::vector<std::pair<uint32_t, uint32_t>> ranked;  // (area, layer) of this rank
             uint32_t w3 = 16;
             for (uint32_t ai = 0; ai < na; ai++) {
                 bsm_area* a = b->areas[ai];
@@ -188,12 +199,15 @@ static int batch_plan(bsm_batch* b) {
                     // first batch cycle runs with par = 1 and must write buffer cur ^ 1
                     const uint32_t px = (uint32_t)L.cur;
                     const uint32_t* dw = b->dynamic ? b->dyn_d + ai * BSM_MAX_LAYERS + l : nullptr;
+                    ranked.push_back({ai, l});
                     for (uint32_t t = 0; t < L.d.n_tufts; t++) {
                         TuftJob j = make_tuft_job(a, l, t, b->tune, a->rnd_d);
                         j.par_xor = px;
                         j.dyn = dw;
-                        t_ids.push_back((uint32_t)b->tjobs.size());
-                        t_w.push_back(tuft_chunk(a, l, t, b->tune.chunk));
+                        const bool side = !spatial && b->tune.overlap &&
+                                          !early[ai].meets(L.t[t].slc_bits);
+                        (side ? side_ids : t_ids).push_back((uint32_t)b->tjobs.size());
+                        (side ? side_w : t_w).push_back(tuft_chunk(a, l, t, b->tune.chunk));
                         b->tjobs.push_back(j);
                     }
                     if (spatial) {
@@ -213,10 +227,26 @@ static int batch_plan(bsm_batch* b) {
                     }
                 }
             }
+            if (t_ids.empty()) {
+                t_ids.swap(side_ids);
+                t_w.swap(side_w);
+            }
             if (t_ids.empty()) continue;
+            int32_t join = -1;
+            if (!side_ids.empty()) {
+                Stage sd{};
+                if (plan_f1(b->tjobs, side_ids, side_w, b->tune, items, sd)) return 1;
+                sd.stream = 1;
+                join = (int32_t)b->stages.size();
+                b->stages.push_back(sd);
+            }
             Stage s1{};
             if (plan_f1(b->tjobs, t_ids, t_w, b->tune, items, s1)) return 1;
             b->stages.push_back(s1);
+            if (pass == 1)
+                for (auto& al : ranked)
+                    early[al.first].add_range(b->areas[al.first]->L[al.second].d.base_axn_slc,
+                                              b->areas[al.first]->L[al.second].d.depth);
             Stage s2{};
             if (pass == 0) {
                 if (plan_f2(b->sjobs, s_ids, b->tune, items, s2)) return 1;
@@ -231,10 +261,12 @@ static int batch_plan(bsm_batch* b) {
                 }
                 if (plan_f3(b->pjobs, p_ids, items, s2)) return 1;
                 s2.W = w3;
+                s2.join = join;
             }
             b->stages.push_back(s2);
         }
     }
+    if (side_events(b->side, b->stages.size())) return 1;
     (void)hipFree(b->tjobs_d);
     (void)hipFree(b->sjobs_d);
     (void)hipFree(b->pjobs_d);
@@ -263,12 +295,21 @@ static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
     }
     kb_prologue<<<dim3(na, b->io_parts), 256, 0, b->stream>>>(b->io_d, b->counter_d);
     BSM_HIP(hipGetLastError());
-    for (size_t i = 0; i < b->stages.size(); i++) {
-        if (ev) BSM_HIP(hipEventRecord(ev[2 * i], b->stream));
-        if (launch_stage(b->stream, b->stages[i], b->tjobs_d, b->sjobs_d, b->pjobs_d, b->items_d,
-                         par, 0))
-            return 1;
-        if (ev) BSM_HIP(hipEventRecord(ev[2 * i + 1], b->stream));
+    if (ev) {
+        // profiling: one stream, stages in list order, each launch timed alone
+        for (size_t i = 0; i < b->stages.size(); i++) {
+            BSM_HIP(hipEventRecord(ev[2 * i], b->stream));
+            if (launch_stage(b->stream, b->stages[i], b->tjobs_d, b->sjobs_d, b->pjobs_d,
+                             b->items_d, par, 0))
+                return 1;
+            BSM_HIP(hipEventRecord(ev[2 * i + 1], b->stream));
+        }
+    } else if (enqueue_stages(b->stream, b->side, b->stages, false,
+                              [&](const Stage& st, hipStream_t s) {
+                                  return launch_stage(s, st, b->tjobs_d, b->sjobs_d, b->pjobs_d,
+                                                      b->items_d, par, 0);
+                              })) {
+        return 1;
     }
     kb_tick<<<1, 1, 0, b->stream>>>(b->counter_d);
     BSM_HIP(hipGetLastError());
@@ -384,6 +425,7 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
     b->ctx = ctx;
     b->areas.assign(areas, areas + n_areas);
     BSM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    if (side_init(b->side)) return 1;
     BSM_HIP(hipEventCreate(&b->ev0));
     BSM_HIP(hipEventCreate(&b->ev1));
     BSM_HIP(hipMalloc((void**)&b->rng_d, sizeof(uint32_t) * 4 * n_areas));
@@ -457,6 +499,7 @@ int bsm_batch_destroy(bsm_batch* b) {
     bsm_batch_sync(b);
     (void)hipSetDevice(b->ctx->device);
     free_graphs(b);
+    if (side_events(b->side, b->stages.size())) return 1;
     (void)hipFree(b->tjobs_d);
     (void)hipFree(b->sjobs_d);
     (void)hipFree(b->pjobs_d);
@@ -473,6 +516,7 @@ int bsm_batch_destroy(bsm_batch* b) {
     (void)hipFree(b->counter_d);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
+    side_destroy(b->side);
     (void)hipStreamDestroy(b->stream);
     for (bsm_area* a : b->areas) {
         a->plan_dirty = true;

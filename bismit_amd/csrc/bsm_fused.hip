@@ -1030,6 +1030,15 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     }
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
+    st.alg_bytes = 0;
+    for (uint32_t id : job_ids) {
+        // per synapse 3 B metadata + 1 B strength in, 1 B state out; per dendrite
+        // raw/state out + activity in/out; per cell-tuft 4 B in + 8 B out
+        // (pyramidal); per SSC cell its energy
+        const TuftJob& j = jobs[id];
+        st.alg_bytes += (uint64_t)j.C * j.spt * 5 + (uint64_t)j.C * j.dens * 4 +
+                        (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0);
+    }
     for (uint32_t id : job_ids) {
         const TuftJob& j = jobs[id];
         uint32_t ntv = (j.V + TV - 1) / TV, ntu = (j.U + TU - 1) / TU;
@@ -1128,8 +1137,13 @@ int build_plan(bsm_area* a) {
     a->pjobs.clear();
     a->stages.clear();
     std::vector<WorkItem> items;
+    a->flip_layers.clear();
     const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
     const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
+    // slices written in a cycle before the pyramidal tufts run
+    SliceSet early;
+    for (uint32_t l = 0; l < a->n_layers && sscs; l++)
+        if (a->L[l].d.spatial) early.add_range(a->L[l].d.base_axn_slc, a->L[l].d.depth);
     for (int pass = 0; pass < 2; pass++) {
         for (uint32_t l = 0; l < a->n_layers; l++) {
             LayerRt& L = a->L[l];
@@ -1137,16 +1151,34 @@ int build_plan(bsm_area* a) {
             if ((pass == 0) != spatial) continue;
             if (spatial && !sscs) continue;
             if (!spatial && !pyrs) continue;
-            std::vector<uint32_t> ids, ws;
+            // pyramidal tufts that read nothing written earlier in the cycle may
+            // run on the side stream from the cycle's start
+            std::vector<uint32_t> ids, ws, side_ids, side_ws;
             for (uint32_t t = 0; t < L.d.n_tufts; t++) {
-                ids.push_back((uint32_t)a->tjobs.size());
-                ws.push_back(tuft_chunk(a, l, t, a->tune.chunk));
+                const bool side = !spatial && a->tune.overlap && !early.meets(L.t[t].slc_bits);
+                (side ? side_ids : ids).push_back((uint32_t)a->tjobs.size());
+                (side ? side_ws : ws).push_back(tuft_chunk(a, l, t, a->tune.chunk));
                 a->tjobs.push_back(make_tuft_job(a, l, t, a->tune, a->rnd_d));
+            }
+            if (ids.empty()) {  // keep one main-stream tuft stage per layer
+                ids.swap(side_ids);
+                ws.swap(side_ws);
+            }
+            int32_t join = -1;
+            if (!side_ids.empty()) {
+                Stage sd{};
+                if (plan_f1(a->tjobs, side_ids, side_ws, a->tune, items, sd)) return 1;
+                sd.layer = l;
+                sd.stream = 1;
+                join = (int32_t)a->stages.size();
+                a->stages.push_back(sd);
             }
             Stage s1{};
             if (plan_f1(a->tjobs, ids, ws, a->tune, items, s1)) return 1;
             s1.layer = l;
             a->stages.push_back(s1);
+            a->flip_layers.push_back(l);
+            if (!spatial) early.add_range(L.d.base_axn_slc, L.d.depth);
             Stage s2{};
             if (spatial) {
                 std::vector<uint32_t> sid{(uint32_t)a->sjobs.size()};
@@ -1158,11 +1190,13 @@ int build_plan(bsm_area* a) {
                 a->pjobs.push_back(make_pyr_job(a, l, a->rnd_d, &w3));
                 if (plan_f3(a->pjobs, pid, items, s2)) return 1;
                 s2.W = w3;
+                s2.join = join;
             }
             s2.layer = l;
             a->stages.push_back(s2);
         }
     }
+    if (side_events(a->side, a->stages.size())) return 1;
     (void)hipFree(a->tjobs_d);
     (void)hipFree(a->sjobs_d);
     (void)hipFree(a->pjobs_d);
@@ -1214,6 +1248,8 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
         o.n_items = st.n_items;
         o.lds_bytes = st.lds_bytes;
         o.grid = st.grid;
+        o.stream = st.stream;
+        o.alg_bytes = st.alg_bytes;
     }
     return 0;
 }
@@ -1272,15 +1308,16 @@ int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len)
     return 0;
 }
 
-int cycle_fused(bsm_area* a, uint32_t rnd_ofs) {
-    for (const Stage& st : a->stages) {
-        LayerRt& L = a->L[st.layer];
-        if (st.kind == 1) L.cur ^= 1;
-        if (launch_stage(a->stream, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
-                         (uint32_t)L.cur, rnd_ofs))
-            return 1;
-    }
-    return 0;
+// One plain cycle: every cycled layer's state parity flips once, then the
+// stages run on the area stream (and its side stream).
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial) {
+    for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
+    return enqueue_stages(a->stream, a->side, a->stages, serial,
+                          [&](const Stage& st, hipStream_t s) {
+                              return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d,
+                                                  a->items_d, (uint32_t)a->L[st.layer].cur,
+                                                  rnd_ofs);
+                          });
 }
 
 // Upload n_cycles of rnd draws (user-given or drawn from the area stream) to
@@ -1323,7 +1360,7 @@ int bsm_area_cycle(bsm_area* a, const int32_t* rnd, uint32_t n_cycles) {
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (upload_rnd(a, rnd, n_cycles)) return 1;
     for (uint32_t c = 0; c < n_cycles; c++)
-        if (cycle_fused(a, c * a->n_slots)) return 1;
+        if (cycle_fused(a, c * a->n_slots, false)) return 1;
     return 0;
 }
 
@@ -1369,7 +1406,7 @@ int bsm_area_run(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
             launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
                           frame_len))
             return 1;
-        if (cycle_fused(a, c * a->n_slots)) return 1;
+        if (cycle_fused(a, c * a->n_slots, false)) return 1;
     }
     return 0;
 }
@@ -1389,7 +1426,7 @@ int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_fr
             launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
                           frame_len))
             return 1;
-        if (cycle_fused(a, c * a->n_slots)) return 1;
+        if (cycle_fused(a, c * a->n_slots, false)) return 1;
     }
     BSM_HIP(hipEventRecord(a->ev1, a->stream));
     BSM_HIP(hipEventSynchronize(a->ev1));
@@ -1439,10 +1476,11 @@ int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages,
     for (auto& e : ev) BSM_HIP(hipEventCreate(&e));
     BSM_HIP(hipStreamSynchronize(a->stream));
     for (uint32_t c = 0; c < n_cycles; c++) {
+        // one stream, stages in list order, so that each launch is timed alone
+        for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
         for (uint32_t i = 0; i < ns; i++) {
             const Stage& st = a->stages[i];
             LayerRt& L = a->L[st.layer];
-            if (st.kind == 1) L.cur ^= 1;
             BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i)], a->stream));
             if (launch_stage(a->stream, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
                              (uint32_t)L.cur, c * a->n_slots))

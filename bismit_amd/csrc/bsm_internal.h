@@ -144,6 +144,7 @@ struct LayerRt {
 struct Tuning {
     uint32_t tile_v = 8, tile_u = 8, halo = 8, chunk = 16;
     uint32_t inhib_tile = 0;  // side of the square inhibition tile; 0 = chosen by the planner
+    uint32_t overlap = 1;     // run independent pyramidal tufts on the side stream
 };
 
 // One launch of a fused stage over a contiguous range of work items.
@@ -158,6 +159,70 @@ struct Stage {
     uint32_t stage_cap;  // F1: LDS bytes reserved for staged windows
     uint32_t tv, tu;     // tile extents in columns
     uint32_t grid;
+    uint64_t alg_bytes = 0;  // F1: algorithmic HBM bytes of one launch
+    uint32_t stream = 0; // 0: main stream; 1: side stream, forked after the cycle's intake
+    int32_t join = -1;   // main-stream stages: index of the side stage to wait for, or -1
+};
+
+// A second stream for stages that may run beside the main sequence of a cycle
+// (pyramidal tufts whose sources nothing earlier in the cycle writes).
+struct Side {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;  // one per stage index (side stages record theirs)
+};
+int side_init(Side& s);
+int side_events(Side& s, size_t n_stages);
+void side_destroy(Side& s);
+
+// Enqueues one cycle's stages.  Side stages fork from `main` at this point and
+// record their join events; main stages wait for the side stage they name; any
+// side stage no main stage named is joined at the end, so work on `main` after
+// this call always follows every stage.  serial: all stages on `main` in order.
+template <class Launch>
+int enqueue_stages(hipStream_t main, Side& side, const std::vector<Stage>& stages, bool serial,
+                   Launch&& launch) {
+    bool any_side = false;
+    for (const Stage& st : stages) any_side |= st.stream == 1;
+    if (serial || !any_side) {
+        for (const Stage& st : stages)
+            if (launch(st, main)) return 1;
+        return 0;
+    }
+    if (side.join.size() < stages.size()) return fail("side-stream events not prepared");
+    BSM_HIP(hipEventRecord(side.fork, main));
+    BSM_HIP(hipStreamWaitEvent(side.stream, side.fork, 0));
+    std::vector<char> joined(stages.size(), 1);
+    for (size_t i = 0; i < stages.size(); i++) {
+        if (stages[i].stream != 1) continue;
+        if (launch(stages[i], side.stream)) return 1;
+        BSM_HIP(hipEventRecord(side.join[i], side.stream));
+        joined[i] = 0;
+    }
+    for (const Stage& st : stages) {
+        if (st.stream == 1) continue;
+        if (st.join >= 0 && (size_t)st.join < stages.size() && !joined[st.join]) {
+            BSM_HIP(hipStreamWaitEvent(main, side.join[st.join], 0));
+            joined[st.join] = 1;
+        }
+        if (launch(st, main)) return 1;
+    }
+    for (size_t i = 0; i < stages.size(); i++)
+        if (!joined[i]) BSM_HIP(hipStreamWaitEvent(main, side.join[i], 0));
+    return 0;
+}
+
+// Slices written during a cycle before a pyramidal layer's tufts run: the
+// spatial layers' axon slices and those of pyramidal layers planned earlier.
+struct SliceSet {
+    uint64_t bits[4] = {0, 0, 0, 0};
+    void add_range(uint32_t first, uint32_t n) {
+        for (uint32_t s = first; s < first + n && s < 256; s++) bits[s >> 6] |= 1ull << (s & 63);
+    }
+    bool meets(const uint64_t* other) const {
+        return (bits[0] & other[0]) | (bits[1] & other[1]) | (bits[2] & other[2]) |
+               (bits[3] & other[3]);
+    }
 };
 
 }  // namespace bsm
@@ -197,6 +262,8 @@ struct bsm_area {
     std::vector<bsm::SscJob> sjobs;
     std::vector<bsm::PyrJob> pjobs;
     std::vector<bsm::Stage> stages;
+    std::vector<uint32_t> flip_layers;  // layers whose state parity flips each cycle
+    bsm::Side side;
     bool plan_dirty;
     hipEvent_t ev0, ev1;
     hipStream_t batch_stream;  // non-null while the area belongs to a batch
@@ -226,7 +293,7 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
                uint32_t* n_stages);
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
-int cycle_fused(bsm_area* a, uint32_t rnd_ofs);
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial);
 int ensure_rnd(bsm_area* a, size_t n_ints);
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids);

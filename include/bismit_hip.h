@@ -229,6 +229,9 @@ typedef struct bsm_tuning {
     uint32_t chunk;      /* synapses per lane chunk: 4, 8 or 16 */
     uint32_t inhib_tile; /* side of the square inhibition tile: 8, 16 or 32;
                             1 = the planner's choice (the default) */
+    uint32_t overlap;    /* 1: pyramidal tufts whose sources no earlier stage of the
+                            cycle writes run on a second stream beside the spatial
+                            layers' stages (the default); 2: one stream */
 } bsm_tuning;
 int bsm_area_set_tuning(bsm_area* area, const bsm_tuning* t);
 int bsm_area_get_tuning(bsm_area* area, bsm_tuning* t);
@@ -269,6 +272,12 @@ typedef struct bsm_stage_info {
     uint32_t n_items;    /* work items (tiles) of the launch */
     uint32_t lds_bytes;
     uint32_t grid;       /* workgroups launched */
+    uint32_t stream;     /* 0: the area/batch stream; 1: the side stream (runs beside
+                            the spatial layers' stages, joined before the layer's
+                            pyramidal post stage) */
+    uint64_t alg_bytes;  /* gather+dendrites+tufts stages: algorithmic HBM bytes of one
+                            launch (5 B per synapse, 4 B per dendrite, 12 B per
+                            pyramidal cell-tuft, 1 B per SSC cell); 0 otherwise */
 } bsm_stage_info;
 int bsm_area_stage_plan(bsm_area* area, uint32_t max_stages, bsm_stage_info* out,
                         uint32_t* n_stages);

@@ -61,6 +61,8 @@ def test_static_batch_matches_plain(ctx, graph):
         ctx.upload(d, f)
         devs.append(d)
     b = Batch(batched, ctx)
+    if not graph:
+        b.set_tuning(overlap=2)   # one stream; the graphed run forks a side stream
     b.set_inputs(devs, [f.shape[1] for f in frames], [f.shape[1] for f in frames],
                  [s.input_range()[0] for s in schemes], n)
     b.cycle(n // 2, graph=graph)
@@ -100,7 +102,7 @@ def test_batch_profile_stages_advances_like_cycles(ctx):
                  [s.input_range()[0] for s in schemes], n)
     b.cycle(5, graph=True)
     prof = b.profile_stages(4)
-    assert [k for k, _ in prof] == [1, 2, 1, 3]
+    assert [k for k, _ in prof] == [1, 2, 1, 1, 3]
     assert all(t > 0 for _, t in prof)
     b.cycle(3, graph=True)
     b.sync()
@@ -155,9 +157,13 @@ def test_config3_plan_uses_fast_wide_variant(ctx):
     plan = b.stage_plan()
     print(plan, flush=True)
     f1 = [p for p in plan if p["kind"] == 1]
-    assert len(f1) == 2
+    assert [p["stream"] for p in f1] == [0, 1, 0]   # distal tufts beside the SSC stages
     for p in f1:
         assert p["chunk"] == 16 and p["fast"] == 1, p
+    assert sum(p["alg_bytes"] for p in f1) == sum(
+        sum(s.cells(li) * (t.syns_per_tft * 5 + t.dens_per_tft * 4 + (12 if lay.kind == 2 else 0))
+            for t in lay.tufts) + (s.cells(li) if lay.kind == 1 else 0)
+        for s in schemes for li, lay in enumerate(s.layers))
     b.close()
     for a in areas:
         a.close()

@@ -123,8 +123,11 @@ def test_cycle_config1_1000_cycles(ctx):
     learning on, 1000 cycles, every buffer compared after every cycle."""
     scheme = recipes.config1_area()
     g, o = make_pair(ctx, scheme, layout_seed=0)
-    # host-written layouts within 6 columns: both tuft launches check-free
-    assert [(p["kind"], p["fast"]) for p in g.stage_plan()] == [(1, 1), (2, 0), (1, 1), (3, 0)]
+    # host-written layouts within 6 columns: every tuft launch check-free; the
+    # distal tuft (own layer's slices) runs on the side stream
+    plan = g.stage_plan()
+    assert [(p["kind"], p["stream"]) for p in plan] == [(1, 0), (2, 0), (1, 1), (1, 0), (3, 0)]
+    assert all(p["fast"] == 1 for p in plan if p["kind"] == 1)
     frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=0).frames(1000)
     run_cycles(g, o, scheme, frames, 1000, tag="c1")
     # the run did learn and fire
@@ -135,7 +138,7 @@ def test_cycle_config1_1000_cycles(ctx):
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(tile_v=4, tile_u=4, halo=1, chunk=4),
+    dict(tile_v=4, tile_u=4, halo=1, chunk=4, overlap=2),
     dict(tile_v=16, tile_u=8, halo=8, chunk=8, inhib_tile=32),
     dict(tile_v=8, tile_u=16, halo=3, chunk=16),
     dict(tile_v=3, tile_u=5, halo=2, chunk=1, inhib_tile=8),
@@ -191,10 +194,15 @@ def test_cycle_config2_full_size(ctx):
     run_cycles(g, o, scheme, frames, 3, tag="c2")
     # the path measured at this size: both tuft launches check-free, 16 synapses per lane
     plan = g.stage_plan()
-    assert [p["kind"] for p in plan] == [1, 2, 1, 3]
+    assert [(p["kind"], p["stream"]) for p in plan] == [(1, 0), (2, 0), (1, 1), (1, 0), (3, 0)]
     for p in plan:
         if p["kind"] == 1:
             assert p["chunk"] == 16 and p["fast"] == 1, p
+    # algorithmic bytes of the pyramidal tuft launches: 5 B/synapse, 4 B/dendrite,
+    # 12 B/cell-tuft, one tuft each
+    cells = 2 * 256 * 256
+    assert [p["alg_bytes"] for p in plan if p["kind"] == 1][1:] == \
+        [cells * 512 * 5 + cells * 8 * 4 + cells * 12] * 2
     g.close()
     o.close()
 

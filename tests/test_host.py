@@ -41,7 +41,7 @@ def test_struct_layouts_match_header(tmp_path):
                    'sizeof(bsm_area_desc), offsetof(bsm_layer_desc, tufts),'
                    'offsetof(bsm_area_desc, layers), offsetof(bsm_tuft_desc, threshold),'
                    'sizeof(bsm_tuning), sizeof(bsm_stage_info),'
-                   'offsetof(bsm_stage_info, grid));return 0;}\n')
+                   'offsetof(bsm_stage_info, alg_bytes));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -50,7 +50,7 @@ def test_struct_layouts_match_header(tmp_path):
     want = [C.sizeof(L.SliceDesc), C.sizeof(L.TuftDesc), C.sizeof(L.LayerDesc),
             C.sizeof(L.AreaDesc), L.LayerDesc.tufts.offset, L.AreaDesc.layers.offset,
             L.TuftDesc.threshold.offset, C.sizeof(L.Tuning), C.sizeof(L.StageInfo),
-            L.StageInfo.grid.offset]
+            L.StageInfo.alg_bytes.offset]
     assert got == want
 
 
