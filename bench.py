@@ -311,6 +311,20 @@ def main():
         g[1] += p["alg_bytes"]
         g[2] += 1
     area.set_tuning(overlap=2 if args.serial else 1)
+    # the host-buffer boundary: each cycle's frame handed over from host memory
+    # (bsm_area_intake of a host frame, then bsm_area_cycle), PCIe included
+    n_host = min(200, args.steps)
+    area.sync()
+    t0 = time.perf_counter()
+    for c in range(n_host):
+        area.intake(frames[c % n_frames])
+        area.cycle(1)
+    area.sync()
+    host_s = (time.perf_counter() - t0) / n_host
+    pcie = {"synapse_updates_per_s": n_syn / host_s, "ms_per_cycle": 1000 * host_s,
+            "cycles": n_host,
+            "how": "host frame per cycle through bsm_area_intake (synchronous copy), then "
+                   "bsm_area_cycle with host-drawn rnd; Python host loop"}
     (dk, dl), (dt, alg_tot, n_launch) = max(groups.items(), key=lambda kv: kv[1][0])
     assert alg_tot == stage_bytes(scheme, dl, dk)   # the engine's count equals the recipe's
     avg_s = dt / n_prof / n_launch / 1000.0
@@ -346,6 +360,7 @@ def main():
         "roofline": roofline,
         "stages": stages,
         "timed_schedule": [{"kernel": kinds[k], "stream": st} for k, st in schedule],
+        "host_frames_pcie_inclusive": pcie,
     }
     if not args.no_batched:
         out["batched"] = batched_leg(ctx)
