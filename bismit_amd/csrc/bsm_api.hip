@@ -277,14 +277,20 @@ void layout_stats_update(bsm_area* a, int buf, uint32_t layer, size_t offset, si
     }
 }
 
-int side_init(Side& s) {
-    BSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-    BSM_HIP(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
-    return 0;
-}
-
-int side_events(Side& s, size_t n_stages) {
-    while (s.join.size() < n_stages) {
+// The side stream exists only while a plan has side stages: HIP spreads new
+// streams over the hardware queues in creation order, so streams created
+// eagerly beside every area stream would crowd the area streams onto fewer
+// queues when many areas run concurrently.
+int side_prepare(Side& s, const std::vector<Stage>& stages) {
+    bool any = false;
+    for (const Stage& st : stages) any |= st.stream == 1;
+    if (!any) {
+        side_destroy(s);
+        return 0;
+    }
+    if (!s.stream) BSM_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (!s.fork) BSM_HIP(hipEventCreateWithFlags(&s.fork, hipEventDisableTiming));
+    while (s.join.size() < stages.size()) {
         hipEvent_t e;
         BSM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         s.join.push_back(e);
@@ -293,6 +299,7 @@ int side_events(Side& s, size_t n_stages) {
 }
 
 void side_destroy(Side& s) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
     for (hipEvent_t e : s.join) (void)hipEventDestroy(e);
     s.join.clear();
     if (s.fork) (void)hipEventDestroy(s.fork);
@@ -468,7 +475,6 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->plan_dirty = true;
     a->batch_stream = nullptr;
     a->layout_epoch = 0;
-    if (side_init(a->side)) return 1;
     *out = a;
     return 0;
 }

@@ -5,11 +5,13 @@
 // Per batch cycle (stream order == reference enqueue order per area,
 // src/cortex/cortical_area.rs:716-789):
 //   prologue : per area, draw its rnd slots from its xorshift128 stream and copy
-//              its next input frame into its axon space (AxonSpace::intake)
+//              its next input frame into its axon space (AxonSpace::intake); the
+//              frame counter advances
 //   for each spatial layer rank k : F1 over all areas' SSC tufts, F2 over all SSC layers
 //   for each pyramidal layer rank k: F1 over all areas' pyramidal tufts, F3
-//   tick     : advance the device frame counter
-// Each area's layers keep their own state-buffer parity through a per-job xor.
+// Pyramidal tufts whose sources no earlier stage writes run their F1 on a side
+// stream from the prologue on, joined before their layer's F3.  Each area's
+// layers keep their own state-buffer parity through a per-job xor.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -41,9 +43,11 @@ constexpr uint32_t COPY_SEG = 16384;
 // xorshift128, Marsaglia 2003) and updates its dynamic words; every block
 // copies part `part` (COPY_SEG bytes) of the area's frame.
 __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io,
-                                                   const uint32_t* __restrict__ counter) {
+                                                   uint32_t* __restrict__ counter, uint32_t par) {
     const AreaIo A = io[blockIdx.x];
     const uint32_t part = blockIdx.y;
+    // frame counter: this cycle reads slot par, the next cycle (other parity) slot par ^ 1
+    if (blockIdx.x == 0 && part == 0 && threadIdx.x == 0) counter[(par & 1) ^ 1] = counter[par & 1] + 1;
     if (A.dyn) {
         // dynamic batch: only areas with a frame this cycle step; their layers
         // flip parity and are marked active, the others are marked idle
@@ -74,14 +78,13 @@ __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io
         const uint64_t fo = *A.frame_ofs;
         if (A.len && fo != ~0ull) src = A.stage + fo;
     } else if (A.len && A.n_frames) {
-        src = A.frames + (size_t)(*counter % A.n_frames) * A.stride;
+        src = A.frames + (size_t)(counter[par & 1] % A.n_frames) * A.stride;
     }
     const uint32_t lo = part * COPY_SEG;
     if (src && lo < A.len)
         block_copy(A.dst + lo, src + lo, min(COPY_SEG, A.len - lo), threadIdx.x, blockDim.x);
 }
 
-__global__ void kb_tick(uint32_t* counter) { *counter += 1; }
 
 struct TractRt {
     const uint8_t* src;
@@ -116,7 +119,7 @@ struct bsm_batch {
     bsm::WorkItem* items_d = nullptr;
     bsm::AreaIo* io_d = nullptr;
     uint32_t* rng_d = nullptr;      // 4 words per area
-    uint32_t* counter_d = nullptr;
+    uint32_t* counter_d = nullptr;  // frame counters, one per cycle parity
     uint32_t bpar = 0;
     bsm::Tuning tune;
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -266,7 +269,7 @@ static int batch_plan(bsm_batch* b) {
             b->stages.push_back(s2);
         }
     }
-    if (side_events(b->side, b->stages.size())) return 1;
+    if (side_prepare(b->side, b->stages)) return 1;
     (void)hipFree(b->tjobs_d);
     (void)hipFree(b->sjobs_d);
     (void)hipFree(b->pjobs_d);
@@ -293,7 +296,7 @@ static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
         kb_tracts<<<dim3(b->n_tracts, b->tract_parts), 256, 0, b->stream>>>(b->tracts_d);
         BSM_HIP(hipGetLastError());
     }
-    kb_prologue<<<dim3(na, b->io_parts), 256, 0, b->stream>>>(b->io_d, b->counter_d);
+    kb_prologue<<<dim3(na, b->io_parts), 256, 0, b->stream>>>(b->io_d, b->counter_d, par);
     BSM_HIP(hipGetLastError());
     if (ev) {
         // profiling: one stream, stages in list order, each launch timed alone
@@ -311,8 +314,6 @@ static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
                               })) {
         return 1;
     }
-    kb_tick<<<1, 1, 0, b->stream>>>(b->counter_d);
-    BSM_HIP(hipGetLastError());
     return 0;
 }
 
@@ -425,12 +426,11 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
     b->ctx = ctx;
     b->areas.assign(areas, areas + n_areas);
     BSM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    if (side_init(b->side)) return 1;
     BSM_HIP(hipEventCreate(&b->ev0));
     BSM_HIP(hipEventCreate(&b->ev1));
     BSM_HIP(hipMalloc((void**)&b->rng_d, sizeof(uint32_t) * 4 * n_areas));
-    BSM_HIP(hipMalloc((void**)&b->counter_d, sizeof(uint32_t)));
-    BSM_HIP(hipMemset(b->counter_d, 0, sizeof(uint32_t)));
+    BSM_HIP(hipMalloc((void**)&b->counter_d, 2 * sizeof(uint32_t)));
+    BSM_HIP(hipMemset(b->counter_d, 0, 2 * sizeof(uint32_t)));
     std::vector<uint32_t> rng(4 * n_areas);
     b->io.resize(n_areas);
     for (uint32_t i = 0; i < n_areas; i++) {
@@ -499,7 +499,7 @@ int bsm_batch_destroy(bsm_batch* b) {
     bsm_batch_sync(b);
     (void)hipSetDevice(b->ctx->device);
     free_graphs(b);
-    if (side_events(b->side, b->stages.size())) return 1;
+    if (side_prepare(b->side, b->stages)) return 1;
     (void)hipFree(b->tjobs_d);
     (void)hipFree(b->sjobs_d);
     (void)hipFree(b->pjobs_d);
@@ -549,7 +549,7 @@ int bsm_batch_set_inputs(bsm_batch* b, const uint8_t* const* dev_frames, const u
         io.n_frames = n_frames;
     }
     BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * b->io.size(), hipMemcpyHostToDevice));
-    BSM_HIP(hipMemset(b->counter_d, 0, sizeof(uint32_t)));
+    BSM_HIP(hipMemset(b->counter_d, 0, 2 * sizeof(uint32_t)));
     return 0;
 }
 

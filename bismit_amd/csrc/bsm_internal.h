@@ -171,8 +171,7 @@ struct Side {
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;  // one per stage index (side stages record theirs)
 };
-int side_init(Side& s);
-int side_events(Side& s, size_t n_stages);
+int side_prepare(Side& s, const std::vector<Stage>& stages);
 void side_destroy(Side& s);
 
 // Enqueues one cycle's stages.  Side stages fork from `main` at this point and
