@@ -1048,16 +1048,6 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
                 items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)a, (uint16_t)b});
     }
     st.n_items = (uint32_t)items.size() - st.item0;
-    // largest tiles first (synapse bytes plus a fixed per-tile cost), so that the
-    // launch does not end on a few large tiles of one area
-    auto cost = [&](const WorkItem& w) {
-        const TuftJob& j = jobs[w.job];
-        const uint64_t tvn = std::min<uint64_t>(TV, j.V - (uint64_t)w.tv * TV);
-        const uint64_t tun = std::min<uint64_t>(TU, j.U - (uint64_t)w.tu * TU);
-        return j.depth * tvn * tun * j.spt + 4096;
-    };
-    std::stable_sort(items.begin() + st.item0, items.end(),
-                     [&](const WorkItem& x, const WorkItem& y) { return cost(x) > cost(y); });
     st.lds_bytes = lds;
     st.W = W;
     st.nst = nst <= 1 ? 1 : (nst <= 2 ? 2 : 4);
