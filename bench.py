@@ -6,8 +6,7 @@ SDR input with 50% frame-to-frame overlap, learning on.  Input frames are
 resident in HBM before the timed span; each timed cycle copies its frame into
 the axon space and runs every fused stage.
 
-Prints one JSON line (see the driver contract).  --gpus N > 1 exits: this
-engine runs on one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this engine runs on one GPU.
 """
 import argparse
 import json
@@ -20,11 +19,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
+# Algorithmic bytes of one synapse-update: the compulsory traffic of a fused cycle
+# that materialises every synapse buffer the reference keeps.
+ALG_BYTES_PER_SYNAPSE = 8
+ALG_BASIS = ("8 B per synapse-update: 3 B source metadata + 1 B strength + 1 B previous "
+             "state + 1 B new state + 2 B flags read and written (dendrite/tuft bytes excluded)")
+ENGINE_BASIS = ("bytes this engine must move: 5 B per synapse (metadata, strength, new state; "
+                "the previous-active flag is derived from the previous state buffer) + 4 B per "
+                "dendrite + 12 B per pyramidal cell-tuft")
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
 def stage_bytes(scheme, li, kind):
-    """Algorithmic HBM bytes of one fused stage launch on layer li:
+    """HBM bytes this engine moves in one fused stage launch on layer li:
     F1: per synapse 3 B metadata + 1 B strength read, 1 B state write; per
         dendrite raw/state writes + activity read/write (4 B); per cell-tuft 4 B
         read + 8 B written (pyramidal), per SSC cell 1 B energy read.
@@ -174,10 +181,11 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True, serial=False):
               "chunk": p["chunk"], "fast": p["fast"], "tile": [p["tile_v"], p["tile_u"]],
               "items": p["n_items"]}
         if k == 1:
-            st["alg_bytes_per_launch"] = p["alg_bytes"]
-            st["hbm_frac"] = round(p["alg_bytes"] / (t / n_prof / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+            st["engine_bytes_per_launch"] = p["alg_bytes"]
+            st["hbm_frac_engine_bytes"] = round(p["alg_bytes"] / (t / n_prof / 1000.0) / 1e9 /
+                                                HBM_PEAK_GBS, 4)
         stages.append(st)
-    cyc_alg = sum(st.get("alg_bytes_per_launch", 0) for st in stages)
+    cyc_alg = sum(st.get("engine_bytes_per_launch", 0) for st in stages)
     b.close()
     for a in areas:
         a.close()
@@ -196,8 +204,8 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True, serial=False):
             "cortex_loop_ms_per_cycle_by_mode": {"side_stream": 1000 * loop_t[1],
                                                  "one_stream": 1000 * loop_t[2]},
             "tuning": best, "stages": stages,
-            "hbm_frac_cycle": cyc_alg * n_cycles / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS,
-            "hbm_frac_at_5B_per_synapse": batched * 5 / (HBM_PEAK_GBS * 1e9)}
+            "hbm_frac": batched * ALG_BYTES_PER_SYNAPSE / (HBM_PEAK_GBS * 1e9),
+            "hbm_frac_engine_bytes": cyc_alg * n_cycles / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS}
 
 
 def serving_leg(ctx, wall_s=20.0, rate=200.0, budget_gb=96, max_sessions=64):
@@ -302,7 +310,7 @@ def main():
     stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name, "stream": p["stream"],
                "avg_us": round(1000.0 * t / n_prof, 2), "chunk": p["chunk"], "fast": p["fast"],
                "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"],
-               "alg_bytes_per_launch": p["alg_bytes"]}
+               "engine_bytes_per_launch": p["alg_bytes"]}
               for (k, li, t), p in zip(prof, plan)]
     groups = {}
     for (k, li, t), p in zip(prof, plan):
@@ -328,16 +336,27 @@ def main():
     (dk, dl), (dt, alg_tot, n_launch) = max(groups.items(), key=lambda kv: kv[1][0])
     assert alg_tot == stage_bytes(scheme, dl, dk)   # the engine's count equals the recipe's
     avg_s = dt / n_prof / n_launch / 1000.0
-    alg = alg_tot / n_launch
+    # algorithmic bytes: ALG_BYTES_PER_SYNAPSE per synapse-update of the launch
+    syn_launch = scheme.cells(dl) * sum(t.syns_per_tft for t in scheme.layers[dl].tufts) / n_launch
+    alg = ALG_BYTES_PER_SYNAPSE * syn_launch
     achieved = alg / avg_s / 1e9
+    # the bytes this engine has to move for the same launch (5 B per synapse,
+    # 4 B per dendrite, 12 B per pyramidal cell-tuft)
+    moved = alg_tot / n_launch
+    moved_gbs = moved / avg_s / 1e9
     traffic = read_traffic()
     ceil = stream_ceiling(ctx)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
-                "launches_per_cycle": n_launch,
+                "launches_per_cycle": n_launch, "synapse_updates_per_launch": syn_launch,
                 "alg_bytes_per_launch": alg, "avg_launch_us": round(avg_s * 1e6, 2),
-                "stream_ceiling": ceil, "frac_of_stream_ceiling": round(achieved / ceil["gbs"], 4)}
+                "alg_basis": ALG_BASIS,
+                "engine_bytes": {"per_launch": moved, "achieved": round(moved_gbs, 1),
+                                 "frac": round(moved_gbs / HBM_PEAK_GBS, 4),
+                                 "frac_of_stream_ceiling": round(moved_gbs / ceil["gbs"], 4),
+                                 "basis": ENGINE_BASIS},
+                "stream_ceiling": ceil}
 
     out = {
         "metric": "synapse-updates/s (cycle+LTP), batched vs plain on 1 GPU; p50/p99 frame latency",
