@@ -227,7 +227,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000, help="timed cycles per run")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--runs", type=int, default=5, help="timed runs (median reported)")
-    ap.add_argument("--cpu-cycles", type=int, default=40)
+    ap.add_argument("--cpu-cycles", type=int, default=250,
+                    help="CPU baseline sample (about 12 s of oracle work at 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
