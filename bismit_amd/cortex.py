@@ -186,6 +186,11 @@ class CorticalArea:
         L.check(self.lib.bsm_area_profile_stages(self.h, n_cycles, m, ms, kind, lay, C.byref(n)))
         return [(kind[i], lay[i], ms[i]) for i in range(n.value)]
 
+    def regrow(self, seed, epoch, reach=6):
+        """Deterministic regrowth of every synapse with strength <= -25
+        (CorticalArea::regrow, src/cortex/cortical_area.rs:792-806)."""
+        L.check(self.lib.bsm_area_regrow(self.h, seed, epoch, reach))
+
     def stage_plan(self):
         """The fused launches of one cycle: kernel variant and tiling of each."""
         arr = (L.StageInfo * 64)()

@@ -228,6 +228,100 @@ int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, u
     return 0;
 }
 
+// ---- deterministic regrowth (CorticalArea::regrow, src/cortex/cortical_area.rs:792-806;
+// Synapses::regrow_syn, src/cortex/synapses.rs:411-501; SynSrcSlices::gen_src,
+// src/map/syn_src.rs:448-490) with counter-based draws instead of an entropy generator.
+
+struct RegrowSrc {
+    uint8_t ids[BSM_MAX_TUFT_SRCS];
+    uint16_t v_size[BSM_MAX_TUFT_SRCS], u_size[BSM_MAX_TUFT_SRCS];
+    uint8_t v_scale[BSM_MAX_TUFT_SRCS], u_scale[BSM_MAX_TUFT_SRCS];
+    uint32_t tile[BSM_MAX_TUFT_SRCS];  // hex tile size (0: non-spatial source)
+};
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Row r of the hex tile of radius rv (gen_syn_offs, src/map/syn_src.rs:31-107): the
+// u range [max(-rv, -v - rv), min(rv, -v + rv)] rescaled by u_scale / v_scale.
+__host__ __device__ inline void hex_row(int32_t v, int32_t rv, int32_t vsc, int32_t usc,
+                                        int32_t* uz, int32_t* un) {
+    const int32_t lo = v < 0 ? -v - rv : -rv, hi = v > 0 ? rv - v : rv;
+    *uz = lo * usc / vsc;
+    *un = hi * usc / vsc;
+}
+
+__host__ __device__ inline uint32_t hex_tile_size(int32_t rv, int32_t vsc, int32_t usc) {
+    uint32_t n = 0;
+    for (int32_t v = -rv; v <= rv; v++) {
+        int32_t uz, un;
+        hex_row(v, rv, vsc, usc, &uz, &un);
+        n += (uint32_t)(un - uz + 1);
+    }
+    return n;
+}
+
+// One thread per dendrite of one tuft; synapses in order; a draw is kept when no
+// other synapse of the dendrite has its (slice, v, u), the 32nd draw always.
+__global__ __launch_bounds__(256) void k_regrow(uint64_t key, uint64_t n_den, uint32_t spd,
+                                                uint32_t n_src, RegrowSrc src, int32_t reach,
+                                                uint8_t* __restrict__ slc, int8_t* __restrict__ vo,
+                                                int8_t* __restrict__ uo, int8_t* __restrict__ str) {
+    const uint64_t d = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n_den) return;
+    const uint64_t base = d * spd;
+    for (uint32_t k = 0; k < spd; k++) {
+        if (str[base + k] > -25) continue;
+        const uint64_t ti = base + k;
+        for (uint32_t att = 0; att < 32; att++) {
+            const uint64_t h = mix64(key + (ti * 32 + att + 1) * 0x9E3779B97F4A7C15ull);
+            const uint32_t slot = (uint32_t)(h & 0xFFFF) % n_src;
+            int32_t v, u, s;
+            if (src.tile[slot] == 0) {
+                const int32_t rv = src.v_size[slot] / 2, ru = src.u_size[slot] / 2;
+                v = (int32_t)(((h >> 16) & 0xFFFF) % (uint64_t)(2 * rv + 1)) - rv;
+                u = (int32_t)(((h >> 32) & 0xFFFF) % (uint64_t)(2 * ru + 1)) - ru;
+                s = 0;
+            } else {
+                const int32_t vsc = src.v_scale[slot], usc = src.u_scale[slot];
+                const int32_t rv = (reach * vsc) >> 4, ru = (reach * usc) >> 4;
+                uint32_t q = (uint32_t)(((h >> 16) & 0xFFFFFF) % src.tile[slot]);
+                v = -rv;
+                u = 0;
+                for (int32_t row = -rv; row <= rv; row++) {
+                    int32_t uz, un;
+                    hex_row(row, rv, vsc, usc, &uz, &un);
+                    const uint32_t w = (uint32_t)(un - uz + 1);
+                    if (q < w) {
+                        v = row;
+                        u = uz + (int32_t)q;
+                        break;
+                    }
+                    q -= w;
+                }
+                const int32_t inten = ((rv - (v < 0 ? -v : v)) + (ru - (u < 0 ? -u : u))) >> 3;
+                s = inten * ((int32_t)(((h >> 40) & 0xFFFF) % 7) - 3);
+            }
+            const uint8_t sl = src.ids[slot];
+            bool unique = true;
+            for (uint32_t j = 0; j < spd && unique; j++)
+                if (j != k && slc[base + j] == sl && vo[base + j] == (int8_t)v &&
+                    uo[base + j] == (int8_t)u)
+                    unique = false;
+            if (unique || att == 31) {
+                slc[base + k] = sl;
+                vo[base + k] = (int8_t)v;
+                uo[base + k] = (int8_t)u;
+                str[base + k] = (int8_t)s;
+                break;
+            }
+        }
+    }
+}
+
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids) {
     TuftRt& T = a->L[l].t[t];
@@ -702,6 +796,60 @@ int bsm_area_draw_rnd(bsm_area* a, int32_t* out, uint32_t n_cycles) {
     if (!a || (!out && n_cycles)) return fail("null arg");
     uint64_t n = (uint64_t)n_cycles * a->n_slots;
     for (uint64_t i = 0; i < n; i++) out[i] = (int32_t)rng_next(a->rng);
+    return 0;
+}
+
+int bsm_area_regrow(bsm_area* a, uint64_t seed, uint64_t epoch, int32_t reach) {
+    if (!a) return fail("null area");
+    if (a->batch_stream) return fail("area belongs to a batch");
+    if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
+    BSM_HIP(hipSetDevice(a->ctx->device));
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+            TuftRt& T = L.t[t];
+            const uint32_t n_src = std::min<uint32_t>(T.d.n_src_slcs, BSM_MAX_TUFT_SRCS);
+            if (!n_src) continue;
+            RegrowSrc src{};
+            uint32_t new_reach = 0;
+            for (uint32_t k = 0; k < n_src; k++) {
+                const SliceRt& sl = a->slices_h[T.d.src_slc_ids[k]];
+                src.ids[k] = T.d.src_slc_ids[k];
+                src.v_size[k] = sl.v_size;
+                src.u_size[k] = sl.u_size;
+                src.v_scale[k] = sl.v_scale;
+                src.u_scale[k] = sl.u_scale;
+                const bool spatial = sl.v_scale && sl.u_scale;
+                const int32_t rv = (reach * sl.v_scale) >> 4;
+                src.tile[k] = spatial ? hex_tile_size(rv, sl.v_scale, sl.u_scale) : 0u;
+                // largest |offset| a regrown synapse of this source can take
+                int32_t mo = spatial ? std::max(rv, (rv * sl.u_scale + sl.v_scale - 1) / sl.v_scale)
+                                     : std::max<int32_t>(sl.v_size / 2, sl.u_size / 2);
+                new_reach = std::max(new_reach, (uint32_t)mo);
+            }
+            const uint64_t key = mix64(seed ^ (epoch * 0xD1B54A32D192ED03ull)) ^
+                                 ((uint64_t)(l * 8 + t) << 56);
+            const uint64_t n_den = (uint64_t)L.cells * T.d.dens_per_tft;
+            const uint32_t grid = (uint32_t)((n_den + 255) / 256);
+            if (n_den) {
+                k_regrow<<<grid, 256, 0, a->stream>>>(key, n_den, T.d.syns_per_den, n_src, src,
+                                                      reach, L.syn_slc + T.syn_idz,
+                                                      (int8_t*)L.syn_v + T.syn_idz,
+                                                      (int8_t*)L.syn_u + T.syn_idz,
+                                                      (int8_t*)L.syn_str + T.syn_idz);
+                BSM_HIP(hipGetLastError());
+            }
+            // the source summary widens to the regrown sources
+            T.reach_v = std::max(T.reach_v, new_reach);
+            T.reach_u = std::max(T.reach_u, new_reach);
+            T.reach_max = std::max(T.reach_v, T.reach_u);
+            for (uint32_t k = 0; k < n_src; k++)
+                T.slc_bits[src.ids[k] >> 6] |= 1ull << (src.ids[k] & 63);
+        }
+    }
+    a->plan_dirty = true;
+    a->layout_epoch++;
+    BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;
 }
 

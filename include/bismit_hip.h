@@ -302,6 +302,20 @@ int bsm_gen_tuft_layout(uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n
 /* Fill every tuft of every layer of the area with bsm_gen_tuft_layout and
  * zero strengths (runs the generator on the host, then uploads). */
 int bsm_area_init_layout(bsm_area* area, uint64_t seed, int32_t reach);
+/* Deterministic regrowth, replacing CorticalArea::regrow (src/cortex/cortical_area.rs:
+ * 792-806) -> Synapses::grow / regrow_syn (src/cortex/synapses.rs:411-501) and
+ * SynSrcSlices::gen_src (src/map/syn_src.rs:448-490): every synapse with strength
+ * <= -25 (SYNAPSE_STRENGTH_FLOOR) takes a new source and strength.  The reference draws
+ * them from an entropy-seeded generator on the host; here they come from a
+ * counter-based hash of (seed, epoch, layer, tuft, synapse, attempt), drawn on the
+ * device: a uniform source slice of the tuft, a uniform offset in the hex tile of
+ * radius (reach * scale) >> 4 (non-spatial slices: half the slice size), strength
+ * ((rv - |v|) + (ru - |u|)) >> 3 times a uniform draw in -3..3 (0 when non-spatial),
+ * redrawn up to 31 times while another synapse of the dendrite has the same source.
+ * The reference calls it every SYNAPSE_REGROWTH_INTERVAL + 1 = 201 cycles unless
+ * regrowth is disabled; the caller decides when (epoch numbers the calls).
+ * Completes on return. */
+int bsm_area_regrow(bsm_area* area, uint64_t seed, uint64_t epoch, int32_t reach);
 
 /* ---- deterministic rnd schedule helpers ---- */
 uint64_t bsm_splitmix64(uint64_t* state);

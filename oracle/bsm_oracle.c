@@ -715,3 +715,99 @@ int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles) {
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ regrowth */
+
+static uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Hex tile of gen_syn_offs (src/map/syn_src.rs:31-107) for a spatial slice:
+ * position q (row-major) -> (v, u); returns the tile size when q is past it. */
+static uint32_t hex_tile_at(int32_t rv, int32_t vsc, int32_t usc, uint32_t q, int32_t* v,
+                            int32_t* u) {
+    uint32_t n = 0;
+    for (int32_t vo = -rv; vo <= rv; vo++) {
+        const int32_t lo = (vo < 0 ? -vo - rv : -rv);   /* max(-rv, -vo - rv) */
+        const int32_t hi = (vo > 0 ? rv - vo : rv);     /* min(rv, -vo + rv) */
+        const int32_t uz = lo * usc / vsc, un = hi * usc / vsc;
+        const uint32_t w = (uint32_t)(un - uz + 1);
+        if (q < n + w) {
+            *v = vo;
+            *u = uz + (int32_t)(q - n);
+            return 0;
+        }
+        n += w;
+    }
+    return n;
+}
+
+int orc_area_regrow(orc_area* a, uint64_t seed, uint64_t epoch, int32_t reach) {
+    if (reach < 0 || reach > 127) return 1;
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        orc_layer* L = &a->L[l];
+        for (uint32_t t = 0; t < L->d.n_tufts; t++) {
+            const orc_tuft* T = &L->t[t];
+            const uint32_t n_src = T->d.n_src_slcs, spd = T->d.syns_per_den;
+            if (n_src == 0) continue;
+            const uint64_t key = mix64(seed ^ (epoch * 0xD1B54A32D192ED03ull)) ^
+                                 ((uint64_t)(l * 8 + t) << 56);
+            /* tile sizes per source */
+            uint32_t tile[8];
+            for (uint32_t k = 0; k < n_src; k++) {
+                const bsm_slice_desc* s = &a->slc[T->d.src_slc_ids[k]];
+                if (s->v_scale == 0 || s->u_scale == 0) {
+                    tile[k] = 0;
+                } else {
+                    int32_t dv, du;
+                    tile[k] = hex_tile_at((reach * s->v_scale) >> 4, s->v_scale, s->u_scale,
+                                          0xFFFFFFFFu, &dv, &du);
+                }
+            }
+            const uint64_t n_den = (uint64_t)L->cells * T->d.dens_per_tft;
+            for (uint64_t d = 0; d < n_den; d++) {
+                const uint64_t base = T->syn_idz + d * spd;
+                for (uint32_t k = 0; k < spd; k++) {
+                    const uint64_t i = base + k;
+                    if (L->syn_str[i] > -25) continue;
+                    const uint64_t ti = d * spd + k;   /* tuft-relative synapse index */
+                    for (uint32_t att = 0; att < 32; att++) {
+                        const uint64_t h = mix64(key + (ti * 32 + att + 1) * 0x9E3779B97F4A7C15ull);
+                        const uint32_t slot = (uint32_t)(h & 0xFFFF) % n_src;
+                        const uint8_t slc = T->d.src_slc_ids[slot];
+                        const bsm_slice_desc* s = &a->slc[slc];
+                        int32_t v, u, str;
+                        if (tile[slot] == 0) {
+                            const int32_t rv = (int32_t)(s->v_size / 2), ru = (int32_t)(s->u_size / 2);
+                            v = (int32_t)(((h >> 16) & 0xFFFF) % (uint64_t)(2 * rv + 1)) - rv;
+                            u = (int32_t)(((h >> 32) & 0xFFFF) % (uint64_t)(2 * ru + 1)) - ru;
+                            str = 0;
+                        } else {
+                            const int32_t rv = (reach * s->v_scale) >> 4, ru = (reach * s->u_scale) >> 4;
+                            hex_tile_at(rv, s->v_scale, s->u_scale,
+                                        (uint32_t)(((h >> 16) & 0xFFFFFF) % tile[slot]), &v, &u);
+                            const int32_t inten = ((rv - abs(v)) + (ru - abs(u))) >> 3;
+                            str = inten * ((int32_t)(((h >> 40) & 0xFFFF) % 7) - 3);
+                        }
+                        int unique = 1;
+                        for (uint32_t j = 0; j < spd && unique; j++) {
+                            if (j == k) continue;
+                            unique = !(L->syn_slc[base + j] == slc && L->syn_v[base + j] == (int8_t)v &&
+                                       L->syn_u[base + j] == (int8_t)u);
+                        }
+                        if (unique || att == 31) {
+                            L->syn_slc[i] = slc;
+                            L->syn_v[i] = (int8_t)v;
+                            L->syn_u[i] = (int8_t)u;
+                            L->syn_str[i] = (int8_t)str;
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}

@@ -56,6 +56,28 @@ int orc_k_tft_dst_mtp(orc_area* a, uint32_t layer, uint32_t tuft, int32_t rnd);
 /* whole cycle, reference enqueue order; rnd = n_cycles * n_slots draws */
 int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles);
 
+/* Deterministic regrowth (CorticalArea::regrow, src/cortex/cortical_area.rs:792-806;
+ * Synapses::grow / regrow_syn, src/cortex/synapses.rs:411-501; SynSrcSlices::gen_src,
+ * src/map/syn_src.rs:448-490).  Every synapse with strength <= -25
+ * (SYNAPSE_STRENGTH_FLOOR) gets a new source drawn from a counter-based hash instead of
+ * an entropy-seeded generator:
+ *   key = mix64(seed ^ epoch * 0xD1B54A32D192ED03) ^ ((layer * 8 + tuft) << 56)
+ *   draw a of tuft-relative synapse i: h = mix64(key + (i * 32 + a + 1) * 0x9E3779B97F4A7C15)
+ *   slice = src_slc_ids[(h & 0xFFFF) % n_src]
+ *   spatial slice: rv = (reach * v_scale) >> 4, ru = (reach * u_scale) >> 4; offsets
+ *     enumerate the hex tile of gen_syn_offs (src/map/syn_src.rs:31-107): rows v = -rv..rv,
+ *     u = trunc(max(-rv, -v - rv) * u_scale / v_scale) .. trunc(min(rv, -v + rv) * u_scale /
+ *     v_scale); position ((h >> 16) & 0xFFFFFF) % tile size in row-major order;
+ *     strength = (((rv - |v|) + (ru - |u|)) >> 3) * ((((h >> 40) & 0xFFFF) % 7) - 3)
+ *   non-spatial slice (a scale of 0): v = ((h >> 16) & 0xFFFF) % (2 rv + 1) - rv with
+ *     rv = v_size / 2, u likewise with u_size, strength 0
+ *   the draw is kept if no other synapse of the dendrite has the same (slice, v, u)
+ *   (SynSrcIdxCache::insert), else the next draw is tried; the 32nd draw is always kept
+ *   (the reference's "source saturated" case).
+ * Dendrites are independent; within a dendrite synapses are visited in order.
+ * mix64 is the splitmix64 output function (Steele, Lea, Flood 2014). */
+int orc_area_regrow(orc_area* a, uint64_t seed, uint64_t epoch, int32_t reach);
+
 #ifdef __cplusplus
 }
 #endif

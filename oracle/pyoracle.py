@@ -65,6 +65,7 @@ def load():
             getattr(lib, "orc_k_" + n).argtypes = [P, C.c_uint32, C.c_int32]
         lib.orc_k_ssc_mtp.argtypes = [P, C.c_uint32, C.c_uint32]
         lib.orc_area_cycle.argtypes = [P, C.POINTER(C.c_int32), C.c_uint32]
+        lib.orc_area_regrow.argtypes = [P, C.c_uint64, C.c_uint64, C.c_int32]
         _lib = lib
     return _lib
 
@@ -122,6 +123,11 @@ class OracleArea:
         rc = self.lib.orc_area_cycle(self.h, rnd.ctypes.data_as(C.POINTER(C.c_int32)), n)
         if rc:
             raise RuntimeError("orc_area_cycle failed")
+
+    def regrow(self, seed, epoch, reach=6):
+        """Deterministic regrowth of every synapse with strength <= -25."""
+        if self.lib.orc_area_regrow(self.h, seed, epoch, reach):
+            raise RuntimeError("orc_area_regrow failed")
 
     def k(self, name, *args):
         rc = getattr(self.lib, "orc_k_" + name)(self.h, *args)

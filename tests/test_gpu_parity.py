@@ -246,3 +246,60 @@ def test_cycle_generic_gather_paths(ctx):
     frames = inputs.SdrStream(48 * 48, 0.04, 0.5, seed=5).frames(20)
     run_cycles(g, o, scheme, frames, 20, tag="generic")
     g.close()
+
+
+@pytest.mark.parametrize("which", ["small", "c1", "apical", "nonspatial"])
+def test_regrow_matches_oracle(ctx, which):
+    """bsm_area_regrow against the oracle's restatement on random state: every
+    synapse with strength <= -25 gets the same new source and strength."""
+    if which == "nonspatial":
+        scheme = recipes.cortical_area("n", 24, pyr_depth=2, ssc_dens=3, ssc_spd=8, pyr_dens=3,
+                                       pyr_spd=8)
+        scheme.slices[0] = recipes.SliceScheme.nonspatial(230, 230)
+    else:
+        scheme = SCHEMES[which]()
+    print("case", which, flush=True)
+    g, o = make_pair(ctx, scheme)
+    rng = np.random.default_rng(17)
+    randomize(g, o, scheme, rng)
+    before = g.read("SYN_SRC_COL_V_OFFS", 0).copy()
+    for epoch in range(3):
+        g.regrow(seed=12345, epoch=epoch, reach=6)
+        o.regrow(seed=12345, epoch=epoch, reach=6)
+        compare(g, o, scheme, f"{which} regrow epoch {epoch}")
+    assert not np.array_equal(before, g.read("SYN_SRC_COL_V_OFFS", 0))
+    # the regrown area still cycles identically (the planner re-derives the gather path)
+    frames = inputs.SdrStream(scheme.input_range()[1], 0.05, 0.5, seed=3).frames(5)
+    run_cycles(g, o, scheme, frames, 5, tag=f"{which} after regrow")
+    g.close()
+
+
+def test_cycle_config1_with_regrowth(ctx):
+    """Config 1 with regrowth at the end of cycles 201, 402 and 603, as the
+    reference's counter schedules it (src/cortex/cortical_area.rs:792-806);
+    strengths start in [-40, 10] so that regrowth has work; every buffer
+    compared after every cycle."""
+    scheme = recipes.config1_area()
+    g, o = make_pair(ctx, scheme, layout_seed=4)
+    rng = np.random.default_rng(23)
+    for li in range(len(scheme.layers)):
+        st = rng.integers(-40, 11, g.buf_len("SYN_STRENGTHS", li)).astype(np.int8)
+        g.write("SYN_STRENGTHS", li, st)
+        o.write("SYN_STRENGTHS", li, st)
+    frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=2).frames(603)
+    rnd = g.draw_rnd(603)
+    changed = 0
+    for c in range(603):
+        g.intake(frames[c])
+        o.intake(frames[c])
+        r = rnd[c * g.n_slots:(c + 1) * g.n_slots]
+        g.cycle(1, r)
+        o.cycle(r, 1)
+        if (c + 1) % 201 == 0:
+            pre = g.read("SYN_SRC_COL_V_OFFS", 0)
+            g.regrow(seed=77, epoch=(c + 1) // 201 - 1, reach=6)
+            o.regrow(seed=77, epoch=(c + 1) // 201 - 1, reach=6)
+            changed += int(np.count_nonzero(pre != g.read("SYN_SRC_COL_V_OFFS", 0)))
+        compare(g, o, scheme, f"c1 regrow cycle {c}")
+    assert changed > 0
+    g.close()

@@ -428,3 +428,113 @@ def test_ssc_cycle_energy_rules():
     o.k("ssc_cycle", 0)
     for c, (_, _, en2, st2) in enumerate(cases):
         assert (e[c], s[c]) == (en2, st2), c
+
+
+# ------------------------------------------------------------------ regrowth
+
+def _regrow_area(scheme, seed=1, weak_frac=None):
+    """Oracle area whose synapses all start on (slice 0, 0, 0) with random
+    strengths (all <= -25 when weak_frac is None)."""
+    o = P.OracleArea(scheme)
+    rng = np.random.default_rng(seed)
+    for li in range(len(scheme.layers)):
+        n = o.buf("SYN_STRENGTHS", li).size
+        if weak_frac is None:
+            st = np.full(n, -100, dtype=np.int8)
+        else:
+            st = rng.integers(-127, 128, n).astype(np.int8)
+        o.write("SYN_STRENGTHS", li, st)
+        # every synapse starts on a valid source of its tuft
+        lay = scheme.layers[li]
+        cells = scheme.cells(li)
+        parts = [np.full(cells * t.syns_per_tft, t.src_slcs[0], dtype=np.uint8) for t in lay.tufts]
+        o.write("SYN_SRC_SLC_IDS", li, np.concatenate(parts))
+    return o
+
+
+def _tuft_view(o, scheme, li, t):
+    """(slc, v, u, strength) of tuft t of layer li, shaped (dendrites, spd)."""
+    lay = scheme.layers[li]
+    cells = scheme.cells(li)
+    z = sum(cells * x.syns_per_tft for x in lay.tufts[:t])
+    n = cells * lay.tufts[t].syns_per_tft
+    spd = lay.tufts[t].syns_per_den
+    out = []
+    for name in ("SYN_SRC_SLC_IDS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS", "SYN_STRENGTHS"):
+        out.append(o.read(name, li)[z:z + n].astype(np.int32).reshape(-1, spd))
+    return out
+
+
+def test_regrow_spatial_tile_is_the_hex_ball_and_unique_per_dendrite():
+    r = 6
+    s = recipes.cortical_area("rg", 16, input_side=16, ssc_dens=4, ssc_spd=32, pyr_dens=4,
+                              pyr_spd=32)
+    o = _regrow_area(s)
+    o.regrow(seed=7, epoch=0, reach=r)
+    slc, v, u, st = _tuft_view(o, s, 0, 0)           # SSC tuft: source slice 0, scale 16
+    assert np.all(slc == 0)
+    ball = {(a, b) for a in range(-r, r + 1) for b in range(-r, r + 1)
+            if max(abs(a), abs(b), abs(a + b)) <= r}
+    assert len(ball) == 3 * r * (r + 1) + 1
+    seen = set(zip(v.ravel().tolist(), u.ravel().tolist()))
+    assert seen == ball                               # inside the tile, and all of it drawn
+    # 32 synapses per dendrite over a 127-position tile: sources unique per dendrite
+    keys = slc * 65536 + (v + 128) * 256 + (u + 128)
+    for row in keys:
+        assert np.unique(row).size == row.size
+    # strength = ((r - |v|) + (r - |u|)) >> 3 times a draw from -3..3
+    inten = ((r - np.abs(v)) + (r - np.abs(u))) >> 3
+    assert np.all(st[inten == 0] == 0)
+    k = st[inten > 0] // inten[inten > 0]
+    assert np.all(st[inten > 0] % inten[inten > 0] == 0)
+    assert set(np.unique(k).tolist()) == set(range(-3, 4))
+    o.close()
+
+
+def test_regrow_scaled_and_nonspatial_sources():
+    # input 24x24 on a 64x64 area: scale 6, reach 6 -> scaled reach (6 * 6) >> 4 = 2
+    s = recipes.cortical_area("rs", 64, input_side=24, ssc_dens=2, ssc_spd=8, pyr_dens=2,
+                              pyr_spd=8)
+    o = _regrow_area(s)
+    o.regrow(seed=3, epoch=5, reach=6)
+    slc, v, u, st = _tuft_view(o, s, 0, 0)
+    ball2 = {(a, b) for a in range(-2, 3) for b in range(-2, 3) if max(abs(a), abs(b), abs(a + b)) <= 2}
+    assert set(zip(v.ravel().tolist(), u.ravel().tolist())) == ball2
+    assert np.all(st == 0)                            # ((2 - |v|) + (2 - |u|)) >> 3 == 0
+    o.close()
+    # a non-spatial source slice (scale 0, size 230): v, u in [-115, 115], strength 0
+    s = recipes.cortical_area("rn", 24, ssc_dens=3, ssc_spd=8, pyr_dens=3, pyr_spd=8)
+    s.slices[0] = recipes.SliceScheme.nonspatial(230, 230)
+    o = _regrow_area(s)
+    o.regrow(seed=3, epoch=5, reach=6)
+    slc, v, u, st = _tuft_view(o, s, 0, 0)
+    assert v.min() >= -115 and v.max() <= 115 and u.min() >= -115 and u.max() <= 115
+    assert v.min() < -100 and v.max() > 100           # the whole range is used
+    assert np.all(st == 0)
+    o.close()
+
+
+def test_regrow_touches_only_weak_synapses_and_is_deterministic():
+    s = recipes.cortical_area("rw", 16, input_side=16, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                              pyr_spd=16)
+    a = _regrow_area(s, seed=11, weak_frac=True)
+    b = _regrow_area(s, seed=11, weak_frac=True)
+    c = _regrow_area(s, seed=11, weak_frac=True)
+    before = {(li, n): a.read(n, li) for li in range(2)
+              for n in ("SYN_SRC_SLC_IDS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS", "SYN_STRENGTHS")}
+    a.regrow(seed=9, epoch=2, reach=6)
+    b.regrow(seed=9, epoch=2, reach=6)
+    c.regrow(seed=9, epoch=3, reach=6)
+    changed_any = False
+    for li in range(2):
+        weak = before[(li, "SYN_STRENGTHS")] <= -25
+        for n in ("SYN_SRC_SLC_IDS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS", "SYN_STRENGTHS"):
+            x = a.read(n, li)
+            assert np.array_equal(x[~weak], before[(li, n)][~weak]), (li, n)
+            assert np.array_equal(x, b.read(n, li))
+            changed_any |= not np.array_equal(x, c.read(n, li))
+        # regrown strengths are small: |s| <= 3 * intensity <= 3
+        assert np.all(np.abs(a.read("SYN_STRENGTHS", li)[weak]) <= 3)
+    assert changed_any                                # another epoch draws other sources
+    for o in (a, b, c):
+        o.close()
