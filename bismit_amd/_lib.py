@@ -157,6 +157,8 @@ SIGNATURES = [
                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_uint32]),
     ("bsm_batch_sync", C.c_int, [_P]),
     ("bsm_area_regrow", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_int32]),
+    ("bsm_batch_regrow", C.c_int, [_P, C.POINTER(C.c_uint8), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64), C.c_int32]),
     ("bsm_cortex_cycle", C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.c_uint32, C.c_uint32]),
     ("bsm_batch_time_cycles", C.c_int, [_P, C.c_uint32, C.c_int, C.POINTER(C.c_float)]),
     ("bsm_area_stage_plan", C.c_int, [_P, C.c_uint32, C.POINTER(StageInfo),

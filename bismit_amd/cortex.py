@@ -300,6 +300,20 @@ class Batch:
         L.check(self.lib.bsm_batch_query(self.h, C.byref(d)))
         return bool(d.value)
 
+    def regrow(self, seeds, epochs, reach=6, which=None):
+        """Deterministic regrowth of the batched areas (area i: seeds[i],
+        epochs[i]; only where which[i] when which is given), the same draws as
+        CorticalArea.regrow.  Waits for the enqueued cycles."""
+        n = len(self.areas)
+        sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        ep = (C.c_uint64 * n)(*[int(x) for x in epochs])
+        wh = None
+        if which is not None:
+            w = np.ascontiguousarray(which, dtype=np.uint8)
+            assert w.size == n
+            wh = L.u8ptr(w)
+        L.check(self.lib.bsm_batch_regrow(self.h, wh, sd, ep, reach))
+
     def set_inputs(self, dev_frames, strides, lens, offsets, n_frames):
         n = len(self.areas)
         fr = (C.c_void_p * n)(*[p.value if p is not None else None for p in dev_frames])

@@ -424,6 +424,59 @@ int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
     return 0;
 }
 
+int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int32_t reach,
+                  bool* widened) {
+    *widened = false;
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+            TuftRt& T = L.t[t];
+            const uint32_t n_src = std::min<uint32_t>(T.d.n_src_slcs, BSM_MAX_TUFT_SRCS);
+            if (!n_src) continue;
+            RegrowSrc src{};
+            uint32_t new_reach = 0;
+            for (uint32_t k = 0; k < n_src; k++) {
+                const SliceRt& sl = a->slices_h[T.d.src_slc_ids[k]];
+                src.ids[k] = T.d.src_slc_ids[k];
+                src.v_size[k] = sl.v_size;
+                src.u_size[k] = sl.u_size;
+                src.v_scale[k] = sl.v_scale;
+                src.u_scale[k] = sl.u_scale;
+                const bool spatial = sl.v_scale && sl.u_scale;
+                const int32_t rv = (reach * sl.v_scale) >> 4;
+                src.tile[k] = spatial ? hex_tile_size(rv, sl.v_scale, sl.u_scale) : 0u;
+                // largest |offset| a regrown synapse of this source can take
+                int32_t mo = spatial ? std::max(rv, (rv * sl.u_scale + sl.v_scale - 1) / sl.v_scale)
+                                     : std::max<int32_t>(sl.v_size / 2, sl.u_size / 2);
+                new_reach = std::max(new_reach, (uint32_t)mo);
+            }
+            const uint64_t key = mix64(seed ^ (epoch * 0xD1B54A32D192ED03ull)) ^
+                                 ((uint64_t)(l * 8 + t) << 56);
+            const uint64_t n_den = (uint64_t)L.cells * T.d.dens_per_tft;
+            const uint32_t grid = (uint32_t)((n_den + 255) / 256);
+            if (n_den) {
+                k_regrow<<<grid, 256, 0, s>>>(key, n_den, T.d.syns_per_den, n_src, src, reach,
+                                              L.syn_slc + T.syn_idz, (int8_t*)L.syn_v + T.syn_idz,
+                                              (int8_t*)L.syn_u + T.syn_idz,
+                                              (int8_t*)L.syn_str + T.syn_idz);
+                BSM_HIP(hipGetLastError());
+            }
+            // the source summary widens to the regrown sources; the fused plan
+            // stays exact while the summary is unchanged
+            if (new_reach > T.reach_v || new_reach > T.reach_u) *widened = true;
+            T.reach_v = std::max(T.reach_v, new_reach);
+            T.reach_u = std::max(T.reach_u, new_reach);
+            T.reach_max = std::max(T.reach_v, T.reach_u);
+            for (uint32_t k = 0; k < n_src; k++) {
+                const uint64_t bit = 1ull << (src.ids[k] & 63);
+                if (!(T.slc_bits[src.ids[k] >> 6] & bit)) *widened = true;
+                T.slc_bits[src.ids[k] >> 6] |= bit;
+            }
+        }
+    }
+    return 0;
+}
+
 }  // namespace bsm
 
 using namespace bsm;
@@ -801,54 +854,15 @@ int bsm_area_draw_rnd(bsm_area* a, int32_t* out, uint32_t n_cycles) {
 
 int bsm_area_regrow(bsm_area* a, uint64_t seed, uint64_t epoch, int32_t reach) {
     if (!a) return fail("null area");
-    if (a->batch_stream) return fail("area belongs to a batch");
+    if (a->batch_stream) return fail("area belongs to a batch (use bsm_batch_regrow)");
     if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
     BSM_HIP(hipSetDevice(a->ctx->device));
-    for (uint32_t l = 0; l < a->n_layers; l++) {
-        LayerRt& L = a->L[l];
-        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
-            TuftRt& T = L.t[t];
-            const uint32_t n_src = std::min<uint32_t>(T.d.n_src_slcs, BSM_MAX_TUFT_SRCS);
-            if (!n_src) continue;
-            RegrowSrc src{};
-            uint32_t new_reach = 0;
-            for (uint32_t k = 0; k < n_src; k++) {
-                const SliceRt& sl = a->slices_h[T.d.src_slc_ids[k]];
-                src.ids[k] = T.d.src_slc_ids[k];
-                src.v_size[k] = sl.v_size;
-                src.u_size[k] = sl.u_size;
-                src.v_scale[k] = sl.v_scale;
-                src.u_scale[k] = sl.u_scale;
-                const bool spatial = sl.v_scale && sl.u_scale;
-                const int32_t rv = (reach * sl.v_scale) >> 4;
-                src.tile[k] = spatial ? hex_tile_size(rv, sl.v_scale, sl.u_scale) : 0u;
-                // largest |offset| a regrown synapse of this source can take
-                int32_t mo = spatial ? std::max(rv, (rv * sl.u_scale + sl.v_scale - 1) / sl.v_scale)
-                                     : std::max<int32_t>(sl.v_size / 2, sl.u_size / 2);
-                new_reach = std::max(new_reach, (uint32_t)mo);
-            }
-            const uint64_t key = mix64(seed ^ (epoch * 0xD1B54A32D192ED03ull)) ^
-                                 ((uint64_t)(l * 8 + t) << 56);
-            const uint64_t n_den = (uint64_t)L.cells * T.d.dens_per_tft;
-            const uint32_t grid = (uint32_t)((n_den + 255) / 256);
-            if (n_den) {
-                k_regrow<<<grid, 256, 0, a->stream>>>(key, n_den, T.d.syns_per_den, n_src, src,
-                                                      reach, L.syn_slc + T.syn_idz,
-                                                      (int8_t*)L.syn_v + T.syn_idz,
-                                                      (int8_t*)L.syn_u + T.syn_idz,
-                                                      (int8_t*)L.syn_str + T.syn_idz);
-                BSM_HIP(hipGetLastError());
-            }
-            // the source summary widens to the regrown sources
-            T.reach_v = std::max(T.reach_v, new_reach);
-            T.reach_u = std::max(T.reach_u, new_reach);
-            T.reach_max = std::max(T.reach_v, T.reach_u);
-            for (uint32_t k = 0; k < n_src; k++)
-                T.slc_bits[src.ids[k] >> 6] |= 1ull << (src.ids[k] & 63);
-        }
+    bool widened = false;
+    if (regrow_launch(a, a->stream, seed, epoch, reach, &widened)) return 1;
+    if (widened) {
+        a->plan_dirty = true;
+        a->layout_epoch++;
     }
-    a->plan_dirty = true;
-    a->layout_epoch++;
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;
 }

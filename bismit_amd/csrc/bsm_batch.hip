@@ -681,6 +681,28 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     return run_cycles(b, 1, use_graph);
 }
 
+int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,
+                     const uint64_t* epochs, int32_t reach) {
+    if (!b || !seeds || !epochs) return fail("null arg");
+    if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
+    // every enqueued cycle completes first (the side stream joins the batch
+    // stream within each cycle); sync also brings dynamic parities to the host
+    if (bsm_batch_sync(b)) return 1;
+    for (size_t i = 0; i < b->areas.size(); i++) {
+        if (which && !which[i]) continue;
+        bsm_area* a = b->areas[i];
+        bool widened = false;
+        if (regrow_launch(a, b->stream, seeds[i], epochs[i], reach, &widened)) return 1;
+        if (widened) {
+            // the next batched cycle re-plans (layout epochs differ from the plan's)
+            a->plan_dirty = true;
+            a->layout_epoch++;
+        }
+    }
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
 int bsm_batch_query(bsm_batch* b, int* done) {
     if (!b || !done) return fail("null arg");
     hipError_t e = hipStreamQuery(b->stream);

@@ -288,6 +288,11 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
 int build_plan(bsm_area* a);
 int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out);
 int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len);
+// Enqueues the deterministic regrowth of every tuft of the area on stream s and
+// widens the tufts' source summaries; *widened reports whether a summary grew
+// (the fused plan is then stale).
+int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int32_t reach,
+                  bool* widened);
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
                uint32_t* n_stages);
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,

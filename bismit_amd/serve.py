@@ -9,7 +9,10 @@ streams 2%-sparse SDR frames at its own rate.  The server
     due, through one dynamic batch (one persistent launch per stage, captured as
     a hipGraph; idle areas are masked on the device),
   - records per-frame latency = completion of the frame's cycle - the frame's
-    arrival time, and aggregate synapse-updates/s.
+    arrival time, and aggregate synapse-updates/s,
+  - regrows a session's weak synapses after every `regrow_every` of its cycles
+    (the reference's CorticalArea::cycle regrows every 201 cycles,
+    src/cortex/cortical_area.rs:773-776,792-806), all due sessions in one masked call.
 Sessions that arrive while others run join at the next admission epoch; their
 frames overlap on the device with everyone else's.  Each area keeps its own rnd
 stream, state parity and input, so its result is the same as a plain replay of
@@ -89,6 +92,10 @@ class Session:
     def due(self, k):
         return self.t0 + k / self.spec.rate_hz
 
+    def regrow_seed(self, k):
+        """Regrowth seed of area k (its epochs count the session's regrowths)."""
+        return self.spec.seed + 15485863 * k + 1
+
     def finished(self):
         return self.done >= self.spec.n_frames
 
@@ -136,8 +143,11 @@ class ServeStats:
 
 class Server:
     def __init__(self, ctx, hbm_budget_bytes=96 << 30, max_sessions=64, epoch_s=0.05,
-                 use_graph=True, keep_frames=False, on_retire=None, setup_workers=2):
+                 use_graph=True, keep_frames=False, on_retire=None, setup_workers=2,
+                 regrow_every=201, regrow_reach=6):
         self.ctx = ctx
+        self.regrow_every = regrow_every      # 0: regrowth off
+        self.regrow_reach = regrow_reach
         self.budget = hbm_budget_bytes
         self.max_sessions = max_sessions
         self.epoch_s = epoch_s
@@ -164,6 +174,25 @@ class Server:
         areas = [a for s in self.resident for a in s.areas]
         if areas:
             self.batch = Batch(areas, self.ctx, dynamic=True)
+
+    def _regrow_due(self, active_set):
+        """Regrowth of the active sessions whose cycle about to complete is a
+        multiple of regrow_every (it belongs to that cycle, as upstream)."""
+        n = len(self.batch.areas)
+        which = np.zeros(n, np.uint8)
+        seeds = [0] * n
+        epochs = [0] * n
+        idx = 0
+        for s in self.resident:
+            c = s.done + 1
+            if id(s) in active_set and c % self.regrow_every == 0:
+                for k in range(len(s.areas)):
+                    which[idx + k] = 1
+                    seeds[idx + k] = s.regrow_seed(k)
+                    epochs[idx + k] = c // self.regrow_every - 1
+            idx += len(s.areas)
+        if which.any():
+            self.batch.regrow(seeds, epochs, self.regrow_reach, which)
 
     def _setup(self, spec, fp, st):
         t0 = time.perf_counter()
@@ -275,6 +304,8 @@ class Server:
             frames = np.concatenate(chunks) if chunks else np.zeros(0, np.uint8)
             t0 = clock()
             self.batch.cycle_masked(mask, frames, ofs, graph=self.use_graph)
+            if self.regrow_every:
+                self._regrow_due(active_set)
             self.batch.sync()
             t1 = clock()
             st.gpu_busy_s += t1 - t0

@@ -300,7 +300,7 @@ int bsm_gen_tuft_layout(uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n
                         uint32_t n_src, const uint8_t* src_ids, int32_t reach,
                         uint8_t* out_slc, int8_t* out_v, int8_t* out_u);
 /* Fill every tuft of every layer of the area with bsm_gen_tuft_layout and
- * zero strengths (runs the generator on the host, then uploads). */
+ * zero strengths (the same generator, run on the device). */
 int bsm_area_init_layout(bsm_area* area, uint64_t seed, int32_t reach);
 /* Deterministic regrowth, replacing CorticalArea::regrow (src/cortex/cortical_area.rs:
  * 792-806) -> Synapses::grow / regrow_syn (src/cortex/synapses.rs:411-501) and
@@ -314,7 +314,7 @@ int bsm_area_init_layout(bsm_area* area, uint64_t seed, int32_t reach);
  * redrawn up to 31 times while another synapse of the dendrite has the same source.
  * The reference calls it every SYNAPSE_REGROWTH_INTERVAL + 1 = 201 cycles unless
  * regrowth is disabled; the caller decides when (epoch numbers the calls).
- * Completes on return. */
+ * Completes on return.  Areas inside a batch regrow through bsm_batch_regrow. */
 int bsm_area_regrow(bsm_area* area, uint64_t seed, uint64_t epoch, int32_t reach);
 
 /* ---- deterministic rnd schedule helpers ---- */
@@ -338,6 +338,13 @@ int bsm_batch_set_input_ranges(bsm_batch* b, const uint32_t* axn_offset, const u
 int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* frames,
                            const uint64_t* frame_ofs, uint64_t frame_bytes, int use_graph);
 int bsm_batch_query(bsm_batch* b, int* done);
+/* Regrowth of batched areas (the batch's form of bsm_area_regrow, same draws):
+ * area i regrows with seeds[i] and epochs[i] when which is null or which[i] is
+ * nonzero, so each area of a dynamic batch keeps its own 201-cycle schedule
+ * (src/cortex/cortical_area.rs:792-806).  Waits for the batch's enqueued cycles
+ * and completes on return; a widened source summary re-plans the next cycle. */
+int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,
+                     const uint64_t* epochs, int32_t reach);
 /* Inter-area tracts (thalamic routing, src/subcortex/thalamus/thalamus.rs:71-285,
  * src/cortex/axon_space.rs:509-629): at the start of every batch cycle, before
  * any area cycles, each tract copies len axon states of area src_area (as the

@@ -234,9 +234,62 @@ def test_dynamic_batch_masks(ctx, graph):
         assert_same(batched[i], plain[i], s, f"dyn area {i}")
 
 
+def test_static_batch_regrowth_matches_plain(ctx):
+    """Regrowth of batched areas (bsm_batch_regrow, a masked call then a full
+    one) between graph-captured batched cycles equals each plain area's own
+    regrowth at the same cycles; strengths start in [-40, 10] so regrowth has
+    work, and the batch re-plans when a regrown source summary widens."""
+    schemes = uneven_schemes()
+    n = 50
+    batched = make_areas(ctx, schemes, base_seed=300)
+    plain = make_areas(ctx, schemes, base_seed=300)
+    rng = np.random.default_rng(5)
+    for s, x, y in zip(schemes, batched, plain):
+        for li in range(len(s.layers)):
+            st = rng.integers(-40, 11, x.buf_len("SYN_STRENGTHS", li)).astype(np.int8)
+            x.write("SYN_STRENGTHS", li, st)
+            y.write("SYN_STRENGTHS", li, st)
+    frames = [frames_for(s, n, 70 + i) for i, s in enumerate(schemes)]
+    devs = []
+    for f in frames:
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        devs.append(d)
+    seeds = [900 + 13 * i for i in range(len(schemes))]
+    which = np.array([1, 0, 1, 1, 0], np.uint8)
+    b = Batch(batched, ctx)
+    b.set_inputs(devs, [f.shape[1] for f in frames], [f.shape[1] for f in frames],
+                 [s.input_range()[0] for s in schemes], n)
+    b.cycle(20, graph=True)
+    before = batched[0].read("SYN_SRC_COL_V_OFFS", 0).copy()
+    b.regrow(seeds, [0] * len(schemes), reach=6, which=which)
+    assert not np.array_equal(before, batched[0].read("SYN_SRC_COL_V_OFFS", 0))
+    b.cycle(20, graph=True)
+    b.regrow(seeds, [1] * len(schemes), reach=6)
+    b.cycle(10, graph=True)
+    b.sync()
+    b.close()
+    for i, a in enumerate(plain):
+        for c in range(n):
+            a.intake(frames[i][c])
+            a.cycle(1)
+            if c == 19 and which[i]:
+                a.regrow(seeds[i], 0, reach=6)
+            if c == 39:
+                a.regrow(seeds[i], 1, reach=6)
+    for i, s in enumerate(schemes):
+        assert_same(batched[i], plain[i], s, f"regrow area {i}")
+    for a in batched + plain:
+        a.close()
+    for d in devs:
+        ctx.free(d)
+
+
 def test_serving_sessions_match_solo_replay(ctx):
     """Continuous batching: sessions arrive while others run, are admitted as
-    capacity frees up, and each session's final state equals its solo replay."""
+    capacity frees up, and each session's final state equals its solo replay.
+    Regrowth every 25 of a session's cycles (batched, masked to the due
+    sessions) matches the solo area's own regrowth at the same counts."""
     specs = [SessionSpec(i, 0.02 * i, [16 + 8 * (i % 3)] * (1 + i % 2), 200.0 + 50 * i,
                          0.15 + 0.05 * (i % 3), 1000 + i) for i in range(8)]
     finals = {}
@@ -247,8 +300,9 @@ def test_serving_sessions_match_solo_replay(ctx):
              for name in layer_bufs(sc, li)} | {("AXN_STATES", 0): a.read("AXN_STATES")}
             for a, sc in zip(sess.areas, sess.schemes)])
 
+    every = 25
     srv = Server(ctx, hbm_budget_bytes=1 << 30, max_sessions=4, epoch_s=0.01,
-                 on_retire=on_retire)
+                 on_retire=on_retire, regrow_every=every)
     st = srv.run(specs, wall_limit_s=60.0)
     summ = st.summary()
     print(summ, flush=True)
@@ -263,9 +317,11 @@ def test_serving_sessions_match_solo_replay(ctx):
             solo = CorticalArea(sc, ctx, rnd_seed=spec.seed + 7919 * i,
                                 layout_seed=spec.seed + 104729 * i, reach=6)
             stream = inputs.SdrStream(sc.input_range()[1], 0.02, 0.5, seed=spec.seed + i)
-            for _ in range(done):
+            for c in range(1, done + 1):
                 solo.intake(stream.next_frame())
                 solo.cycle(1)
+                if c % every == 0:
+                    solo.regrow(spec.seed + 15485863 * i + 1, c // every - 1, reach=6)
             for (name, li), arr in bufs[i].items():
                 assert np.array_equal(solo.read(name, li), arr), (sid, i, name, li)
             solo.close()
