@@ -104,6 +104,17 @@ SIGNATURES = [
     ("bsm_area_slice_idz", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("bsm_area_buf_len", C.c_int, [_P, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)]),
     ("bsm_area_read", C.c_int, [_P, C.c_int, C.c_uint32, C.c_size_t, C.c_size_t, _P]),
+    ("bsm_sampler_create", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
+                                     C.POINTER(_P)]),
+    ("bsm_sampler_recv", C.c_int, [_P, C.c_int, _P, C.c_uint64, C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_int)]),
+    ("bsm_sampler_recv_many", C.c_int, [C.POINTER(_P), C.c_uint32, C.c_int, C.POINTER(_P),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_int)]),
+    ("bsm_sampler_set_backpressure", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("bsm_sampler_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]),
+    ("bsm_sampler_destroy", C.c_int, [_P]),
     ("bsm_area_write", C.c_int, [_P, C.c_int, C.c_uint32, C.c_size_t, C.c_size_t, _P]),
     ("bsm_area_fill", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint8]),
     ("bsm_area_intake", C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_int]),

@@ -3,7 +3,8 @@
 Reference interface (src/cortex/cortex.rs:46-120, src/cortex/cortical_area.rs:376-1395):
   Cortex::cycle            -> Cortex.cycle(inputs)
   CorticalArea::cycle      -> CorticalArea.cycle(n, rnd)
-  CorticalArea::sampler    -> CorticalArea.sampler(kind, layer)
+  CorticalArea::sampler    -> CorticalArea.sampler(kind, buffer_kind, backpressure)
+  CorticalArea::regrow     -> CorticalArea.regrow(seed, epoch, reach)
   *Test traits (cycle_solo, set_all_to_zero, read_idx_direct, write_to_axon, ...)
                            -> read/write/fill and the per-kernel k_* methods
 Every call goes to libbismit_hip.so; nothing here computes the cycle.
@@ -210,10 +211,12 @@ class CorticalArea:
         return {"tile_v": t.tile_v, "tile_u": t.tile_u, "halo": t.halo, "chunk": t.chunk,
                 "inhib_tile": t.inhib_tile, "overlap": t.overlap}
 
-    def sampler(self, kind, layer=0):
-        """A callable returning the current contents of one buffer
-        (CorticalArea::sampler, src/cortex/cortical_area.rs:1017-1284)."""
-        return lambda: self.read(kind, layer)
+    def sampler(self, kind, buffer_kind=None, backpressure=False):
+        """Per-cycle readback of one buffer kind (CorticalArea::sampler,
+        src/cortex/cortical_area.rs:1017-1284): returns a TractReceiver whose
+        recv() yields the frame of a cycle.  kind is a sampler.SamplerKind."""
+        from . import sampler as S
+        return S.create(self, kind, buffer_kind or S.SamplerBufferKind.SINGLE, backpressure)
 
     # ------------------------------------------------------------- kernels
     def k_syn_flags(self, layer):
@@ -385,6 +388,11 @@ class Cortex:
             if name in inputs:
                 area.intake(inputs[name])
             area.cycle(1)
+
+    def finish_queues(self):
+        """Cortex::finish_queues: wait for every area's enqueued work."""
+        for a in self.areas.values():
+            a.sync()
 
     def cycle_device(self, dev_frames, lens, n_frames, n_cycles, offsets=None):
         """n_cycles host-driven cycles in C (bsm_cortex_cycle): each cycle, each

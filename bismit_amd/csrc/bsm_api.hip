@@ -513,6 +513,9 @@ int bsm_ctx_create(int device, bsm_ctx** out) {
 }
 
 int bsm_ctx_destroy(bsm_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    sample_arena_destroy(ctx);
     delete ctx;
     return 0;
 }
@@ -622,6 +625,9 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->plan_dirty = true;
     a->batch_stream = nullptr;
     a->layout_epoch = 0;
+    a->sampler_gen = 0;
+    a->sample_plan = nullptr;
+    a->cycles_done = 0;
     *out = a;
     return 0;
 }
@@ -631,6 +637,7 @@ int bsm_area_destroy(bsm_area* a) {
     if (a->batch_stream) return fail("area belongs to a batch; destroy the batch first");
     (void)hipSetDevice(a->ctx->device);
     (void)hipStreamSynchronize(a->stream);
+    samplers_detach(a);
     (void)hipFree(a->arena);
     (void)hipFree(a->rnd_d);
     (void)hipFree(a->tjobs_d);
@@ -678,7 +685,9 @@ int bsm_area_slice_idz(bsm_area* a, uint32_t slc, uint32_t* idz) {
 
 // Resolve (buf, layer) to a device range.  SYN_FLAG_SETS is virtual: static
 // bits in syn_flag_static, SYN_PREV_ACTIVE from the previous state buffer.
-static int resolve(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len) {
+}  // extern "C"
+
+int bsm::resolve_buf(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len) {
     if (buf == BSM_AXN_STATES) {
         *ptr = a->axn;
         *len = a->axn_len;
@@ -722,17 +731,19 @@ static int resolve(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* 
     }
 }
 
+extern "C" {
+
 int bsm_area_buf_len(bsm_area* a, int buf, uint32_t layer, size_t* len) {
     if (!a || !len) return fail("null arg");
     uint8_t* p;
-    return resolve(a, buf, layer, &p, len);
+    return resolve_buf(a, buf, layer, &p, len);
 }
 
 int bsm_area_read(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len, void* dst) {
     if (!a || (!dst && len)) return fail("null arg");
     uint8_t* p;
     size_t n;
-    if (resolve(a, buf, layer, &p, &n)) return 1;
+    if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("read range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
     BSM_HIP(hipStreamSynchronize(a->stream));
@@ -756,7 +767,7 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
     if (!a || (!src && len)) return fail("null arg");
     uint8_t* p;
     size_t n;
-    if (resolve(a, buf, layer, &p, &n)) return 1;
+    if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("write range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
     BSM_HIP(hipStreamSynchronize(a->stream));
@@ -784,7 +795,7 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
     if (!a) return fail("null area");
     uint8_t* p;
     size_t n;
-    if (resolve(a, buf, layer, &p, &n)) return 1;
+    if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
     if (buf == BSM_SYN_FLAG_SETS) {

@@ -141,6 +141,7 @@ struct bsm_batch {
     uint32_t io_parts = 1;      // prologue blocks per area (largest axon space / COPY_SEG)
     uint32_t tract_parts = 1;   // tract blocks per tract (longest tract / COPY_SEG)
     std::vector<uint32_t> epochs;   // areas' layout epochs the plan was made for
+    bsm::SamplePlan* sample_plan = nullptr;   // samplers of the batched areas
 };
 
 namespace bsm {
@@ -358,18 +359,22 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = 
         } else if (enqueue_cycle(b, b->bpar, ev ? ev + (size_t)2 * c * b->stages.size() : nullptr)) {
             return 1;
         }
-    }
-    // host bookkeeping: every batched layer's current buffer flipped n times
-    // (dynamic batches read the parity words back at sync instead)
-    if (b->dynamic) return 0;
-    for (bsm_area* a : b->areas) {
-        const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
-        const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
-        for (uint32_t l = 0; l < a->n_layers; l++) {
-            LayerRt& L = a->L[l];
-            const bool spatial = L.d.spatial != 0;
-            if ((spatial && sscs) || (!spatial && pyrs)) L.cur ^= (int)(n & 1u);
+        // dynamic batches keep parities on the device (read back at sync) and
+        // sample after the masked cycle
+        if (b->dynamic) continue;
+        // host bookkeeping: every batched layer's current buffer flips each cycle
+        for (bsm_area* a : b->areas) {
+            const bool sscs = !(a->settings & BSM_SET_DISABLE_SSCS);
+            const bool pyrs = !(a->settings & BSM_SET_DISABLE_PYRS);
+            for (uint32_t l = 0; l < a->n_layers; l++) {
+                LayerRt& L = a->L[l];
+                const bool spatial = L.d.spatial != 0;
+                if ((spatial && sscs) || (!spatial && pyrs)) L.cur ^= 1;
+            }
         }
+        if (sample_after_cycle(b->sample_plan, b->stream, b->areas.data(), b->areas.size(),
+                               nullptr, nullptr))
+            return 1;
     }
     return 0;
 }
@@ -514,6 +519,7 @@ int bsm_batch_destroy(bsm_batch* b) {
     }
     (void)hipFree(b->rng_d);
     (void)hipFree(b->counter_d);
+    sample_plan_free(b->sample_plan);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
     side_destroy(b->side);
@@ -678,7 +684,8 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     if (frame_bytes) std::memcpy(h + hdr, frames, frame_bytes);
     BSM_HIP(hipMemcpyAsync(b->xfer_d, h, total, hipMemcpyHostToDevice, b->stream));
     BSM_HIP(hipEventRecord(b->xfer_ev[k], b->stream));
-    return run_cycles(b, 1, use_graph);
+    if (run_cycles(b, 1, use_graph)) return 1;
+    return sample_after_cycle(b->sample_plan, b->stream, b->areas.data(), n, active, b->dyn_d);
 }
 
 int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,

@@ -1312,12 +1312,15 @@ int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len)
 // stages run on the area stream (and its side stream).
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial) {
     for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
-    return enqueue_stages(a->stream, a->side, a->stages, serial,
-                          [&](const Stage& st, hipStream_t s) {
-                              return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d,
-                                                  a->items_d, (uint32_t)a->L[st.layer].cur,
-                                                  rnd_ofs);
-                          });
+    if (enqueue_stages(a->stream, a->side, a->stages, serial,
+                       [&](const Stage& st, hipStream_t s) {
+                           return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d,
+                                               a->items_d, (uint32_t)a->L[st.layer].cur,
+                                               rnd_ofs);
+                       }))
+        return 1;
+    // readback of the area's samplers, after the cycle on its stream
+    return sample_after_cycle(a->sample_plan, a->stream, &a, 1, nullptr, nullptr);
 }
 
 // Upload n_cycles of rnd draws (user-given or drawn from the area stream) to
@@ -1487,6 +1490,7 @@ int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages,
                 return 1;
             BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i) + 1], a->stream));
         }
+        if (sample_after_cycle(a->sample_plan, a->stream, &a, 1, nullptr, nullptr)) return 1;
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     for (uint32_t i = 0; i < ns; i++) {

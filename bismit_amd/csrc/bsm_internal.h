@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -226,9 +227,16 @@ struct SliceSet {
 
 }  // namespace bsm
 
+namespace bsm {
+struct SampleArena;   // pinned + device slots of every sampler of a context
+struct SamplePlan;    // device descriptors of the samplers one launch serves
+}  // namespace bsm
+
 struct bsm_ctx {
     int device;
     hipDeviceProp_t prop;
+    bsm::SampleArena* samples = nullptr;
+    std::atomic<uint64_t> sampler_gen{0};   // bumped by every sampler create/destroy
 };
 
 struct bsm_area {
@@ -267,6 +275,12 @@ struct bsm_area {
     hipEvent_t ev0, ev1;
     hipStream_t batch_stream;  // non-null while the area belongs to a batch
     uint32_t layout_epoch;     // bumped whenever the synapse source summary changes
+    // readback (samplers): the area's open samplers, the generation of that
+    // set, the plan serving the plain form, and the number of cycles done
+    std::vector<bsm_sampler*> samplers;
+    uint64_t sampler_gen;
+    bsm::SamplePlan* sample_plan;
+    uint64_t cycles_done;
 };
 
 namespace bsm {
@@ -298,6 +312,19 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial);
+// Device pointer and length of one buffer kind of the area (the current state
+// buffer for BSM_SYN_STATES; the static flag bits for BSM_SYN_FLAG_SETS).
+int resolve_buf(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len);
+// Readback after a cycle (bsm_sample.hip).  Counts the cycle of every area that
+// stepped (active == null: all) and, where those areas have samplers, enqueues
+// their frames on stream s.  Parities come from the host (a->L[l].cur) unless
+// dyn (BSM_MAX_LAYERS device words per area, bit 0 the parity) is given.
+int sample_after_cycle(SamplePlan*& plan, hipStream_t s, bsm_area* const* areas, size_t n,
+                       const uint8_t* active, const uint32_t* dyn);
+void sample_plan_free(SamplePlan*& plan);
+// Closes the area's samplers (their frames end) before the area goes away.
+void samplers_detach(bsm_area* a);
+void sample_arena_destroy(bsm_ctx* c);
 int ensure_rnd(bsm_area* a, size_t n_ints);
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids);

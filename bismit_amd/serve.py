@@ -8,8 +8,10 @@ streams 2%-sparse SDR frames at its own rate.  The server
   - at every shared cycle steps exactly the areas whose session has a frame
     due, through one dynamic batch (one persistent launch per stage, captured as
     a hipGraph; idle areas are masked on the device),
-  - records per-frame latency = completion of the frame's cycle - the frame's
-    arrival time, and aggregate synapse-updates/s,
+  - reads every stepped area's output axons back through a sampler (one gather
+    kernel and one copy per cycle for all sessions), hands them to the session
+    (`on_output`), and records per-frame latency = the frame's output readable
+    on the host - the frame's arrival time, and aggregate synapse-updates/s,
   - regrows a session's weak synapses after every `regrow_every` of its cycles
     (the reference's CorticalArea::cycle regrows every 201 cycles,
     src/cortex/cortical_area.rs:773-776,792-806), all due sessions in one masked call.
@@ -33,6 +35,7 @@ import numpy as np
 from . import recipes
 from .cortex import Batch, CorticalArea, footprint
 from .inputs import SdrStream
+from .sampler import SamplerKind, recv_many
 
 
 @dataclass
@@ -73,8 +76,14 @@ def area_scheme(side, name):
     return recipes.cortical_area(name, side)
 
 
+def output_layer(scheme):
+    """The layer whose axon slices are a session area's output: the last
+    pyramidal layer."""
+    return max(i for i, lay in enumerate(scheme.layers) if not lay.spatial)
+
+
 class Session:
-    def __init__(self, spec: SessionSpec, ctx, t_admit, keep_frames=False):
+    def __init__(self, spec: SessionSpec, ctx, t_admit, keep_frames=False, sample_outputs=True):
         self.spec = spec
         self.t0 = t_admit      # reset when the session joins the batch
         self.schemes = [area_scheme(s, f"s{spec.sid}a{i}") for i, s in enumerate(spec.sides)]
@@ -88,6 +97,10 @@ class Session:
         self.done = 0                      # frames completed
         self.synapses = sum(sc.n_synapses() for sc in self.schemes)
         self.log = [] if keep_frames else None
+        # per-cycle output readback (no backpressure: a slow consumer skips frames)
+        self.outputs = [a.sampler(SamplerKind.Axons(output_layer(sc)))
+                        for a, sc in zip(self.areas, self.schemes)] if sample_outputs else []
+        self.last_outputs = [None] * len(self.areas)
 
     def due(self, k):
         return self.t0 + k / self.spec.rate_hz
@@ -100,6 +113,8 @@ class Session:
         return self.done >= self.spec.n_frames
 
     def close(self):
+        for rx in self.outputs:
+            rx.close()
         for a in self.areas:
             a.close()
 
@@ -144,8 +159,10 @@ class ServeStats:
 class Server:
     def __init__(self, ctx, hbm_budget_bytes=96 << 30, max_sessions=64, epoch_s=0.05,
                  use_graph=True, keep_frames=False, on_retire=None, setup_workers=2,
-                 regrow_every=201, regrow_reach=6):
+                 regrow_every=201, regrow_reach=6, sample_outputs=True, on_output=None):
         self.ctx = ctx
+        self.sample_outputs = sample_outputs
+        self.on_output = on_output            # on_output(session, [Frame per area])
         self.regrow_every = regrow_every      # 0: regrowth off
         self.regrow_reach = regrow_reach
         self.budget = hbm_budget_bytes
@@ -175,6 +192,20 @@ class Server:
         if areas:
             self.batch = Batch(areas, self.ctx, dynamic=True)
 
+    def _deliver(self, active_sessions):
+        """Every stepped area's output frame of this cycle, to its session."""
+        rxs = [rx for s in active_sessions for rx in s.outputs]
+        frames = recv_many(rxs, wait_for_frame=False)
+        k = 0
+        for s in active_sessions:
+            got = frames[k:k + len(s.outputs)]
+            k += len(s.outputs)
+            if any(f is None for f in got):
+                raise RuntimeError(f"session {s.spec.sid}: output frame missing")
+            s.last_outputs = got
+            if self.on_output:
+                self.on_output(s, got)
+
     def _regrow_due(self, active_set):
         """Regrowth of the active sessions whose cycle about to complete is a
         multiple of regrow_every (it belongs to that cycle, as upstream)."""
@@ -196,7 +227,7 @@ class Server:
 
     def _setup(self, spec, fp, st):
         t0 = time.perf_counter()
-        sess = Session(spec, self.ctx, 0.0, self.keep_frames)
+        sess = Session(spec, self.ctx, 0.0, self.keep_frames, self.sample_outputs)
         sess.fp = fp
         st.setup_s.append(time.perf_counter() - t0)
         with self._ready_lock:
@@ -307,6 +338,8 @@ class Server:
             if self.regrow_every:
                 self._regrow_due(active_set)
             self.batch.sync()
+            if self.sample_outputs:
+                self._deliver(active_sessions)
             t1 = clock()
             st.gpu_busy_s += t1 - t0
             for s in active_sessions:

@@ -144,6 +144,7 @@ enum bsm_rnd_kind { BSM_RND_DEN = 0, BSM_RND_CTRL = 1, BSM_RND_MTP = 2 };
 typedef struct bsm_ctx bsm_ctx;
 typedef struct bsm_area bsm_area;
 typedef struct bsm_batch bsm_batch;
+typedef struct bsm_sampler bsm_sampler;
 
 /* ---- context (replaces ProQue/Context construction, src/cortex/cortex.rs:51-90) ---- */
 const char* bsm_last_error(void);
@@ -180,6 +181,37 @@ int bsm_area_fill(bsm_area* area, int buf, uint32_t layer, uint8_t val);
 int bsm_area_intake(bsm_area* area, uint32_t axn_offset, const uint8_t* src, uint32_t len,
                     int src_is_device);
 int bsm_area_output(bsm_area* area, uint32_t axn_offset, uint8_t* host_dst, uint32_t len);
+
+/* ---- samplers: per-cycle readback (replaces CorticalArea::sampler and
+ *      cycle_samplers, src/cortex/cortical_area.rs:808-1284, with the single-buffer
+ *      tract channel of src/subcortex/tract_channel.rs:369-480) ----
+ * A sampler reads bytes [offset, offset + len) of one buffer kind of a layer
+ * (layer ignored for BSM_AXN_STATES) at the end of every cycle the area steps,
+ * plain or batched.  Channel semantics: a cycle sends a frame when the previous
+ * one has been received; otherwise it waits for the receiver when backpressure
+ * is on and skips the frame when it is off, so a receiver always gets the frame
+ * of the first cycle after its last receive.  Frames are gathered on the device
+ * into pinned host slots (one kernel and one copy per cycle for all samplers of
+ * an area or batch), ordered after the cycle on its stream.
+ * Create and destroy on the thread that drives the area; recv and backpressure
+ * calls may come from any thread.  Destroying an area closes its samplers (recv
+ * then reports no frame; the handle still needs bsm_sampler_destroy). */
+int bsm_sampler_create(bsm_area* area, int buf, uint32_t layer, uint64_t offset, uint64_t len,
+                       int backpressure, bsm_sampler** out);
+/* Takes the pending frame into dst (dst_len >= the sampler length): *got = 1 and
+ * *cycle = the area's cycle count at that frame (1 = the first cycle after the
+ * area was created).  No frame pending: waits for the next when wait_for_frame
+ * is nonzero (returns *got = 0 if the sampler closes meanwhile), else *got = 0. */
+int bsm_sampler_recv(bsm_sampler* s, int wait_for_frame, void* dst, uint64_t dst_len,
+                     uint64_t* cycle, int* got);
+/* bsm_sampler_recv over n samplers in order (one call per cycle for a server
+ * that hands every session its output); cycles may be null. */
+int bsm_sampler_recv_many(bsm_sampler* const* s, uint32_t n, int wait_for_frame,
+                          void* const* dst, const uint64_t* dst_len, uint64_t* cycles, int* got);
+/* Sets backpressure and returns the prior state in *prior (may be null). */
+int bsm_sampler_set_backpressure(bsm_sampler* s, int backpressure, int* prior);
+int bsm_sampler_info(bsm_sampler* s, uint64_t* len, int* backpressure, int* pending);
+int bsm_sampler_destroy(bsm_sampler* s);
 
 /* ---- per-kernel entry points, reference argument semantics (parity units) ---- */
 int bsm_k_syn_flags(bsm_area* a, uint32_t layer);                        /* cl/syns.cl:23-33 */

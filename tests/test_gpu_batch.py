@@ -298,19 +298,26 @@ def test_serving_sessions_match_solo_replay(ctx):
         finals[sess.spec.sid] = (sess.spec, sess.done, [
             {(name, li): a.read(name, li) for li in range(len(sc.layers))
              for name in layer_bufs(sc, li)} | {("AXN_STATES", 0): a.read("AXN_STATES")}
-            for a, sc in zip(sess.areas, sess.schemes)])
+            for a, sc in zip(sess.areas, sess.schemes)], sess.last_outputs)
+
+    n_out = [0]
+
+    def on_output(sess, frames):
+        n_out[0] += 1
+        assert all(f.cycle == sess.done + 1 for f in frames)
 
     every = 25
     srv = Server(ctx, hbm_budget_bytes=1 << 30, max_sessions=4, epoch_s=0.01,
-                 on_retire=on_retire, regrow_every=every)
+                 on_retire=on_retire, regrow_every=every, on_output=on_output)
     st = srv.run(specs, wall_limit_s=60.0)
     summ = st.summary()
     print(summ, flush=True)
     assert summ["sessions_completed"] == len(specs)
     assert summ["max_resident_sessions"] <= 4
     assert summ["batch_rebuilds"] > 2
-    from bismit_amd.serve import area_scheme
-    for sid, (spec, done, bufs) in finals.items():
+    assert n_out[0] == summ["frames"]      # one delivery per session-cycle
+    from bismit_amd.serve import area_scheme, output_layer
+    for sid, (spec, done, bufs, outs) in finals.items():
         assert done == spec.n_frames
         for i, side in enumerate(spec.sides):
             sc = area_scheme(side, f"r{sid}{i}")
@@ -324,6 +331,10 @@ def test_serving_sessions_match_solo_replay(ctx):
                     solo.regrow(spec.seed + 15485863 * i + 1, c // every - 1, reach=6)
             for (name, li), arr in bufs[i].items():
                 assert np.array_equal(solo.read(name, li), arr), (sid, i, name, li)
+            # the last output frame the session received is the final output slices
+            off, n = sc.layer_axn_range(output_layer(sc))
+            assert outs[i].cycle == done
+            assert np.array_equal(outs[i].data, solo.read("AXN_STATES")[off:off + n]), (sid, i)
             solo.close()
 
 

@@ -1,0 +1,226 @@
+"""Samplers (per-cycle readback): every frame equals a read of the same buffer
+after the same cycle, in the plain form, static and dynamic batches; the
+single-buffer channel skips frames without backpressure and holds the cycle
+loop with it (src/subcortex/tract_channel.rs:381-480)."""
+import threading
+
+import numpy as np
+import pytest
+
+from bismit_amd import inputs, recipes
+from bismit_amd.cortex import Batch, CorticalArea
+from bismit_amd.sampler import SamplerKind, _KINDS
+
+pytestmark = pytest.mark.gpu
+
+SSC_ONLY_MISSING = {"SomaFlagSets", "TuftStates", "TuftBestDenIds", "TuftBestDenStatesRaw",
+                    "TuftBestDenStates", "TuftPrevStates", "TuftPrevBestDenIds",
+                    "TuftPrevBestDenStatesRaw", "TuftPrevBestDenStates"}
+
+
+def kinds_for(scheme):
+    out = [SamplerKind.Axons(None)]
+    for li, lay in enumerate(scheme.layers):
+        out.append(SamplerKind.Axons(li))
+        for name in _KINDS:
+            if name == "Axons" or (lay.spatial and name in SSC_ONLY_MISSING):
+                continue
+            out.append(SamplerKind(name, li))
+    return out
+
+
+def expected(area, kind):
+    if kind.buf == "AXN_STATES":
+        ax = area.read("AXN_STATES")
+        if kind.layer is None:
+            return ax
+        off, n = area.scheme.layer_axn_range(kind.layer)
+        return ax[off:off + n]
+    return area.read(kind.buf, kind.layer)
+
+
+def glyphs(scheme, n, seed):
+    return inputs.GlyphStream(side=24, p=0.15, hold=3, seed=seed).frames(n)
+
+
+def test_plain_sampler_frames_every_kind(ctx):
+    scheme = recipes.config1_area()
+    a = CorticalArea(scheme, ctx, rnd_seed=31, layout_seed=32)
+    ks = kinds_for(scheme)
+    rxs = [a.sampler(k) for k in ks]
+    fr = glyphs(scheme, 12, 5)
+    for c in range(12):
+        a.intake(fr[c])
+        a.cycle(1)
+        for k, rx in zip(ks, rxs):
+            f = rx.recv(wait_for_frame=True)
+            assert f is not None and f.cycle == c + 1, (k, c)
+            assert np.array_equal(f.data, expected(a, k)), (k, c)
+    # the frames carry activity, so a stale or wrong buffer would show
+    assert np.count_nonzero(expected(a, SamplerKind.SynStates(0))) > 0
+    for rx in rxs:
+        rx.close()
+    a.close()
+
+
+def test_sampler_skips_without_backpressure(ctx):
+    scheme = recipes.config1_area()
+    a = CorticalArea(scheme, ctx, rnd_seed=7, layout_seed=8)
+    rx = a.sampler(SamplerKind.Axons(None))
+    fr = glyphs(scheme, 8, 9)
+    a.intake(fr[0])
+    a.cycle(1)
+    first = a.read("AXN_STATES")
+    for c in range(1, 5):             # frames of cycles 2..5 are skipped
+        a.intake(fr[c])
+        a.cycle(1)
+    assert rx.pending()
+    f = rx.recv(False)
+    assert f.cycle == 1 and np.array_equal(f.data, first)
+    assert rx.recv(False) is None     # nothing pending until the next cycle
+    a.intake(fr[5])
+    a.cycle(1)
+    f = rx.recv(False)
+    assert f.cycle == 6 and np.array_equal(f.data, a.read("AXN_STATES"))
+    assert rx.set_backpressure(True) is False and rx.backpressure_stale()
+    rx.close()
+    a.close()
+
+
+def test_sampler_backpressure_holds_the_cycle_loop(ctx):
+    """With backpressure every cycle's frame reaches the receiver (another
+    thread); the frames equal a twin area read after each cycle."""
+    scheme = recipes.config1_area()
+    n = 10
+    fr = glyphs(scheme, n, 11)
+    twin = CorticalArea(scheme, ctx, rnd_seed=12, layout_seed=13)
+    want = []
+    for c in range(n):
+        twin.intake(fr[c])
+        twin.cycle(1)
+        want.append(twin.read("SOMA_STATES", 1))
+    twin.close()
+    a = CorticalArea(scheme, ctx, rnd_seed=12, layout_seed=13)
+    rx = a.sampler(SamplerKind.SomaStates(1), backpressure=True)
+    got = []
+
+    def receive():
+        for _ in range(n):
+            got.append(rx.recv(True))
+
+    t = threading.Thread(target=receive)
+    t.start()
+    for c in range(n):
+        a.intake(fr[c])
+        a.cycle(1)                    # blocks while the previous frame is unreceived
+    t.join(60)
+    assert not t.is_alive()
+    assert [f.cycle for f in got] == list(range(1, n + 1))
+    for c in range(n):
+        assert np.array_equal(got[c].data, want[c]), c
+    rx.close()
+    a.close()
+
+
+def uneven():
+    return [
+        recipes.cortical_area("s0", 24, input_side=16, ssc_depth=1, pyr_depth=2, ssc_spd=16,
+                              pyr_spd=20),
+        recipes.cortical_area("s1", 40, ssc_depth=2, pyr_depth=1, ssc_spd=32, pyr_spd=64),
+        recipes.cortical_area("s2", 32, ssc_depth=1, pyr_depth=2, ssc_spd=28, pyr_spd=36),
+    ]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_static_batch_samplers(ctx, graph):
+    schemes = uneven()
+    n = 9
+    batched = [CorticalArea(s, ctx, rnd_seed=60 + i, layout_seed=70 + i)
+               for i, s in enumerate(schemes)]
+    plain = [CorticalArea(s, ctx, rnd_seed=60 + i, layout_seed=70 + i)
+             for i, s in enumerate(schemes)]
+    frames = [inputs.SdrStream(s.input_range()[1], 0.03, 0.5, seed=80 + i).frames(n)
+              for i, s in enumerate(schemes)]
+    devs = []
+    for f in frames:
+        d = ctx.alloc(f.nbytes)
+        ctx.upload(d, f)
+        devs.append(d)
+    b = Batch(batched, ctx)
+    b.set_inputs(devs, [f.shape[1] for f in frames], [f.shape[1] for f in frames],
+                 [s.input_range()[0] for s in schemes], n)
+    # samplers created after the batch: the batch picks them up
+    probes = [(i, k) for i in range(len(schemes))
+              for k in (SamplerKind.Axons(None), SamplerKind.SynFlagSets(1),
+                        SamplerKind.SynStates(0), SamplerKind.TuftBestDenIds(1))]
+    rxs = [batched[i].sampler(k) for i, k in probes]
+    for c in range(n):
+        b.cycle(1, graph=graph)
+        for i, a in enumerate(plain):
+            a.intake(frames[i][c])
+            a.cycle(1)
+        for (i, k), rx in zip(probes, rxs):
+            f = rx.recv(True)
+            assert f.cycle == c + 1
+            assert np.array_equal(f.data, expected(plain[i], k)), (i, k, c)
+    b.close()
+    for rx in rxs:
+        rx.close()
+    for a in batched + plain:
+        a.close()
+    for d in devs:
+        ctx.free(d)
+
+
+def test_dynamic_batch_samplers_follow_the_active_mask(ctx):
+    schemes = uneven()
+    n = 8
+    batched = [CorticalArea(s, ctx, rnd_seed=90 + i, layout_seed=95 + i)
+               for i, s in enumerate(schemes)]
+    plain = [CorticalArea(s, ctx, rnd_seed=90 + i, layout_seed=95 + i)
+             for i, s in enumerate(schemes)]
+    streams = [inputs.SdrStream(s.input_range()[1], 0.03, 0.5, seed=99 + i)
+               for i, s in enumerate(schemes)]
+    b = Batch(batched, ctx, dynamic=True)
+    rxs = [a.sampler(SamplerKind.Axons(None)) for a in batched]
+    rx_syn = batched[1].sampler(SamplerKind.SynFlagSets(0))
+    rng = np.random.default_rng(3)
+    done = [0] * len(schemes)
+    for c in range(n):
+        mask = (rng.random(len(schemes)) < 0.6).astype(np.uint8)
+        mask[c % len(schemes)] = 1
+        chunks, ofs, pos = [], np.full(len(schemes), -1, np.int64), 0
+        for i in range(len(schemes)):
+            if mask[i]:
+                fr = streams[i].next_frame()
+                chunks.append(fr)
+                ofs[i] = pos
+                pos += fr.size
+                plain[i].intake(fr)
+                plain[i].cycle(1)
+                done[i] += 1
+        b.cycle_masked(mask, np.concatenate(chunks), ofs, graph=True)
+        for i, rx in enumerate(rxs):
+            f = rx.recv(False) if not mask[i] else rx.recv(True)
+            if not mask[i]:
+                assert f is None, (i, c)
+                continue
+            assert f.cycle == done[i]
+            assert np.array_equal(f.data, plain[i].read("AXN_STATES")), (i, c)
+        if mask[1]:
+            f = rx_syn.recv(True)
+            assert np.array_equal(f.data, plain[1].read("SYN_FLAG_SETS", 0)), c
+    b.close()
+    for rx in rxs + [rx_syn]:
+        rx.close()
+    for a in batched + plain:
+        a.close()
+
+
+def test_sampler_closes_with_its_area(ctx):
+    a = CorticalArea(recipes.config1_area(), ctx, rnd_seed=1, layout_seed=2)
+    rx = a.sampler(SamplerKind.DenStates(0))
+    a.cycle(1)
+    a.close()
+    assert rx.recv(True) is None      # closed: no frame, no wait
+    rx.close()
