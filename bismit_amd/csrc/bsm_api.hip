@@ -515,7 +515,7 @@ int bsm_ctx_create(int device, bsm_ctx** out) {
 int bsm_ctx_destroy(bsm_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
-    sample_arena_destroy(ctx);
+    frame_pool_destroy(ctx);
     delete ctx;
     return 0;
 }

@@ -228,14 +228,14 @@ struct SliceSet {
 }  // namespace bsm
 
 namespace bsm {
-struct SampleArena;   // pinned + device slots of every sampler of a context
+struct FramePool;     // pooled pinned host blocks and events for sampler frames
 struct SamplePlan;    // device descriptors of the samplers one launch serves
 }  // namespace bsm
 
 struct bsm_ctx {
     int device;
     hipDeviceProp_t prop;
-    bsm::SampleArena* samples = nullptr;
+    bsm::FramePool* frames = nullptr;
     std::atomic<uint64_t> sampler_gen{0};   // bumped by every sampler create/destroy
 };
 
@@ -324,7 +324,7 @@ int sample_after_cycle(SamplePlan*& plan, hipStream_t s, bsm_area* const* areas,
 void sample_plan_free(SamplePlan*& plan);
 // Closes the area's samplers (their frames end) before the area goes away.
 void samplers_detach(bsm_area* a);
-void sample_arena_destroy(bsm_ctx* c);
+void frame_pool_destroy(bsm_ctx* c);
 int ensure_rnd(bsm_area* a, size_t n_ints);
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids);

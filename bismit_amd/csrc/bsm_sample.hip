@@ -3,13 +3,13 @@
 // The reference enqueues one async buffer read per sampler per cycle, ordered
 // after the kernels that write the buffer, into a single-buffer tract channel
 // (CorticalArea::cycle_samplers, src/cortex/cortical_area.rs:808-991;
-// TractInner::send/recv, src/subcortex/tract_channel.rs:381-480).  Here every
-// sampler owns a slot in a per-context arena of matched device and pinned host
-// chunks.  After a cycle, one gather kernel copies the ranges of all samplers
-// due this cycle into their device slots -- resolving the state parity of
-// double-buffered buffers and composing the previous-active flag bit on the
-// device -- and one copy per contiguous run of slots moves them to the host.
-// A shared event marks the frames complete; a receiver waits for it only.
+// TractInner::send/recv, src/subcortex/tract_channel.rs:381-480).  Here, after a
+// cycle, one gather kernel packs the ranges of every sampler due this cycle into
+// a staging buffer -- resolving the state parity of double-buffered buffers and
+// composing the previous-active flag bit on the device -- and one copy moves the
+// packed frames into a pinned host block.  The block, with the event that marks
+// it complete, is shared by the frames it holds and returns to a per-context
+// pool when the last of them has been received or dropped.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
@@ -24,13 +24,12 @@ namespace bsm {
 
 constexpr uint32_t SAMPLE_MAX = 256;        // samplers per gather launch
 constexpr uint32_t SAMPLE_SEG = 65536;      // bytes per gather block
-constexpr size_t SAMPLE_ALIGN = 256;
-constexpr size_t SAMPLE_CHUNK = 8u << 20;   // default arena chunk
+constexpr size_t SAMPLE_ALIGN = 256;        // frame alignment in the staging buffer
+constexpr size_t BLOCK_MIN = 64u << 10;     // smallest pooled host block
 
 struct SampleDesc {
     const uint8_t* src[2];      // by layer parity (equal when not double-buffered)
     const uint8_t* prev[2];     // BSM_SYN_FLAG_SETS: previous states by parity, else null
-    uint8_t* dst;               // device slot
     const uint32_t* par_word;   // dynamic batches: parity word on the device, else null
     uint64_t len;
 };
@@ -39,11 +38,13 @@ struct SampleArgs {
     uint32_t n;
     uint32_t par_bits[SAMPLE_MAX / 32];   // host parity of launch entry j
     uint16_t idx[SAMPLE_MAX];             // descriptor of launch entry j
+    uint32_t dst[SAMPLE_MAX];             // staging offset of entry j, in SAMPLE_ALIGN units
 };
 
-// Blocks (entry j, part): copy part `part` of entry j's range into its slot.
+// Blocks (entry j, part): copy part `part` of entry j's range into the staging
+// buffer at the entry's offset.
 __global__ __launch_bounds__(256) void k_sample(const SampleDesc* __restrict__ desc,
-                                                SampleArgs args) {
+                                                uint8_t* __restrict__ stage, SampleArgs args) {
     const uint32_t j = blockIdx.x;
     const SampleDesc d = desc[args.idx[j]];
     const uint64_t lo = (uint64_t)blockIdx.y * SAMPLE_SEG;
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(256) void k_sample(const SampleDesc* __restrict__ d
                                     : ((args.par_bits[j >> 5] >> (j & 31u)) & 1u);
     const uint32_t n = (uint32_t)min((uint64_t)SAMPLE_SEG, d.len - lo);
     const uint8_t* src = d.src[par] + lo;
-    uint8_t* dst = d.dst + lo;
+    uint8_t* dst = stage + (uint64_t)args.dst[j] * SAMPLE_ALIGN + lo;
     if (!d.prev[0]) {
         block_copy(dst, src, n, threadIdx.x, blockDim.x);
         return;
@@ -64,36 +65,36 @@ __global__ __launch_bounds__(256) void k_sample(const SampleDesc* __restrict__ d
         d8[i] = (uint8_t)((s8[i] & ~SYN_PREV_ACTIVE) | (p8[i] ? SYN_PREV_ACTIVE : 0));
 }
 
-struct SampleChunk {
-    uint8_t* dev = nullptr;
-    uint8_t* host = nullptr;
-    size_t size = 0;
-    std::map<size_t, size_t> free_;   // offset -> bytes
-};
-
-struct SampleArena {
+struct FramePool {
     std::mutex m;
-    std::vector<SampleChunk> chunks;
-    std::vector<hipEvent_t> ev_pool;
+    std::map<size_t, std::vector<uint8_t*>> free_;   // capacity -> pinned host blocks
+    std::vector<hipEvent_t> events;
 };
 
-// Completion of one cycle's readback, shared by the frames it carries.
-struct FrameEvent {
-    SampleArena* arena;
+// One cycle's packed frames in pinned host memory, and their completion.
+struct FrameBlock {
+    FramePool* pool;
+    uint8_t* host;
+    size_t cap;
     hipEvent_t ev;
-    ~FrameEvent() {
-        std::lock_guard<std::mutex> g(arena->m);
-        arena->ev_pool.push_back(ev);
+    ~FrameBlock() {
+        // the copy into the block must have ended before another cycle reuses it
+        (void)hipEventSynchronize(ev);
+        std::lock_guard<std::mutex> g(pool->m);
+        pool->free_[cap].push_back(host);
+        pool->events.push_back(ev);
     }
 };
 
 struct SamplePlan {
-    std::vector<bsm_sampler*> smp;        // arena order
+    std::vector<bsm_sampler*> smp;
     std::vector<uint32_t> area_of;        // index into the areas of the launch
     std::vector<std::pair<bsm_area*, uint64_t>> key;   // areas and their sampler generations
     const uint32_t* dyn = nullptr;
     SampleDesc* desc_d = nullptr;
     size_t desc_cap = 0;
+    uint8_t* stage_d = nullptr;           // packed frames of one cycle
+    size_t stage_cap = 0;
 };
 
 enum SampleState { SMP_EMPTY = 0, SMP_FILLING = 1, SMP_PENDING = 2 };
@@ -104,106 +105,79 @@ struct bsm_sampler {
     bsm_area* area;
     int buf;
     uint32_t layer;
-    uint64_t offset, len;
-    uint32_t chunk;
-    size_t off, bytes;
-    uint8_t* dev_slot;    // this sampler's device and pinned host slots
-    uint8_t* host_slot;
+    uint64_t offset, len, bytes;          // bytes: len rounded up to the staging alignment
     std::mutex m;
     std::condition_variable cv;
     int state = bsm::SMP_EMPTY;
     bool backpressure = false;
     bool closed = false;
-    bool reading = false;   // a receiver is copying the frame out of the host slot
     uint64_t frame_cycle = 0;
-    std::shared_ptr<bsm::FrameEvent> ev;
+    std::shared_ptr<bsm::FrameBlock> blk;   // the block holding the pending frame
+    size_t blk_off = 0;
 };
 
 namespace bsm {
 
-static int arena_alloc(bsm_ctx* c, size_t bytes, uint32_t* chunk, size_t* off, uint8_t** dev,
-                       uint8_t** host) {
-    if (!c->samples) c->samples = new SampleArena();
-    SampleArena& A = *c->samples;
-    std::lock_guard<std::mutex> g(A.m);
-    for (uint32_t k = 0; k < A.chunks.size(); k++) {
-        auto& fl = A.chunks[k].free_;
-        for (auto it = fl.begin(); it != fl.end(); ++it) {
-            if (it->second < bytes) continue;
-            *chunk = k;
-            *off = it->first;
-            const size_t o = it->first + bytes, rest = it->second - bytes;
-            fl.erase(it);
-            if (rest) fl[o] = rest;
-            *dev = A.chunks[k].dev + *off;
-            *host = A.chunks[k].host + *off;
-            return 0;
-        }
-    }
-    SampleChunk ch;
-    ch.size = std::max(SAMPLE_CHUNK, (bytes + (2u << 20) - 1) & ~(size_t)((2u << 20) - 1));
-    BSM_HIP(hipMalloc((void**)&ch.dev, ch.size));
-    hipError_t e = hipHostMalloc((void**)&ch.host, ch.size, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        (void)hipFree(ch.dev);
-        return hip_fail(e, "hipHostMalloc");
-    }
-    if (ch.size > bytes) ch.free_[bytes] = ch.size - bytes;
-    *dev = ch.dev;
-    *host = ch.host;
-    A.chunks.push_back(std::move(ch));
-    *chunk = (uint32_t)(A.chunks.size() - 1);
-    *off = 0;
-    return 0;
+static FramePool* pool_of(bsm_ctx* c) {
+    static std::mutex init;
+    std::lock_guard<std::mutex> g(init);
+    if (!c->frames) c->frames = new FramePool();
+    return c->frames;
 }
 
-static void arena_free(bsm_ctx* c, uint32_t chunk, size_t off, size_t bytes) {
-    SampleArena& A = *c->samples;
-    std::lock_guard<std::mutex> g(A.m);
-    auto& fl = A.chunks[chunk].free_;
-    auto it = fl.emplace(off, bytes).first;
-    auto nx = std::next(it);
-    if (nx != fl.end() && it->first + it->second == nx->first) {   // merge with the next hole
-        it->second += nx->second;
-        fl.erase(nx);
-    }
-    if (it != fl.begin()) {                                        // and with the previous one
-        auto pv = std::prev(it);
-        if (pv->first + pv->second == it->first) {
-            pv->second += it->second;
-            fl.erase(it);
-        }
-    }
-}
-
-static int frame_event(bsm_ctx* c, hipStream_t s, std::shared_ptr<FrameEvent>* out) {
-    SampleArena& A = *c->samples;
+static int frame_block(bsm_ctx* c, size_t bytes, std::shared_ptr<FrameBlock>* out) {
+    FramePool* P = pool_of(c);
+    size_t cap = BLOCK_MIN;
+    while (cap < bytes) cap <<= 1;
+    uint8_t* host = nullptr;
     hipEvent_t ev = nullptr;
     {
-        std::lock_guard<std::mutex> g(A.m);
-        if (!A.ev_pool.empty()) {
-            ev = A.ev_pool.back();
-            A.ev_pool.pop_back();
+        std::lock_guard<std::mutex> g(P->m);
+        auto& v = P->free_[cap];
+        if (!v.empty()) {
+            host = v.back();
+            v.pop_back();
+        }
+        if (!P->events.empty()) {
+            ev = P->events.back();
+            P->events.pop_back();
         }
     }
-    if (!ev) BSM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    auto fe = std::make_shared<FrameEvent>();
-    fe->arena = &A;
-    fe->ev = ev;
-    BSM_HIP(hipEventRecord(ev, s));
-    *out = std::move(fe);
+    if (!host) {
+        hipError_t e = hipHostMalloc((void**)&host, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            if (ev) {
+                std::lock_guard<std::mutex> g(P->m);
+                P->events.push_back(ev);
+            }
+            return hip_fail(e, "hipHostMalloc (sampler frames)");
+        }
+    }
+    if (!ev) {
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> g(P->m);
+            P->free_[cap].push_back(host);
+            return hip_fail(e, "hipEventCreateWithFlags");
+        }
+    }
+    auto b = std::make_shared<FrameBlock>();
+    b->pool = P;
+    b->host = host;
+    b->cap = cap;
+    b->ev = ev;
+    *out = std::move(b);
     return 0;
 }
 
-static int make_desc(bsm_sampler* sp, const uint32_t* dyn_area, uint8_t* slot, SampleDesc* d) {
+static int make_desc(bsm_sampler* sp, const uint32_t* dyn_area, SampleDesc* d) {
     bsm_area* a = sp->area;
     uint8_t* p;
     size_t n;
     if (resolve_buf(a, sp->buf, sp->layer, &p, &n)) return 1;
     *d = SampleDesc{};
-    d->dst = slot;
     d->len = sp->len;
-    d->par_word = (dyn_area && sp->buf != BSM_AXN_STATES) ? dyn_area + sp->layer : nullptr;
+    d->par_word = dyn_area ? dyn_area + sp->layer : nullptr;
     if (sp->buf == BSM_SYN_STATES) {
         const LayerRt& L = a->L[sp->layer];
         d->src[0] = L.syn_state[0] + sp->offset;
@@ -234,29 +208,17 @@ static int plan_build(SamplePlan*& P, bsm_area* const* areas, size_t n, const ui
     P->area_of.clear();
     P->key.clear();
     P->dyn = dyn;
+    size_t total = 0;
     for (size_t i = 0; i < n; i++) {
         P->key.emplace_back(areas[i], areas[i]->sampler_gen);
         for (bsm_sampler* sp : areas[i]->samplers) {
             P->smp.push_back(sp);
             P->area_of.push_back((uint32_t)i);
+            total += sp->bytes;
         }
     }
-    // arena order, so that the frames of consecutive samplers form one copy
-    std::vector<uint32_t> ord(P->smp.size());
-    for (uint32_t k = 0; k < ord.size(); k++) ord[k] = k;
-    std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
-        const bsm_sampler *a = P->smp[x], *b = P->smp[y];
-        return a->chunk != b->chunk ? a->chunk < b->chunk : a->off < b->off;
-    });
-    std::vector<bsm_sampler*> smp(ord.size());
-    std::vector<uint32_t> area_of(ord.size());
-    for (size_t k = 0; k < ord.size(); k++) {
-        smp[k] = P->smp[ord[k]];
-        area_of[k] = P->area_of[ord[k]];
-    }
-    P->smp.swap(smp);
-    P->area_of.swap(area_of);
     if (P->smp.size() > 65535) return fail("too many samplers in one launch set");
+    if (total / SAMPLE_ALIGN > 0xFFFFFFFFull) return fail("sampled bytes exceed the staging range");
     if (P->smp.empty()) return 0;
     if (P->desc_cap < P->smp.size()) {
         if (P->desc_d) BSM_HIP(hipFree(P->desc_d));
@@ -264,11 +226,16 @@ static int plan_build(SamplePlan*& P, bsm_area* const* areas, size_t n, const ui
         P->desc_cap = std::max<size_t>(P->smp.size(), 64);
         BSM_HIP(hipMalloc((void**)&P->desc_d, P->desc_cap * sizeof(SampleDesc)));
     }
+    if (P->stage_cap < total) {
+        if (P->stage_d) BSM_HIP(hipFree(P->stage_d));
+        P->stage_d = nullptr;
+        P->stage_cap = std::max(total, BLOCK_MIN);
+        BSM_HIP(hipMalloc((void**)&P->stage_d, P->stage_cap));
+    }
     std::vector<SampleDesc> h(P->smp.size());
     for (size_t k = 0; k < h.size(); k++) {
-        bsm_sampler* sp = P->smp[k];
         const uint32_t* da = dyn ? dyn + (size_t)P->area_of[k] * BSM_MAX_LAYERS : nullptr;
-        if (make_desc(sp, da, sp->dev_slot, &h[k])) return 1;
+        if (make_desc(P->smp[k], da, &h[k])) return 1;
     }
     BSM_HIP(hipMemcpy(P->desc_d, h.data(), h.size() * sizeof(SampleDesc), hipMemcpyHostToDevice));
     return 0;
@@ -286,6 +253,8 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
     if (plan_stale(P, areas, n, dyn) && plan_build(P, areas, n, dyn)) return 1;
     // which samplers send this cycle (TractInner::send)
     std::vector<uint32_t> inc;
+    std::vector<size_t> at;   // staging offset of each included frame
+    size_t total = 0;
     for (uint32_t k = 0; k < P->smp.size(); k++) {
         const uint32_t i = P->area_of[k];
         if (active && !active[i]) continue;
@@ -300,6 +269,8 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
         sp->state = SMP_FILLING;
         sp->frame_cycle = areas[i]->cycles_done;
         inc.push_back(k);
+        at.push_back(total);
+        total += sp->bytes;
     }
     if (inc.empty()) return 0;
     auto reset = [&] {
@@ -310,7 +281,6 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
             sp->cv.notify_all();
         }
     };
-    bsm_ctx* c = P->smp[inc[0]]->area->ctx;
     for (size_t g0 = 0; g0 < inc.size(); g0 += SAMPLE_MAX) {
         SampleArgs args{};
         args.n = (uint32_t)std::min<size_t>(SAMPLE_MAX, inc.size() - g0);
@@ -319,45 +289,35 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
             const uint32_t k = inc[g0 + j];
             const bsm_sampler* sp = P->smp[k];
             args.idx[j] = (uint16_t)k;
+            args.dst[j] = (uint32_t)(at[g0 + j] / SAMPLE_ALIGN);
             const uint32_t par = sp->buf == BSM_AXN_STATES
                                      ? 0u : (uint32_t)(areas[P->area_of[k]]->L[sp->layer].cur & 1);
             args.par_bits[j >> 5] |= par << (j & 31u);
             parts = std::max<uint64_t>(parts, (sp->len + SAMPLE_SEG - 1) / SAMPLE_SEG);
         }
-        k_sample<<<dim3(args.n, (uint32_t)parts), 256, 0, s>>>(P->desc_d, args);
+        k_sample<<<dim3(args.n, (uint32_t)parts), 256, 0, s>>>(P->desc_d, P->stage_d, args);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
             reset();
             return hip_fail(e, "k_sample");
         }
     }
-    // one copy per run of adjacent slots (inc is in arena order)
-    for (size_t r = 0; r < inc.size();) {
-        const bsm_sampler* first = P->smp[inc[r]];
-        size_t end = first->off + first->bytes, q = r + 1;
-        while (q < inc.size()) {
-            const bsm_sampler* nx = P->smp[inc[q]];
-            if (nx->chunk != first->chunk || nx->off != end) break;
-            end = nx->off + nx->bytes;
-            q++;
-        }
-        hipError_t e = hipMemcpyAsync(first->host_slot, first->dev_slot, end - first->off,
-                                      hipMemcpyDeviceToHost, s);
-        if (e != hipSuccess) {
-            reset();
-            return hip_fail(e, "hipMemcpyAsync (sampler frames)");
-        }
-        r = q;
-    }
-    std::shared_ptr<FrameEvent> fe;
-    if (frame_event(c, s, &fe)) {
+    std::shared_ptr<FrameBlock> blk;
+    if (frame_block(areas[0]->ctx, total, &blk)) {
         reset();
         return 1;
     }
-    for (uint32_t k : inc) {
-        bsm_sampler* sp = P->smp[k];
+    hipError_t e = hipMemcpyAsync(blk->host, P->stage_d, total, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(blk->ev, s);
+    if (e != hipSuccess) {
+        reset();
+        return hip_fail(e, "sampler frame copy");
+    }
+    for (size_t q = 0; q < inc.size(); q++) {
+        bsm_sampler* sp = P->smp[inc[q]];
         std::lock_guard<std::mutex> g(sp->m);
-        sp->ev = fe;
+        sp->blk = blk;
+        sp->blk_off = at[q];
         sp->state = SMP_PENDING;
         sp->cv.notify_all();
     }
@@ -367,25 +327,22 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
 void sample_plan_free(SamplePlan*& P) {
     if (!P) return;
     if (P->desc_d) (void)hipFree(P->desc_d);
+    if (P->stage_d) (void)hipFree(P->stage_d);
     delete P;
     P = nullptr;
 }
 
-// Ends the sampler's frames: waits for a frame in flight, frees its slot.
+// Ends the sampler's frames; a receiver copying one keeps its block alive.
 static void sampler_close(bsm_sampler* sp) {
-    std::shared_ptr<FrameEvent> ev;
     {
-        std::unique_lock<std::mutex> lk(sp->m);
+        std::lock_guard<std::mutex> g(sp->m);
         if (sp->closed) return;
         sp->closed = true;
-        ev = std::move(sp->ev);
         sp->state = SMP_EMPTY;   // a pending frame ends unreceived
+        sp->blk.reset();
         sp->cv.notify_all();
-        sp->cv.wait(lk, [&] { return !sp->reading; });   // the slot is freed below
     }
-    if (ev) (void)hipEventSynchronize(ev->ev);
     bsm_area* a = sp->area;
-    arena_free(a->ctx, sp->chunk, sp->off, sp->bytes);
     auto& v = a->samplers;
     v.erase(std::remove(v.begin(), v.end(), sp), v.end());
     a->sampler_gen = ++a->ctx->sampler_gen;
@@ -396,15 +353,13 @@ void samplers_detach(bsm_area* a) {
     sample_plan_free(a->sample_plan);
 }
 
-void sample_arena_destroy(bsm_ctx* c) {
-    if (!c->samples) return;
-    for (auto& ch : c->samples->chunks) {
-        (void)hipFree(ch.dev);
-        (void)hipHostFree(ch.host);
-    }
-    for (hipEvent_t e : c->samples->ev_pool) (void)hipEventDestroy(e);
-    delete c->samples;
-    c->samples = nullptr;
+void frame_pool_destroy(bsm_ctx* c) {
+    if (!c->frames) return;
+    for (auto& kv : c->frames->free_)
+        for (uint8_t* h : kv.second) (void)hipHostFree(h);
+    for (hipEvent_t e : c->frames->events) (void)hipEventDestroy(e);
+    delete c->frames;
+    c->frames = nullptr;
 }
 
 }  // namespace bsm
@@ -422,7 +377,6 @@ int bsm_sampler_create(bsm_area* a, int buf, uint32_t layer, uint64_t offset, ui
     if (len == 0) return fail("empty sampler range");
     if (offset > n || len > n - offset) return fail("sampler range out of the buffer");
     if ((len + SAMPLE_SEG - 1) / SAMPLE_SEG > 65535) return fail("sampler range too long");
-    BSM_HIP(hipSetDevice(a->ctx->device));
     bsm_sampler* sp = new bsm_sampler();
     sp->area = a;
     sp->buf = buf;
@@ -431,10 +385,6 @@ int bsm_sampler_create(bsm_area* a, int buf, uint32_t layer, uint64_t offset, ui
     sp->len = len;
     sp->bytes = (len + SAMPLE_ALIGN - 1) & ~(uint64_t)(SAMPLE_ALIGN - 1);
     sp->backpressure = backpressure != 0;
-    if (arena_alloc(a->ctx, sp->bytes, &sp->chunk, &sp->off, &sp->dev_slot, &sp->host_slot)) {
-        delete sp;
-        return 1;
-    }
     a->samplers.push_back(sp);
     a->sampler_gen = ++a->ctx->sampler_gen;
     *out = sp;
@@ -446,34 +396,32 @@ int bsm_sampler_recv(bsm_sampler* sp, int wait_for_frame, void* dst, uint64_t ds
     if (!sp || !got) return fail("null arg");
     *got = 0;
     if (dst_len < sp->len || !dst) return fail("destination shorter than the sampler range");
-    std::shared_ptr<FrameEvent> ev;
+    std::shared_ptr<FrameBlock> blk;
+    size_t off;
     uint64_t fc;
-    const uint8_t* host;
     {
         std::unique_lock<std::mutex> lk(sp->m);
-        if (sp->closed) return 0;   // its slot is gone with the area
+        if (sp->closed) return 0;
         if (sp->state != SMP_PENDING) {
             if (!wait_for_frame) return 0;
             sp->cv.wait(lk, [&] { return sp->state == SMP_PENDING || sp->closed; });
             if (sp->closed || sp->state != SMP_PENDING) return 0;
         }
-        ev = sp->ev;
+        blk = sp->blk;
+        off = sp->blk_off;
         fc = sp->frame_cycle;
-        host = sp->host_slot;
-        sp->reading = true;
     }
-    hipError_t e = hipEventSynchronize(ev->ev);
-    if (e == hipSuccess) std::memcpy(dst, host, sp->len);
+    hipError_t e = hipEventSynchronize(blk->ev);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize (sampler frame)");
+    std::memcpy(dst, blk->host + off, sp->len);
     {
         std::lock_guard<std::mutex> g(sp->m);
-        sp->reading = false;
-        if (e == hipSuccess && !sp->closed) {
+        if (!sp->closed) {
             sp->state = SMP_EMPTY;
-            sp->ev.reset();
+            sp->blk.reset();
         }
         sp->cv.notify_all();
     }
-    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize (sampler frame)");
     if (cycle) *cycle = fc;
     *got = 1;
     return 0;
@@ -509,10 +457,7 @@ int bsm_sampler_info(bsm_sampler* sp, uint64_t* len, int* backpressure, int* pen
 
 int bsm_sampler_destroy(bsm_sampler* sp) {
     if (!sp) return 0;
-    if (!sp->closed) {
-        (void)hipSetDevice(sp->area->ctx->device);
-        sampler_close(sp);
-    }
+    if (!sp->closed) sampler_close(sp);
     delete sp;
     return 0;
 }

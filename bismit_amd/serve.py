@@ -337,9 +337,12 @@ class Server:
             self.batch.cycle_masked(mask, frames, ofs, graph=self.use_graph)
             if self.regrow_every:
                 self._regrow_due(active_set)
-            self.batch.sync()
             if self.sample_outputs:
+                # the frames' event follows the cycle on the batch stream, so
+                # receiving them is the wait for the cycle
                 self._deliver(active_sessions)
+            else:
+                self.batch.sync()
             t1 = clock()
             st.gpu_busy_s += t1 - t0
             for s in active_sessions:

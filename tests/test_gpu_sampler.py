@@ -224,3 +224,49 @@ def test_sampler_closes_with_its_area(ctx):
     a.close()
     assert rx.recv(True) is None      # closed: no frame, no wait
     rx.close()
+
+
+def test_flywheel_serves_a_sampler(ctx):
+    """The request loop (flywheel) on its own thread: a Sampler request returns a
+    receiver; with backpressure every cycle of an Iterate(6) run arrives, equal
+    to a twin area read after each cycle."""
+    import queue
+
+    from bismit_amd.cortex import Cortex
+    from bismit_amd.flywheel import Command, Flywheel, Request
+
+    scheme = recipes.config1_area()
+    n = 6
+    fr = glyphs(scheme, n, 21)
+    twin = CorticalArea(scheme, ctx, rnd_seed=40, layout_seed=41)
+    want = []
+    for c in range(n):
+        twin.intake(fr[c])
+        twin.cycle(1)
+        want.append(twin.read("AXN_STATES"))
+    twin.close()
+    cx = Cortex([scheme], ctx, seed=40)          # area 0: rnd_seed 40, layout_seed 41
+    feed = iter(fr)
+    cmd, req, res = queue.Queue(), queue.Queue(), queue.Queue()
+    fw = Flywheel(cx, cmd, scheme.name, inputs={scheme.name: lambda: next(feed)})
+    fw.add_req_res_pair(req, res)
+    req.put(Request.sampler(scheme.name, SamplerKind.Axons(None), None, True))
+    t = threading.Thread(target=fw.spin, daemon=True)
+    t.start()
+    r = res.get(timeout=60)
+    assert r.kind == "Sampler"
+    rx = r.value
+    cmd.put(Command.iterate(n))
+    got = [rx.recv(True) for _ in range(n)]
+    while True:
+        r = res.get(timeout=60)
+        if r.kind == "Status" and not r.value.cycling:
+            break
+    cmd.put(Command.exit())
+    t.join(60)
+    assert not t.is_alive()
+    assert [f.cycle for f in got] == list(range(1, n + 1))
+    for c in range(n):
+        assert np.array_equal(got[c].data, want[c]), c
+    rx.close()
+    cx.close()
