@@ -34,6 +34,7 @@ BUF_NAMES = [
     "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS", "SYN_FLAG_SETS",
 ]
 BUF = {n: i for i, n in enumerate(BUF_NAMES)}
+FILTER_RETINA = 1
 SIGNED_BUFS = {"SYN_STRENGTHS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS"}
 
 
@@ -104,6 +105,13 @@ SIGNATURES = [
     ("bsm_area_slice_idz", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("bsm_area_buf_len", C.c_int, [_P, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)]),
     ("bsm_area_read", C.c_int, [_P, C.c_int, C.c_uint32, C.c_size_t, C.c_size_t, _P]),
+    ("bsm_area_intake_filtered", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P, C.c_int]),
+    ("bsm_k_retina", C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
+    ("bsm_area_set_smoother", C.c_int, [_P, C.c_uint32, C.c_int32]),
+    ("bsm_k_smooth", C.c_int, [_P, C.c_uint32]),
+    ("bsm_smoother_centers", C.c_int, [C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.c_uint32,
+                                       C.POINTER(C.c_uint32)]),
     ("bsm_sampler_create", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
                                      C.POINTER(_P)]),
     ("bsm_sampler_recv", C.c_int, [_P, C.c_int, _P, C.c_uint64, C.POINTER(C.c_uint64),
@@ -240,3 +248,17 @@ def gen_tuft_layout(seed, layer, tuft, n_syns, src_ids, reach):
     check(load().bsm_gen_tuft_layout(C.c_uint64(seed), layer, tuft, C.c_uint64(n_syns), len(src),
                                      u8ptr(src), reach, vptr(slc), vptr(v), vptr(u)))
     return slc, v, u
+
+
+def smoother_centers(radius, v_size, u_size):
+    """(v, u, group_counts) of the engine's smoother center groups (host-only)."""
+    lib = load()
+    counts = (C.c_uint32 * 6)()
+    n = C.c_uint32()
+    check(lib.bsm_smoother_centers(radius, v_size, u_size, None, None, counts, 0, C.byref(n)))
+    v = np.empty(n.value, np.int32)
+    u = np.empty(n.value, np.int32)
+    check(lib.bsm_smoother_centers(radius, v_size, u_size, v.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   u.ctypes.data_as(C.POINTER(C.c_int32)), counts, n.value,
+                                   C.byref(n)))
+    return v, u, list(counts)

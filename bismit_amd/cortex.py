@@ -91,6 +91,9 @@ class CorticalArea:
         self.layer_names = [l.name for l in scheme.layers]
         if init_layout:
             self.init_layout(layout_seed, reach)
+        for li, lay in enumerate(scheme.layers):
+            if lay.smoother_radius:
+                self.set_smoother(li, lay.smoother_radius)
 
     # ------------------------------------------------------------- layout
     def init_layout(self, seed=0, reach=6):
@@ -135,16 +138,45 @@ class CorticalArea:
 
     # ------------------------------------------------------------- intake
     def intake(self, frame, offset=None):
-        """Write a host frame into the input slices (AxonSpace::intake)."""
+        """Write a host frame into the input slices (AxonSpace::intake); through
+        the scheme's sensory filter when it names one and no offset is given."""
+        frame = np.ascontiguousarray(frame, dtype=np.uint8)
+        if offset is None and self.scheme.input_filter:
+            if frame.size != self.scheme.input_range()[1]:
+                raise ValueError("a filtered frame covers the input slices exactly")
+            self._filtered(frame.ctypes.data, 0)
+            return
         if offset is None:
             offset = self.scheme.input_range()[0]
-        frame = np.ascontiguousarray(frame, dtype=np.uint8)
         L.check(self.lib.bsm_area_intake(self.h, offset, L.vptr(frame), frame.size, 0))
 
     def intake_device(self, dptr, length, offset=None):
+        if offset is None and self.scheme.input_filter:
+            if length != self.scheme.input_range()[1]:
+                raise ValueError("a filtered frame covers the input slices exactly")
+            self._filtered(dptr.value if isinstance(dptr, C.c_void_p) else dptr, 1)
+            return
         if offset is None:
             offset = self.scheme.input_range()[0]
         L.check(self.lib.bsm_area_intake(self.h, offset, dptr, length, 1))
+
+    def _filtered(self, src, on_device):
+        if self.scheme.input_filter != "retina":
+            raise ValueError(f"unknown sensory filter {self.scheme.input_filter!r}")
+        s0 = min(self.scheme.input_slcs)
+        L.check(self.lib.bsm_area_intake_filtered(self.h, L.FILTER_RETINA, s0,
+                                                  len(self.scheme.input_slcs), src, on_device))
+
+    # ------------------------------------------------------------- filter / smoother
+    def set_smoother(self, layer, radius):
+        """Attach an ActivitySmoother control layer (radius 0 removes it)."""
+        L.check(self.lib.bsm_area_set_smoother(self.h, self.layer_index(layer), radius))
+
+    def k_smooth(self, layer):
+        L.check(self.lib.bsm_k_smooth(self.h, self.layer_index(layer)))
+
+    def k_retina(self, slc, depth, dptr):
+        L.check(self.lib.bsm_k_retina(self.h, slc, depth, dptr))
 
     # ------------------------------------------------------------- cycle
     def draw_rnd(self, n_cycles):
@@ -273,6 +305,8 @@ class Batch:
     dynamic=True: each cycle steps only the areas of an active mask (serving)."""
 
     def __init__(self, areas, ctx: Context = None, dynamic=False):
+        if any(a.scheme.input_filter for a in areas):
+            raise ValueError("areas with a sensory filter take their input in the plain form")
         self.ctx = ctx or areas[0].ctx
         self.lib = self.ctx.lib
         self.areas = list(areas)

@@ -638,6 +638,8 @@ int bsm_area_destroy(bsm_area* a) {
     (void)hipSetDevice(a->ctx->device);
     (void)hipStreamSynchronize(a->stream);
     samplers_detach(a);
+    for (uint32_t l = 0; l < a->n_layers; l++) smoother_free(a->L[l]);
+    (void)hipFree(a->filt_stage);
     (void)hipFree(a->arena);
     (void)hipFree(a->rnd_d);
     (void)hipFree(a->tjobs_d);

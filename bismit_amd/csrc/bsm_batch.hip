@@ -425,6 +425,9 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
             if (areas[k] == areas[i]) return fail("duplicate area in batch");
         if ((areas[i]->axn_len + COPY_SEG - 1) / COPY_SEG > 65535)
             return fail("axon space too large for the batch prologue");
+        for (uint32_t l = 0; l < areas[i]->n_layers; l++)
+            if (areas[i]->L[l].smooth_n)
+                return fail("areas with an activity smoother cycle in the plain form");
     }
     BSM_HIP(hipSetDevice(ctx->device));
     bsm_batch* b = new bsm_batch();

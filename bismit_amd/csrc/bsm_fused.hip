@@ -1194,6 +1194,16 @@ int build_plan(bsm_area* a) {
             }
             s2.layer = l;
             a->stages.push_back(s2);
+            if (L.smooth_n) {
+                // the layer's activity smoother (a control layer run after its
+                // inhibition); it reads activities F2 writes and writes energies
+                // only the next cycle's F1 reads
+                Stage s4{};
+                s4.kind = 4;
+                s4.layer = l;
+                s4.n_items = 1;
+                a->stages.push_back(s4);
+            }
         }
     }
     if (side_prepare(a->side, a->stages)) return 1;
@@ -1314,9 +1324,7 @@ int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial) {
     for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
     if (enqueue_stages(a->stream, a->side, a->stages, serial,
                        [&](const Stage& st, hipStream_t s) {
-                           return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d,
-                                               a->items_d, (uint32_t)a->L[st.layer].cur,
-                                               rnd_ofs);
+                           return launch_area_stage(a, s, st, rnd_ofs);
                        }))
         return 1;
     // readback of the area's samplers, after the cycle on its stream
@@ -1483,11 +1491,8 @@ int bsm_area_profile_stages(bsm_area* a, uint32_t n_cycles, uint32_t max_stages,
         for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
         for (uint32_t i = 0; i < ns; i++) {
             const Stage& st = a->stages[i];
-            LayerRt& L = a->L[st.layer];
             BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i)], a->stream));
-            if (launch_stage(a->stream, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
-                             (uint32_t)L.cur, c * a->n_slots))
-                return 1;
+            if (launch_area_stage(a, a->stream, st, c * a->n_slots)) return 1;
             BSM_HIP(hipEventRecord(ev[2 * ((size_t)c * ns + i) + 1], a->stream));
         }
         if (sample_after_cycle(a->sample_plan, a->stream, &a, 1, nullptr, nullptr)) return 1;

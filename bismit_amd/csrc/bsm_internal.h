@@ -140,6 +140,13 @@ struct LayerRt {
     uint8_t* tft[8];
     uint8_t *soma, *cel_flags, *cel_energies, *cel_act;
     uint32_t slot0;          // first rnd slot of this layer
+    // activity smoother hosted by this layer: hex-group centers (6 groups), the
+    // center covering each column in each group, per-center selections
+    int32_t smooth_r;
+    uint32_t smooth_n;
+    int32_t* smooth_centers;   // (v, u) per center, group-major
+    uint32_t* smooth_cover;    // SMOOTH_GROUPS x (v_size * u_size) center indices
+    uint32_t* smooth_sel;      // (least active, most active, its activity) per center
 };
 
 struct Tuning {
@@ -150,7 +157,7 @@ struct Tuning {
 
 // One launch of a fused stage over a contiguous range of work items.
 struct Stage {
-    int kind;            // 1 = F1 tufts, 2 = F2 ssc post, 3 = F3 pyr post
+    int kind;            // 1 = F1 tufts, 2 = F2 ssc post, 3 = F3 pyr post, 4 = smoother
     uint32_t item0, n_items;
     uint32_t lds_bytes;
     uint32_t layer;      // plain form: layer whose state parity flips at F1
@@ -281,6 +288,8 @@ struct bsm_area {
     uint64_t sampler_gen;
     bsm::SamplePlan* sample_plan;
     uint64_t cycles_done;
+    uint8_t* filt_stage;       // filtered intake: device copy of a host frame
+    size_t filt_cap;
 };
 
 namespace bsm {
@@ -325,6 +334,12 @@ void sample_plan_free(SamplePlan*& plan);
 // Closes the area's samplers (their frames end) before the area goes away.
 void samplers_detach(bsm_area* a);
 void frame_pool_destroy(bsm_ctx* c);
+// Activity smoother and sensory filter (bsm_filters.hip).
+constexpr uint32_t SMOOTH_GROUPS = 6;
+int launch_smoother(bsm_area* a, uint32_t layer, hipStream_t s);
+void smoother_free(LayerRt& L);
+// A stage of the area's plain plan on stream s (smoother stages included).
+int launch_area_stage(bsm_area* a, hipStream_t s, const Stage& st, uint32_t rnd_ofs);
 int ensure_rnd(bsm_area* a, size_t n_ints);
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids);

@@ -92,6 +92,7 @@ class LayerScheme:
     tufts: List[TuftScheme]
     control: int = L.CTRL_NONE
     inhib_radius: int = 4
+    smoother_radius: int = 0    # ActivitySmoother control layer (0: none)
 
     @property
     def spatial(self):
@@ -107,6 +108,7 @@ class AreaScheme:
     layers: List[LayerScheme]
     settings: int = 0
     input_slcs: List[int] = field(default_factory=list)
+    input_filter: Optional[str] = None   # "retina": intake through the sensory filter
 
     # ---------------------------------------------------------------- sizes
     def slice_idzs(self):

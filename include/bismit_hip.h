@@ -182,6 +182,36 @@ int bsm_area_intake(bsm_area* area, uint32_t axn_offset, const uint8_t* src, uin
                     int src_is_device);
 int bsm_area_output(bsm_area* area, uint32_t axn_offset, uint8_t* host_dst, uint32_t len);
 
+/* ---- sensory filter and activity smoother ---- */
+enum bsm_filter_kind { BSM_FILTER_RETINA = 1 };
+/* Intake through a sensory filter, replacing SensoryFilter::write + cycle with
+ * the `retina` filter (src/cortex/sensory_filter.rs:24-174, cl/filters.cl:34-100)
+ * on a filter chain of one: the frame (depth slices of the dims of axon slice
+ * slc, slice-major) is filtered into axon slices [slc, slc + depth), which must
+ * share their dimensions.  Enqueued on the area stream (a host frame is first
+ * copied to a device staging buffer). */
+int bsm_area_intake_filtered(bsm_area* area, int filter, uint32_t slc, uint32_t depth,
+                             const uint8_t* src, int src_is_device);
+/* The retina kernel alone on a device frame (cl/filters.cl:34-100). */
+int bsm_k_retina(bsm_area* area, uint32_t slc, uint32_t depth, const uint8_t* dev_src);
+/* Attaches an ActivitySmoother control layer with field radius `radius` to
+ * `layer` (0 removes it), replacing src/cortex/smoother.rs:62-180 and
+ * cl/control.cl:227-295.  It runs after the layer's inhibition/learning stage in
+ * every cycle of the plain form -- as the reference does, whose cycle counter
+ * never advances (src/cortex/smoother.rs:162) -- reading the layer's soma
+ * activities and adjusting its soma energies.  The six center groups apply in
+ * start order (the reference launches them together and races where their
+ * fields meet).  Areas with a smoother cycle in the plain form only. */
+int bsm_area_set_smoother(bsm_area* area, uint32_t layer, int32_t radius);
+/* The smoother kernel alone (cl/control.cl:227-295, groups in order). */
+int bsm_k_smooth(bsm_area* area, uint32_t layer);
+/* Centers of the smoother's six hex groups for a v_size x u_size layer
+ * (gen_grp_centers + HexGroupCenters::populate, src/cortex/smoother.rs:33-58,
+ * src/cmn/mod.rs:807-943), group-major, row-major within a group; *n is the
+ * total, written only when it fits cap.  Host-only. */
+int bsm_smoother_centers(int32_t radius, uint32_t v_size, uint32_t u_size, int32_t* v,
+                         int32_t* u, uint32_t* group_counts, uint32_t cap, uint32_t* n);
+
 /* ---- samplers: per-cycle readback (replaces CorticalArea::sampler and
  *      cycle_samplers, src/cortex/cortical_area.rs:808-1284, with the single-buffer
  *      tract channel of src/subcortex/tract_channel.rs:369-480) ----

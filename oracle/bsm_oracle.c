@@ -113,6 +113,10 @@ typedef struct orc_layer {
     uint8_t *tft_prev_best_ids, *tft_prev_best_raw, *tft_prev_best, *tft_prev_states;
     /* cells (src/cortex/pyramidals.rs:66-70, src/cortex/spiny_stellates.rs:61-62) */
     uint8_t *soma, *cel_flags, *cel_energies, *cel_activities;
+    /* activity smoother hosted by the layer: radius, centers (v, u), per group */
+    int32_t smooth_r;
+    uint32_t smooth_n, smooth_counts[6];
+    int32_t* smooth_c;
 } orc_layer;
 
 struct orc_area {
@@ -190,6 +194,7 @@ void orc_area_destroy(orc_area* a) {
                       L->tft_prev_best, L->tft_prev_states, L->soma, L->cel_flags,
                       L->cel_energies, L->cel_activities};
         for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
+        free(L->smooth_c);
     }
     free(a);
 }
@@ -689,6 +694,7 @@ int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles) {
                 e |= cycle_dens(a, l, r);
                 e |= orc_k_ssc_cycle(a, l);
                 e |= run_control(a, l, r);
+                if (a->L[l].smooth_n) e |= orc_k_smooth(a, l);
             }
             for (uint32_t l = 0; l < a->n_layers && learn; l++) {
                 if (!a->L[l].d.spatial) continue;
@@ -703,6 +709,7 @@ int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles) {
                 for (uint32_t t = 0; t < L->d.n_tufts; t++) e |= orc_k_tft_cycle(a, l, t);
                 e |= orc_k_pyr_cycle(a, l);
                 e |= run_control(a, l, r);
+                if (L->smooth_n) e |= orc_k_smooth(a, l);
                 if (learn) {
                     int s0 = layer_slot0(a, l) + L->d.n_tufts + 1;
                     for (uint32_t t = 0; t < L->d.n_tufts; t++)
@@ -808,6 +815,176 @@ int orc_area_regrow(orc_area* a, uint64_t seed, uint64_t epoch, int32_t reach) {
                 }
             }
         }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------ sensory filter */
+
+/* retina (cl/filters.cl:34-100) with get_neighbors_avg (:8-31): per cell, bit 0
+ * of the cell index selects off-center, bit 1 xor bit 0 the large field (radius
+ * 5, else 3); the mean runs over the whole hex field, off-grid cells adding 0
+ * (cel_state_3d_safe, cl/bismit.cl:285-297) but counting; RETNAL_THRESHOLD = 48
+ * (cl/bismit.cl:19). */
+int orc_retina(orc_area* a, uint32_t slc, uint32_t depth, const uint8_t* frame) {
+    if (!depth || slc >= a->n_slices || depth > a->n_slices - slc) return 1;
+    const int32_t V = a->slc[slc].v_size, U = a->slc[slc].u_size;
+    for (uint32_t k = 1; k < depth; k++)
+        if ((int32_t)a->slc[slc + k].v_size != V || (int32_t)a->slc[slc + k].u_size != U) return 1;
+    for (uint32_t s = 0; s < depth; s++) {
+        const uint8_t* cs = frame + (size_t)s * V * U;
+        uint8_t* out = a->axn + a->idz[slc + s];
+        for (int32_t v = 0; v < V; v++) {
+            for (int32_t u = 0; u < U; u++) {
+                const uint32_t cel_idx = s * (uint32_t)(V * U) + (uint32_t)(v * U + u);
+                const int32_t one_of_two = (int32_t)(cel_idx & 1);
+                const int32_t two_of_four = (int32_t)((cel_idx & 3) >> 1) ^ one_of_two;
+                const int32_t radius = two_of_four ? 5 : 3;
+                int32_t sum = 0, count = 0;
+                for (int32_t vo = -radius; vo <= radius; vo++) {
+                    const int32_t uz = -vo - radius > -radius ? -vo - radius : -radius;
+                    const int32_t um = -vo + radius < radius ? -vo + radius : radius;
+                    for (int32_t uo = uz; uo <= um; uo++) {
+                        const int32_t vv = v + vo, uu = u + uo;
+                        if (vv >= 0 && vv < V && uu >= 0 && uu < U) sum += cs[vv * U + uu];
+                        count++;
+                    }
+                }
+                const int32_t avg = sum / count;
+                const int32_t center = cs[v * U + u];
+                int32_t st;
+                if (one_of_two) {
+                    const int32_t cst = center + 48;
+                    st = avg > cst ? avg - cst : 0;
+                } else {
+                    const int32_t nat = avg + 48;
+                    st = center > nat ? center - nat : 0;
+                }
+                out[v * U + u] = (uint8_t)st;
+            }
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ smoother */
+
+/* gen_grp_centers (src/cortex/smoother.rs:33-58): six groups seeded at
+ * (0, d), (-d, d), (-d, 0), (0, -d), (d, -d), (d, 0) with d = (r + 1) / 2; each
+ * group is the set HexGroupCenters::populate (src/cmn/mod.rs:892-921) reaches
+ * from its seed through add_surrounds' six steps (:867-888, clockwise bias:
+ * l = r + 1, s = r) inside [-r, dims + r) (add_center, :852-864).  The set is
+ * what is fixed; the reference's HashSet order is not, so groups list their
+ * centers row-major. */
+int orc_smoother_centers(int32_t r, int32_t V, int32_t U, int32_t* ov, int32_t* ou,
+                         uint32_t* counts, uint32_t cap, uint32_t* n) {
+    if (r < 1 || V < 1 || U < 1) return 1;
+    const int32_t l = r + 1, s = r, ls = l + s, d = (r + 1) / 2;
+    const int32_t seeds[6][2] = {{0, d}, {-d, d}, {-d, 0}, {0, -d}, {d, -d}, {d, 0}};
+    const int32_t sur[6][2] = {{-s, ls}, {l, s}, {ls, -l}, {s, -ls}, {-l, -s}, {-ls, l}};
+    const int32_t W = V + 2 * r, H = U + 2 * r;
+    uint8_t* in_set = (uint8_t*)malloc((size_t)W * H);
+    int32_t* q = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)W * H);
+    uint32_t tot = 0;
+    for (int g = 0; g < 6; g++) {
+        memset(in_set, 0, (size_t)W * H);
+        size_t qn = 0, qi = 0;
+        int32_t sv = seeds[g][0], su = seeds[g][1];
+        if (sv >= -r && su >= -r && sv < V + r && su < U + r) {
+            in_set[(size_t)(sv + r) * H + (su + r)] = 1;
+            q[2 * qn] = sv;
+            q[2 * qn + 1] = su;
+            qn++;
+        }
+        while (qi < qn) {   /* breadth-first over new centers */
+            const int32_t cv = q[2 * qi], cu = q[2 * qi + 1];
+            qi++;
+            for (int k = 0; k < 6; k++) {
+                const int32_t nv = cv + sur[k][0], nu = cu + sur[k][1];
+                if (nv < -r || nu < -r || nv >= V + r || nu >= U + r) continue;
+                uint8_t* m = &in_set[(size_t)(nv + r) * H + (nu + r)];
+                if (*m) continue;
+                *m = 1;
+                q[2 * qn] = nv;
+                q[2 * qn + 1] = nu;
+                qn++;
+            }
+        }
+        counts[g] = (uint32_t)qn;
+        for (int32_t i = 0; i < W; i++)
+            for (int32_t j = 0; j < H; j++)
+                if (in_set[(size_t)i * H + j]) {
+                    if (tot < cap) {
+                        if (ov) ov[tot] = i - r;
+                        if (ou) ou[tot] = j - r;
+                    }
+                    tot++;
+                }
+    }
+    free(in_set);
+    free(q);
+    *n = tot;
+    return 0;
+}
+
+int orc_set_smoother(orc_area* a, uint32_t layer, int32_t radius) {
+    if (layer >= a->n_layers || radius < 0) return 1;
+    orc_layer* L = &a->L[layer];
+    free(L->smooth_c);
+    L->smooth_c = NULL;
+    L->smooth_n = 0;
+    L->smooth_r = 0;
+    if (!radius) return 0;
+    uint32_t n = 0;
+    if (orc_smoother_centers(radius, a->v_size, a->u_size, NULL, NULL, L->smooth_counts, 0, &n))
+        return 1;
+    int32_t* v = (int32_t*)malloc(sizeof(int32_t) * n);
+    int32_t* u = (int32_t*)malloc(sizeof(int32_t) * n);
+    orc_smoother_centers(radius, a->v_size, a->u_size, v, u, L->smooth_counts, n, &n);
+    L->smooth_c = (int32_t*)malloc(sizeof(int32_t) * 2 * n);
+    for (uint32_t i = 0; i < n; i++) {
+        L->smooth_c[2 * i] = v[i];
+        L->smooth_c[2 * i + 1] = u[i];
+    }
+    free(v);
+    free(u);
+    L->smooth_n = n;
+    L->smooth_r = radius;
+    return 0;
+}
+
+/* smooth_activity (cl/control.cl:227-295), bound as in ActivitySmoother::new
+ * (src/cortex/smoother.rs:97-109: the host layer's dims, depth, activities and
+ * energies).  One work-item's body per center, centers of a group in any order
+ * (their fields are disjoint), groups in seed order. */
+int orc_k_smooth(orc_area* a, uint32_t layer) {
+    if (layer >= a->n_layers) return 1;
+    orc_layer* L = &a->L[layer];
+    const int32_t V = a->v_size, U = a->u_size, r = L->smooth_r;
+    for (uint32_t c = 0; c < L->smooth_n; c++) {
+        const int32_t cv = L->smooth_c[2 * c], cu = L->smooth_c[2 * c + 1];
+        uint32_t least_idx = 0, most_idx = 0;
+        uint8_t least_actv = 255, most_actv = 0;
+        for (uint32_t slc = 0; slc < L->d.depth; slc++) {
+            for (int32_t vo = -r; vo <= r; vo++) {
+                const int32_t uz = -vo - r > -r ? -vo - r : -r;
+                const int32_t um = -vo + r < r ? -vo + r : r;
+                for (int32_t uo = uz; uo <= um; uo++) {
+                    const int32_t vv = cv + vo, uu = cu + uo;
+                    const int safe = vv >= 0 && vv < V && uu >= 0 && uu < U;
+                    if (!safe) continue;
+                    const uint32_t idx = slc * (uint32_t)(V * U) + (uint32_t)(vv * U + uu);
+                    const uint8_t x = L->cel_activities[idx];
+                    if (x <= least_actv) { least_idx = idx; least_actv = x; }
+                    if (x >= most_actv) { most_idx = idx; most_actv = x; }
+                }
+            }
+        }
+        const int differ = least_idx != most_idx;
+        const uint8_t le = L->cel_energies[least_idx];
+        L->cel_energies[least_idx] = (uint8_t)(le + ((le < 255) & (most_actv < 255) & differ));
+        const uint8_t me = L->cel_energies[most_idx];
+        L->cel_energies[most_idx] = (uint8_t)(me - ((me > 0) & (most_actv > 0) & differ));
     }
     return 0;
 }

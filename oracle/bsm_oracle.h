@@ -78,6 +78,21 @@ int orc_area_cycle(orc_area* a, const int32_t* rnd, uint32_t n_cycles);
  * mix64 is the splitmix64 output function (Steele, Lea, Flood 2014). */
 int orc_area_regrow(orc_area* a, uint64_t seed, uint64_t epoch, int32_t reach);
 
+/* retina filter (cl/filters.cl:34-100): frame of `depth` slices of the dims of
+ * axon slice slc, slice-major, into axon slices [slc, slc + depth). */
+int orc_retina(orc_area* a, uint32_t slc, uint32_t depth, const uint8_t* frame);
+/* gen_grp_centers (src/cortex/smoother.rs:33-58) with HexGroupCenters
+ * (src/cmn/mod.rs:807-943): the six groups' centers, group-major, each group in
+ * row-major order; *n = total (written only when it fits cap). */
+int orc_smoother_centers(int32_t radius, int32_t v_size, int32_t u_size, int32_t* v, int32_t* u,
+                         uint32_t* group_counts, uint32_t cap, uint32_t* n);
+/* Attaches an ActivitySmoother of field radius `radius` to `layer` (0 removes);
+ * orc_area_cycle then runs it after the layer's control kernel every cycle. */
+int orc_set_smoother(orc_area* a, uint32_t layer, int32_t radius);
+/* smooth_activity (cl/control.cl:227-295), one center at a time, group by group
+ * in start order. */
+int orc_k_smooth(orc_area* a, uint32_t layer);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,8 +66,29 @@ def load():
         lib.orc_k_ssc_mtp.argtypes = [P, C.c_uint32, C.c_uint32]
         lib.orc_area_cycle.argtypes = [P, C.POINTER(C.c_int32), C.c_uint32]
         lib.orc_area_regrow.argtypes = [P, C.c_uint64, C.c_uint64, C.c_int32]
+        lib.orc_retina.argtypes = [P, C.c_uint32, C.c_uint32, P]
+        lib.orc_smoother_centers.argtypes = [C.c_int32, C.c_int32, C.c_int32,
+                                             C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_uint32), C.c_uint32,
+                                             C.POINTER(C.c_uint32)]
+        lib.orc_set_smoother.argtypes = [P, C.c_uint32, C.c_int32]
+        lib.orc_k_smooth.argtypes = [P, C.c_uint32]
         _lib = lib
     return _lib
+
+
+def smoother_centers(radius, v_size, u_size):
+    """(v, u, group_counts) of the smoother's six center groups."""
+    lib = load()
+    counts = (C.c_uint32 * 6)()
+    n = C.c_uint32()
+    if lib.orc_smoother_centers(radius, v_size, u_size, None, None, counts, 0, C.byref(n)):
+        raise RuntimeError("orc_smoother_centers failed")
+    v = np.empty(n.value, np.int32)
+    u = np.empty(n.value, np.int32)
+    lib.orc_smoother_centers(radius, v_size, u_size, v.ctypes.data_as(C.POINTER(C.c_int32)),
+                             u.ctypes.data_as(C.POINTER(C.c_int32)), counts, n.value, C.byref(n))
+    return v, u, list(counts)
 
 
 def set_threads(n):
@@ -86,6 +107,18 @@ class OracleArea:
         self.lib.orc_rnd_slots(self.h, C.byref(n))
         self.n_slots = n.value
         self.layer_names = [l.name for l in scheme.layers]
+        for li, lay in enumerate(scheme.layers):
+            if getattr(lay, "smoother_radius", 0):
+                self.set_smoother(li, lay.smoother_radius)
+
+    def set_smoother(self, layer, radius):
+        if self.lib.orc_set_smoother(self.h, self.li(layer), radius):
+            raise RuntimeError("orc_set_smoother failed")
+
+    def retina(self, slc, depth, frame):
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        if self.lib.orc_retina(self.h, slc, depth, f.ctypes.data):
+            raise RuntimeError("orc_retina failed")
 
     def li(self, layer):
         return self.layer_names.index(layer) if isinstance(layer, str) else int(layer)
@@ -111,6 +144,10 @@ class OracleArea:
         return b
 
     def intake(self, frame, offset=None):
+        if offset is None and getattr(self.scheme, "input_filter", None) == "retina":
+            s0 = min(self.scheme.input_slcs)
+            self.retina(s0, len(self.scheme.input_slcs), frame)
+            return
         if offset is None:
             offset = self.scheme.input_range()[0]
         f = np.asarray(frame, dtype=np.uint8)

@@ -538,3 +538,178 @@ def test_regrow_touches_only_weak_synapses_and_is_deterministic():
     assert changed_any                                # another epoch draws other sources
     for o in (a, b, c):
         o.close()
+
+
+# ------------------------------------------------------------------ smoother
+@pytest.mark.parametrize("r,V,U", [(4, 64, 64), (2, 17, 23), (6, 40, 31), (1, 9, 12)])
+def test_smoother_groups_tile_every_cell_once(r, V, U):
+    """The reference's smoother_overlap invariant (src/tests/hex_tile_fields.rs:
+    140-180): with every field cell counted, each cell is hit exactly once per
+    group, i.e. six times per cycle.  Brute force over the centers' hex fields."""
+    v, u, counts = P.smoother_centers(r, V, U)
+    assert len(counts) == 6 and sum(counts) == v.size
+    hits = np.zeros((6, V, U), np.int32)
+    c0 = 0
+    for g in range(6):
+        for k in range(c0, c0 + counts[g]):
+            for dv in range(-r, r + 1):
+                for du in range(max(-r, -dv - r), min(r, -dv + r) + 1):
+                    vv, uu = v[k] + dv, u[k] + du
+                    if 0 <= vv < V and 0 <= uu < U:
+                        hits[g, vv, uu] += 1
+        c0 += counts[g]
+    assert np.all(hits == 1)
+    assert np.all(hits.sum(axis=0) == 6)
+    # every center lies inside the reference's bounds [-r, dims + r)
+    assert v.min() >= -r and u.min() >= -r and v.max() < V + r and u.max() < U + r
+    # the engine's generator gives the same centers
+    ve, ue, ce = L.smoother_centers(r, V, U)
+    assert np.array_equal(v, ve) and np.array_equal(u, ue) and ce == counts
+
+
+def _smooth_area(depth=1, side=16, r=3):
+    s = recipes.cortical_area("sm", side, ssc_depth=depth, ssc_dens=2, ssc_spd=4, pyr_dens=2,
+                              pyr_spd=4)
+    o = P.OracleArea(s)
+    o.set_smoother(0, r)
+    return s, o
+
+
+def test_smoother_equal_activities_change_nothing():
+    """All activities equal: least and most are the same (last) cell of every
+    field, so no energy moves (cl/control.cl:280-291)."""
+    s, o = _smooth_area()
+    o.write("SOMA_ACTIVITIES", 0, np.full(s.cells(0), 77, np.uint8))
+    e = np.random.default_rng(0).integers(0, 256, s.cells(0)).astype(np.uint8)
+    o.write("SOMA_ENERGIES", 0, e)
+    o.k("smooth", 0)
+    assert np.array_equal(o.read("SOMA_ENERGIES", 0), e)
+    o.close()
+
+
+def test_smoother_moves_one_unit_per_group_and_conserves():
+    """Distinct activities: each center adds one to its field's least active cell
+    and takes one from its most active (energies away from 0 and 255), so the
+    total is conserved and a cell changes by at most six."""
+    s, o = _smooth_area(depth=2, side=20, r=2)
+    n = s.cells(0)
+    rng = np.random.default_rng(1)
+    act = rng.permutation(np.arange(n) % 250).astype(np.uint8) + 1   # 1..250: never 0 or 255
+    o.write("SOMA_ACTIVITIES", 0, act)
+    o.write("SOMA_ENERGIES", 0, np.full(n, 100, np.uint8))
+    o.k("smooth", 0)
+    e = o.read("SOMA_ENERGIES", 0).astype(np.int32)
+    assert e.sum() == 100 * n
+    assert np.abs(e - 100).max() <= 6 and np.count_nonzero(e != 100) > 0
+    # the global extremes are always their fields' pick: they move in all six groups
+    lo = np.flatnonzero(act == act.min())[-1]
+    hi = np.flatnonzero(act == act.max())[-1]
+    assert e[lo] == 106 and e[hi] == 94
+    o.close()
+
+
+def test_smoother_ties_go_to_the_later_cell_and_saturate():
+    s, o = _smooth_area(side=12, r=1)
+    V = U = 12
+    n = s.cells(0)
+    act = np.full(n, 100, np.uint8)
+    # one field of group 0 gets two minima and two maxima
+    v, u, counts = P.smoother_centers(1, V, U)
+    k = next(i for i in range(counts[0]) if 2 <= v[i] < V - 2 and 2 <= u[i] < U - 2)
+    cv, cu = int(v[k]), int(u[k])
+    field = [(cv + dv) * U + (cu + du) for dv in (-1, 0, 1)
+             for du in range(max(-1, -dv - 1), min(1, -dv + 1) + 1)]
+    act[field[0]] = act[field[3]] = 5        # two least active, field order 0 < 3
+    act[field[1]] = act[field[5]] = 200      # two most active, 1 < 5
+    o.write("SOMA_ACTIVITIES", 0, act)
+    en = np.full(n, 50, np.uint8)
+    o.write("SOMA_ENERGIES", 0, en)
+    o.k("smooth", 0)
+    e = o.read("SOMA_ENERGIES", 0)
+    assert e[field[3]] > 50 and e[field[5]] < 50          # later ones picked
+    assert e[field[0]] <= 50 + 5 and e[field[1]] >= 50 - 5
+    # saturation: at 255 the least active cell cannot rise, at 0 the most cannot fall
+    en = np.full(n, 50, np.uint8)
+    en[field[3]] = 255
+    en[field[5]] = 0
+    o.write("SOMA_ACTIVITIES", 0, act)
+    o.write("SOMA_ENERGIES", 0, en)
+    o.k("smooth", 0)
+    e = o.read("SOMA_ENERGIES", 0)
+    assert e[field[3]] == 255 and e[field[5]] == 0
+    o.close()
+
+
+# ------------------------------------------------------------------ retina
+def _retina_area(side=16, depth=1):
+    s = recipes.cortical_area("rt", 32, input_side=side, ssc_dens=2, ssc_spd=4, pyr_dens=2,
+                              pyr_spd=4)
+    return s, P.OracleArea(s)
+
+
+def _retina(o, s, frame):
+    s0 = min(s.input_slcs)
+    o.retina(s0, 1, frame)
+    off, n = s.input_range()
+    return o.read("AXN_STATES")[off:off + n]
+
+
+def test_retina_hand_computed_cases():
+    """cl/filters.cl:34-100 by hand: cell index bits pick the cell type."""
+    side = 16
+    s, o = _retina_area(side)
+    # impulse at on-center small (idx & 3 == 0): 255 - (255 // 37 + 48) = 201
+    for idx, want in [(5 * side + 4, 201),       # idx & 3 == 0: small on-center
+                      (5 * side + 6, 205),       # idx & 3 == 2: large on-center, 255 // 91 = 2
+                      (5 * side + 5, 0)]:        # odd: off-center, a bright center is never on
+        f = np.zeros(side * side, np.uint8)
+        f[idx] = 255
+        out = _retina(o, s, f)
+        assert out[idx] == want, idx
+        assert np.count_nonzero(out) == (1 if want else 0)
+    # dark off-center cells in a bright surround
+    for c, want in [(6 * side + 7, 200),          # idx & 3 == 3: small, (36 * 255) // 37 - 48
+                    (6 * side + 5, 204)]:         # idx & 3 == 1: large, (90 * 255) // 91 - 48
+        f = np.full(side * side, 255, np.uint8)
+        f[c] = 0
+        assert _retina(o, s, f)[c] == want, c
+    # uniform field: interior cells see mean == center, nothing passes the threshold
+    f = np.full(side * side, 180, np.uint8)
+    out = _retina(o, s, f).reshape(side, side)
+    assert np.all(out[5:-5, 5:-5] == 0)
+    # corner of a uniform 255 field: 10 of the 37 field cells are on the grid,
+    # off-grid ones count as zeros: 255 - ((10 * 255) // 37 + 48) = 139
+    f = np.full(side * side, 255, np.uint8)
+    assert _retina(o, s, f)[0] == 139
+    o.close()
+
+
+def test_retina_against_a_convolution():
+    """Hex-field sums by 2-D convolution (scipy), an independent formulation of
+    the neighbourhood; cell types from the index bits."""
+    from scipy.signal import convolve2d
+    side = 24
+    s, o = _retina_area(side)
+    rng = np.random.default_rng(4)
+    f = (rng.random(side * side) < 0.3).astype(np.uint8) * rng.integers(0, 256, side * side,
+                                                                         dtype=np.uint8)
+    out = _retina(o, s, f)
+    img = f.reshape(side, side).astype(np.int64)
+
+    def field_sum(r):
+        m = np.zeros((2 * r + 1, 2 * r + 1), np.int64)
+        for dv in range(-r, r + 1):
+            for du in range(-r, r + 1):
+                if max(abs(dv), abs(du), abs(dv + du)) <= r:
+                    m[dv + r, du + r] = 1
+        return convolve2d(img, m[::-1, ::-1], mode="same"), int(m.sum())
+
+    s3, n3 = field_sum(3)
+    s5, n5 = field_sum(5)
+    idx = np.arange(side * side).reshape(side, side)
+    off = (idx & 1) == 1
+    large = (((idx & 3) >> 1) ^ (idx & 1)) == 1
+    avg = np.where(large, s5 // n5, s3 // n3)
+    want = np.where(off, np.maximum(avg - (img + 48), 0), np.maximum(img - (avg + 48), 0))
+    assert np.array_equal(out, want.ravel().astype(np.uint8))
+    o.close()
