@@ -24,24 +24,34 @@ HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
 ALG_BYTES_PER_SYNAPSE = 8
 ALG_BASIS = ("8 B per synapse-update: 3 B source metadata + 1 B strength + 1 B previous "
              "state + 1 B new state + 2 B flags read and written (dendrite/tuft bytes excluded)")
-ENGINE_BASIS = ("bytes this engine must move: 5 B per synapse (metadata, strength, new state; "
-                "the previous-active flag is derived from the previous state buffer) + 4 B per "
-                "dendrite + 12 B per pyramidal cell-tuft")
+def engine_basis(syn_b):
+    return (f"bytes this engine must move: {syn_b} B per synapse on this launch (source slice "
+            "id when several slices are staged, offsets packed into 1 B when |v|,|u| <= 7, "
+            "strength, new state; the previous-active flag is derived from the previous state "
+            "buffer) + 4 B per dendrite + 12 B per pyramidal cell-tuft")
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
-def stage_bytes(scheme, li, kind):
+def syn_bytes(stage):
+    """Per-synapse HBM bytes of an F1 launch: the slice id unless a single staged
+    slice makes it implicit, the offsets (1 B packed, else 2 B), strength in and
+    state out."""
+    slc = 0 if (stage["fast"] and stage["staged"] <= 1) else 1
+    return slc + (1 if stage["packed"] else 2) + 2
+
+
+def stage_bytes(scheme, li, kind, syn_b):
     """HBM bytes this engine moves in one fused stage launch on layer li:
-    F1: per synapse 3 B metadata + 1 B strength read, 1 B state write; per
-        dendrite raw/state writes + activity read/write (4 B); per cell-tuft 4 B
-        read + 8 B written (pyramidal), per SSC cell 1 B energy read.
+    F1: per synapse syn_b (syn_bytes); per dendrite raw/state writes + activity
+        read/write (4 B); per cell-tuft 4 B read + 8 B written (pyramidal), per
+        SSC cell 1 B energy read.
     F2/F3 are reported but not used for the roofline."""
     lay = scheme.layers[li]
     C = scheme.cells(li)
     if kind == 1:
         b = 0
         for t in lay.tufts:
-            b += C * t.syns_per_tft * 5 + C * t.dens_per_tft * 4
+            b += C * t.syns_per_tft * syn_b + C * t.dens_per_tft * 4
             if lay.kind == 2:
                 b += C * 12
         if lay.kind == 1:
@@ -178,8 +188,8 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True, serial=False):
     for (k, t), p in zip(prof, plan):
         assert k == p["kind"]
         st = {"kernel": kinds[k], "stream": p["stream"], "avg_us": round(1000.0 * t / n_prof, 2),
-              "chunk": p["chunk"], "fast": p["fast"], "tile": [p["tile_v"], p["tile_u"]],
-              "items": p["n_items"]}
+              "chunk": p["chunk"], "fast": p["fast"], "packed": p["packed"],
+              "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"]}
         if k == 1:
             st["engine_bytes_per_launch"] = p["alg_bytes"]
             st["hbm_frac_engine_bytes"] = round(p["alg_bytes"] / (t / n_prof / 1000.0) / 1e9 /
@@ -310,12 +320,13 @@ def main():
     plan = area.stage_plan()
     stages = [{"kernel": kinds[k], "layer": scheme.layers[li].name, "stream": p["stream"],
                "avg_us": round(1000.0 * t / n_prof, 2), "chunk": p["chunk"], "fast": p["fast"],
+               "packed": p["packed"],
                "tile": [p["tile_v"], p["tile_u"]], "items": p["n_items"],
                "engine_bytes_per_launch": p["alg_bytes"]}
               for (k, li, t), p in zip(prof, plan)]
     groups = {}
     for (k, li, t), p in zip(prof, plan):
-        g = groups.setdefault((k, li), [0.0, 0, 0])
+        g = groups.setdefault((k, li), [0.0, 0, 0, p])
         g[0] += t
         g[1] += p["alg_bytes"]
         g[2] += 1
@@ -334,19 +345,23 @@ def main():
             "cycles": n_host,
             "how": "host frame per cycle through bsm_area_intake (synchronous copy), then "
                    "bsm_area_cycle with host-drawn rnd; Python host loop"}
-    (dk, dl), (dt, alg_tot, n_launch) = max(groups.items(), key=lambda kv: kv[1][0])
-    assert alg_tot == stage_bytes(scheme, dl, dk)   # the engine's count equals the recipe's
+    (dk, dl), (dt, alg_tot, n_launch, dstage) = max(groups.items(), key=lambda kv: kv[1][0])
+    syn_b = syn_bytes(dstage)
+    # the engine's count equals the recipe's
+    assert alg_tot == stage_bytes(scheme, dl, dk, syn_b), (alg_tot, syn_b)
     avg_s = dt / n_prof / n_launch / 1000.0
     # algorithmic bytes: ALG_BYTES_PER_SYNAPSE per synapse-update of the launch
     syn_launch = scheme.cells(dl) * sum(t.syns_per_tft for t in scheme.layers[dl].tufts) / n_launch
     alg = ALG_BYTES_PER_SYNAPSE * syn_launch
     achieved = alg / avg_s / 1e9
-    # the bytes this engine has to move for the same launch (5 B per synapse,
+    # the bytes this engine has to move for the same launch (syn_b per synapse,
     # 4 B per dendrite, 12 B per pyramidal cell-tuft)
     moved = alg_tot / n_launch
     moved_gbs = moved / avg_s / 1e9
     traffic = read_traffic()
-    ceil = stream_ceiling(ctx)
+    # the calibration kernel with the launch's read streams (syn_b - 1) and its
+    # one write stream
+    ceil = stream_ceiling(ctx, n_reads=syn_b - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
@@ -356,7 +371,7 @@ def main():
                 "engine_bytes": {"per_launch": moved, "achieved": round(moved_gbs, 1),
                                  "frac": round(moved_gbs / HBM_PEAK_GBS, 4),
                                  "frac_of_stream_ceiling": round(moved_gbs / ceil["gbs"], 4),
-                                 "basis": ENGINE_BASIS},
+                                 "basis": engine_basis(syn_b)},
                 "stream_ceiling": ceil}
 
     out = {
@@ -395,21 +410,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stream_ceiling(ctx, n=128 << 20, reps=5):
-    """Best of `reps` launches of the stream calibration kernel (F1's four read
-    streams and one write stream, no gather): a practical HBM ceiling."""
+def stream_ceiling(ctx, n_reads=4, n=128 << 20, reps=5):
+    """Best of `reps` launches of the stream calibration kernel (F1's read
+    streams and its one write stream, no gather): a practical HBM ceiling."""
     import ctypes as C
     from bismit_amd import _lib as L
-    src, dst = ctx.alloc(4 * n), ctx.alloc(n)
+    src, dst = ctx.alloc(n_reads * n), ctx.alloc(n)
     best = None
     for _ in range(reps):
         ms = C.c_float()
-        L.check(ctx.lib.bsm_calib_stream(ctx.h, dst, src, n, C.byref(ms)))
+        L.check(ctx.lib.bsm_calib_stream(ctx.h, dst, src, n, n_reads, C.byref(ms)))
         best = ms.value if best is None else min(best, ms.value)
     ctx.free(src)
     ctx.free(dst)
-    return {"gbs": round(5 * n / (best / 1000.0) / 1e9, 1),
-            "pattern": f"4 non-temporal 16-B read streams + 1 16-B write stream, {n >> 20} MiB each"}
+    return {"gbs": round((n_reads + 1) * n / (best / 1000.0) / 1e9, 1),
+            "pattern": f"{n_reads} non-temporal 16-B read streams + 1 16-B write stream, "
+                       f"{n >> 20} MiB each"}
 
 
 def C_float_copy(ctx, dst, src, n):

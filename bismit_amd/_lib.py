@@ -77,7 +77,8 @@ class StageInfo(C.Structure):
     _fields_ = [("kind", C.c_int32), ("layer", C.c_uint32), ("chunk", C.c_uint32),
                 ("staged", C.c_uint32), ("fast", C.c_uint32), ("tile_v", C.c_uint32),
                 ("tile_u", C.c_uint32), ("n_items", C.c_uint32), ("lds_bytes", C.c_uint32),
-                ("grid", C.c_uint32), ("stream", C.c_uint32), ("alg_bytes", C.c_uint64)]
+                ("grid", C.c_uint32), ("stream", C.c_uint32), ("alg_bytes", C.c_uint64),
+                ("packed", C.c_uint32), ("_reserved", C.c_uint32)]
 
 
 def stage_dicts(arr, n):
@@ -156,7 +157,7 @@ SIGNATURES = [
                                           C.POINTER(C.c_int), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint32)]),
     ("bsm_calib_copy", C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_float)]),
-    ("bsm_calib_stream", C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(C.c_float)]),
+    ("bsm_calib_stream", C.c_int, [_P, _P, _P, C.c_size_t, C.c_uint32, C.POINTER(C.c_float)]),
     ("bsm_gen_tuft_layout", C.c_int, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64,
                                       C.c_uint32, _U8P, C.c_int32, _P, _P, _P]),
     ("bsm_area_init_layout", C.c_int, [_P, C.c_uint64, C.c_int32]),

@@ -152,6 +152,7 @@ static void plan_area(bsm_area* a, ArenaPlan& ap) {
         ap.add(&L.syn_slc, ns);
         ap.add(&L.syn_v, ns);
         ap.add(&L.syn_u, ns);
+        ap.add(&L.syn_vu, ns);
         ap.add(&L.syn_str, ns);
         ap.add(&L.den_states, nd);
         ap.add(&L.den_raw, nd);
@@ -213,6 +214,26 @@ __global__ void k_layout(uint64_t key, uint64_t n, uint32_t n_src, LayoutSrc src
         v[i] = (int8_t)((int32_t)((uint32_t)((h >> 16) & 0xFFFF) % span) - reach);
         u[i] = (int8_t)((int32_t)((uint32_t)((h >> 32) & 0xFFFF) % span) - reach);
     }
+}
+
+// Packed offsets for the gather: one byte (v + 8) << 4 | (u + 8) per synapse,
+// exact when |v|, |u| <= 7 (the planner reads them only then).
+__global__ __launch_bounds__(256) void k_pack_vu(const int8_t* __restrict__ v,
+                                                 const int8_t* __restrict__ u,
+                                                 uint8_t* __restrict__ vu, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        vu[i] = (uint8_t)((((v[i] + 8) & 15) << 4) | ((u[i] + 8) & 15));
+}
+
+int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len) {
+    LayerRt& L = a->L[layer];
+    if (offset >= L.n_syn || !len) return 0;
+    len = std::min(len, (size_t)L.n_syn - offset);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((len + 255) / 256, 8192);
+    k_pack_vu<<<grid, 256, 0, s>>>(L.syn_v + offset, L.syn_u + offset, L.syn_vu + offset, len);
+    BSM_HIP(hipGetLastError());
+    return 0;
 }
 
 int launch_layout(hipStream_t s, uint64_t seed, uint32_t layer, uint32_t tuft, uint64_t n,
@@ -460,6 +481,7 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
                                               (int8_t*)L.syn_u + T.syn_idz,
                                               (int8_t*)L.syn_str + T.syn_idz);
                 BSM_HIP(hipGetLastError());
+                if (launch_pack_vu(s, a, l, T.syn_idz, (size_t)L.cells * T.spt)) return 1;
             }
             // the source summary widens to the regrown sources; the fused plan
             // stays exact while the summary is unchanged
@@ -603,8 +625,11 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
         *p.first = a->arena + off;
         off += align_up(p.second ? p.second : 1, 256);
     }
-    // Buffers start zero-filled like Buffer::builder().fill_val(0).
+    // Buffers start zero-filled like Buffer::builder().fill_val(0); the packed
+    // offsets of v = u = 0 are 0x88.
     BSM_HIP(hipMemsetAsync(a->arena, 0, ap.total, a->stream));
+    for (uint32_t l = 0; l < a->n_layers; l++)
+        if (a->L[l].n_syn) BSM_HIP(hipMemsetAsync(a->L[l].syn_vu, 0x88, a->L[l].n_syn, a->stream));
     BSM_HIP(hipMemcpyAsync(a->slices_d, a->slices_h.data(), sizeof(SliceRt) * a->n_slices,
                            hipMemcpyHostToDevice, a->stream));
     BSM_HIP(hipStreamSynchronize(a->stream));
@@ -790,6 +815,10 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
     }
     BSM_HIP(hipMemcpy(p + offset, src, len, hipMemcpyHostToDevice));
     layout_stats_update(a, buf, layer, offset, len, (const uint8_t*)src, -1);
+    if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS) {
+        if (launch_pack_vu(a->stream, a, layer, offset, len)) return 1;
+        BSM_HIP(hipStreamSynchronize(a->stream));
+    }
     return 0;
 }
 
@@ -809,6 +838,8 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
         BSM_HIP(hipMemsetAsync(p, val, n, a->stream));
         // a whole-buffer fill replaces the summary of every tuft of the layer
         if (buf != BSM_AXN_STATES) layout_stats_update(a, buf, layer, 0, n, nullptr, val);
+        if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS)
+            if (launch_pack_vu(a->stream, a, layer, 0, n)) return 1;
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;
@@ -957,6 +988,7 @@ int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
                 return 1;
             layout_stats_reset(a, l, t, (uint32_t)reach, T.d.src_slc_ids, T.d.n_src_slcs);
         }
+        if (launch_pack_vu(a->stream, a, l, 0, L.n_syn)) return 1;
         BSM_HIP(hipMemsetAsync(L.syn_str, 0, L.n_syn, a->stream));
     }
     BSM_HIP(hipStreamSynchronize(a->stream));

@@ -207,7 +207,7 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
     }
 }
 
-template <int W, int NST, bool FAST>
+template <int W, int NST, bool FAST, bool PACKED>
 __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                                                const WorkItem* __restrict__ items, uint32_t n_items,
                                                uint32_t par, uint32_t rnd_ofs, uint32_t TV,
@@ -286,8 +286,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
             // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
             const uint32_t nrec = J.C * spt;
-            const Rsrc r_slc = make_rsrc(J.slc, nrec), r_v = make_rsrc(J.vofs, nrec);
+            const Rsrc r_slc = make_rsrc(J.slc, nrec), r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
             const Rsrc r_u = make_rsrc(J.uofs, nrec), r_str = make_rsrc(J.str, nrec);
+            // packed offsets carry v + 8 and u + 8: their bias folds into the base
+            const int32_t pbias = PACKED ? 8 * nc + 8 : 0;
             const Rsrc r_out = make_rsrc(J.st[pu & 1], nrec);
             const int32_t coefA = (nc & 0xFF) | (1 << 8);
             const int32_t coefB = ((nc & 0xFF) << 16) | (1 << 24);
@@ -304,8 +306,8 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 so = rowt[2 * rowq] + rem * W;
                 lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
                 Chunk<W>::load_b(r_slc, so, ms);
-                Chunk<W>::load_b(r_v, so, mv);
-                Chunk<W>::load_b(r_u, so, mu);
+                Chunk<W>::load_b(r_v, so, mv);   // packed: (v + 8) << 4 | (u + 8) per byte
+                if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, mu);
                 Chunk<W>::load_b(r_str, so, mt);
             };
             auto work = [&](const uint32_t* ms, const uint32_t* mv, const uint32_t* mu,
@@ -314,13 +316,22 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 uint32_t out[NW], pw[NW];
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
+                    uint32_t vw, uw;
+                    if constexpr (PACKED) {
+                        vw = (mv[w] >> 4) & 0x0F0F0F0Fu;
+                        uw = mv[w] & 0x0F0F0F0Fu;
+                    } else {
+                        vw = mv[w];
+                        uw = mu[w];
+                    }
+                    const int32_t lbp = lb - pbias;
                     // [vo_b, uo_b, vo_b+1, uo_b+1] byte pairs -> dot4 gives lb + vo*cols + uo
-                    const uint32_t p01 = __builtin_amdgcn_perm(mu[w], mv[w], 0x05010400u);
-                    const uint32_t p23 = __builtin_amdgcn_perm(mu[w], mv[w], 0x07030602u);
-                    int32_t ad0 = __builtin_amdgcn_sdot4((int32_t)p01, coefA, lb, false);
-                    int32_t ad1 = __builtin_amdgcn_sdot4((int32_t)p01, coefB, lb, false);
-                    int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lb, false);
-                    int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lb, false);
+                    const uint32_t p01 = __builtin_amdgcn_perm(uw, vw, 0x05010400u);
+                    const uint32_t p23 = __builtin_amdgcn_perm(uw, vw, 0x07030602u);
+                    int32_t ad0 = __builtin_amdgcn_sdot4((int32_t)p01, coefA, lbp, false);
+                    int32_t ad1 = __builtin_amdgcn_sdot4((int32_t)p01, coefB, lbp, false);
+                    int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lbp, false);
+                    int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lbp, false);
                     if (NST > 1) {
                         ad0 += (int32_t)__umul24(byte_of(ms[w], 0), WS);
                         ad1 += (int32_t)__umul24(byte_of(ms[w], 1), WS);
@@ -820,6 +831,7 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     j.slc = L.syn_slc + T.syn_idz;
     j.vofs = L.syn_v + T.syn_idz;
     j.uofs = L.syn_u + T.syn_idz;
+    j.vu = L.syn_vu + T.syn_idz;
     j.str = L.syn_str + T.syn_idz;
     j.st[0] = L.syn_state[0] + T.syn_idz;
     j.st[1] = L.syn_state[1] + T.syn_idz;
@@ -894,6 +906,8 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     }
     j.fast = fast ? 1u : 0u;
     if (fast) j.halo = T.reach_max;
+    // the packed offsets are exact when every |offset| fits in 3 bits + sign
+    j.packed = (fast && T.reach_v <= PACK_REACH && T.reach_u <= PACK_REACH) ? 1u : 0u;
     return j;
 }
 
@@ -999,13 +1013,15 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st) {
     uint32_t W = 16, nst = 0;
-    bool fast = true;
+    bool fast = true, packed = true;
     for (uint32_t k = 0; k < job_ids.size(); k++) {
         W = std::min(W, job_w[k]);
         nst = std::max(nst, jobs[job_ids[k]].n_staged);
         fast &= jobs[job_ids[k]].fast != 0;
+        packed &= jobs[job_ids[k]].packed != 0;
     }
     fast &= W >= 4;
+    packed &= fast;
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
     uint32_t cap = 0, lds = 0;
     for (;;) {
@@ -1031,12 +1047,14 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
     st.alg_bytes = 0;
+    // per synapse: the slice id unless one staged slice makes it implicit, the
+    // offsets (1 B packed, 2 B otherwise), the strength in and the state out
+    const uint32_t syn_b = ((fast && nst <= 1) ? 0u : 1u) + (packed ? 1u : 2u) + 2u;
     for (uint32_t id : job_ids) {
-        // per synapse 3 B metadata + 1 B strength in, 1 B state out; per dendrite
-        // raw/state out + activity in/out; per cell-tuft 4 B in + 8 B out
-        // (pyramidal); per SSC cell its energy
+        // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
+        // out (pyramidal); per SSC cell its energy
         const TuftJob& j = jobs[id];
-        st.alg_bytes += (uint64_t)j.C * j.spt * 5 + (uint64_t)j.C * j.dens * 4 +
+        st.alg_bytes += (uint64_t)j.C * j.spt * syn_b + (uint64_t)j.C * j.dens * 4 +
                         (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0);
     }
     for (uint32_t id : job_ids) {
@@ -1052,6 +1070,7 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     st.W = W;
     st.nst = nst <= 1 ? 1 : (nst <= 2 ? 2 : 4);
     st.fast = fast ? 1u : 0u;
+    st.packed = packed ? 1u : 0u;
     if (fast) st.nst = nst <= 1 ? 1 : 2;
     st.stage_cap = cap;
     st.tv = TV;
@@ -1230,15 +1249,15 @@ int build_plan(bsm_area* a) {
 
 #define F1_CASE(w, n)                                                                        \
     case (w * 8 + n):                                                                        \
-        kf_tuft<w, n, false><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par,    \
-                                                                rnd_ofs, st.tv, st.tu,      \
-                                                                st.stage_cap);              \
+        kf_tuft<w, n, false, false><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
+                                                                       par, rnd_ofs, st.tv, \
+                                                                       st.tu, st.stage_cap);\
         break;
-#define F1_FAST_CASE(w, n)                                                                   \
-    case (w * 8 + n):                                                                        \
-        kf_tuft<w, n, true><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par,     \
-                                                               rnd_ofs, st.tv, st.tu,       \
-                                                               st.stage_cap);               \
+#define F1_FAST_CASE(w, n, pk)                                                               \
+    case ((w * 8 + n) * 2 + pk):                                                             \
+        kf_tuft<w, n, true, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
+                                                                        par, rnd_ofs, st.tv,\
+                                                                        st.tu, st.stage_cap);\
         break;
 
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
@@ -1260,6 +1279,7 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
         o.grid = st.grid;
         o.stream = st.stream;
         o.alg_bytes = st.alg_bytes;
+        o.packed = st.packed;
     }
     return 0;
 }
@@ -1269,10 +1289,13 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
     if (st.n_items == 0) return 0;
     const WorkItem* it = items + st.item0;
     if (st.kind == 1 && st.fast) {
-        switch (st.W * 8 + st.nst) {
-            F1_FAST_CASE(16, 1) F1_FAST_CASE(16, 2)
-            F1_FAST_CASE(8, 1) F1_FAST_CASE(8, 2)
-            F1_FAST_CASE(4, 1) F1_FAST_CASE(4, 2)
+        switch ((st.W * 8 + st.nst) * 2 + (st.packed ? 1 : 0)) {
+            F1_FAST_CASE(16, 1, 0) F1_FAST_CASE(16, 2, 0)
+            F1_FAST_CASE(8, 1, 0) F1_FAST_CASE(8, 2, 0)
+            F1_FAST_CASE(4, 1, 0) F1_FAST_CASE(4, 2, 0)
+            F1_FAST_CASE(16, 1, 1) F1_FAST_CASE(16, 2, 1)
+            F1_FAST_CASE(8, 1, 1) F1_FAST_CASE(8, 2, 1)
+            F1_FAST_CASE(4, 1, 1) F1_FAST_CASE(4, 2, 1)
         default: return fail("bad fast F1 variant");
         }
     } else if (st.kind == 1) {
@@ -1455,18 +1478,19 @@ __global__ __launch_bounds__(256) void k_calib_copy(const uint4* __restrict__ sr
         dst[i] = src[i];
 }
 
-// The synapse streams of F1 without the gather: four 16-byte non-temporal
-// read streams (n vectors each, back to back in src) and one 16-byte write
-// stream, the load/store forms F1 uses.
+// The synapse streams of F1 without the gather: NR 16-byte non-temporal read
+// streams (n vectors each, back to back in src) and one 16-byte write stream,
+// the load/store forms F1 uses.
+template <int NR>
 __global__ __launch_bounds__(256) void k_calib_stream(const u32x4* __restrict__ src,
                                                       u32x4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        const u32x4 a = __builtin_nontemporal_load(BSM_G(const u32x4, src + i));
-        const u32x4 b = __builtin_nontemporal_load(BSM_G(const u32x4, src + n + i));
-        const u32x4 c = __builtin_nontemporal_load(BSM_G(const u32x4, src + 2 * n + i));
-        const u32x4 d = __builtin_nontemporal_load(BSM_G(const u32x4, src + 3 * n + i));
-        *BSM_G(u32x4, dst + i) = a ^ b ^ c ^ d;
+        u32x4 x = __builtin_nontemporal_load(BSM_G(const u32x4, src + i));
+#pragma unroll
+        for (int k = 1; k < NR; k++)
+            x ^= __builtin_nontemporal_load(BSM_G(const u32x4, src + k * n + i));
+        *BSM_G(u32x4, dst + i) = x;
     }
 }
 }  // namespace bsm
@@ -1523,9 +1547,11 @@ int bsm_area_stage_plan(bsm_area* a, uint32_t max_stages, bsm_stage_info* out,
     return stage_info(a->stages, max_stages, out, n_stages);
 }
 
-int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms) {
+int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, uint32_t n_reads,
+                     float* ms) {
     if (!ctx || !dst || !src) return fail("null arg");
     if (bytes % 16) return fail("bytes must be a multiple of 16");
+    if (n_reads < 1 || n_reads > 4) return fail("n_reads must be in [1, 4]");
     BSM_HIP(hipSetDevice(ctx->device));
     hipEvent_t e0, e1;
     BSM_HIP(hipEventCreate(&e0));
@@ -1533,7 +1559,12 @@ int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, flo
     const size_t n = bytes / 16;
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
     BSM_HIP(hipEventRecord(e0, 0));
-    k_calib_stream<<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n);
+    switch (n_reads) {
+    case 1: k_calib_stream<1><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
+    case 2: k_calib_stream<2><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
+    case 3: k_calib_stream<3><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
+    default: k_calib_stream<4><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
+    }
     BSM_HIP(hipGetLastError());
     BSM_HIP(hipEventRecord(e1, 0));
     BSM_HIP(hipEventSynchronize(e1));

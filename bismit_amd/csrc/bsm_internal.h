@@ -30,6 +30,7 @@ struct TuftJob {
     const uint8_t* slc;
     const int8_t* vofs;
     const int8_t* uofs;
+    const uint8_t* vu;     // packed offsets: (v + 8) << 4 | (u + 8), valid when |v|, |u| <= 7
     const int8_t* str;
     uint8_t* st[2];        // tuft base in the two state buffers; st[par] is written
     uint8_t* den_raw;      // tuft dendrite base
@@ -51,6 +52,7 @@ struct TuftJob {
     uint32_t halo;         // staged reach around the tile, in source-slice columns
     uint32_t par_xor;      // state buffer written = st[(par ^ par_xor) & 1]
     uint32_t fast;         // every synapse sources a staged slice within `halo`
+    uint32_t packed;       // fast, and every |offset| <= 7: the gather reads `vu`
     const uint32_t* dyn;   // dynamic batches: per (area, layer) word, bit0 parity, bit1 active
     // host-side planning copies of the staged slices' geometry
     uint16_t st_vsize[4], st_usize[4];
@@ -135,6 +137,7 @@ struct LayerRt {
     uint8_t* syn_slc;
     int8_t* syn_v;
     int8_t* syn_u;
+    uint8_t* syn_vu;         // packed copy of the offsets for the gather (see TuftJob::vu)
     int8_t* syn_str;
     uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;
     uint8_t* tft[8];
@@ -164,6 +167,7 @@ struct Stage {
     uint32_t W;          // synapse chunk width (bytes per lane)
     uint32_t nst;        // F1: staged-slice template (1, 2 or 4)
     uint32_t fast;       // F1: all jobs take the check-free gather path
+    uint32_t packed;     // F1: all jobs read packed offsets
     uint32_t stage_cap;  // F1: LDS bytes reserved for staged windows
     uint32_t tv, tu;     // tile extents in columns
     uint32_t grid;
@@ -311,6 +315,10 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
 int build_plan(bsm_area* a);
 int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out);
 int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len);
+// Refreshes the packed offsets of synapses [offset, offset + len) of a layer from
+// its v and u offset arrays (after any write to them).
+int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len);
+constexpr uint32_t PACK_REACH = 7;   // |offset| bound of the packed form
 // Enqueues the deterministic regrowth of every tuft of the area on stream s and
 // widens the tufts' source summaries; *widened reports whether a summary grew
 // (the fused plan is then stale).

@@ -325,7 +325,8 @@ int bsm_batch_profile_stages(bsm_batch* batch, uint32_t n_cycles, uint32_t max_s
  * of a cycle, in launch order.  Lets callers and tests see which kernel
  * variant runs (chunk width, check-free gather) and its tiling. */
 typedef struct bsm_stage_info {
-    int32_t kind;        /* 1 gather+dendrites+tufts, 2 SSC post, 3 pyramidal post */
+    int32_t kind;        /* 1 gather+dendrites+tufts, 2 SSC post, 3 pyramidal post,
+                            4 activity smoother */
     uint32_t layer;      /* plain form: the layer; batches: 0 */
     uint32_t chunk;      /* synapses per lane chunk (W) */
     uint32_t staged;     /* F1: staged-slice template */
@@ -337,9 +338,14 @@ typedef struct bsm_stage_info {
     uint32_t stream;     /* 0: the area/batch stream; 1: the side stream (runs beside
                             the spatial layers' stages, joined before the layer's
                             pyramidal post stage) */
-    uint64_t alg_bytes;  /* gather+dendrites+tufts stages: algorithmic HBM bytes of one
-                            launch (5 B per synapse, 4 B per dendrite, 12 B per
-                            pyramidal cell-tuft, 1 B per SSC cell); 0 otherwise */
+    uint64_t alg_bytes;  /* gather+dendrites+tufts stages: HBM bytes this launch must
+                            move: per synapse the source slice id (only when several
+                            slices are staged or the gather is checked), the offsets
+                            (1 B packed, else 2 B), the strength and the new state; 4 B
+                            per dendrite, 12 B per pyramidal cell-tuft, 1 B per SSC
+                            cell; 0 for other stages */
+    uint32_t packed;     /* F1: the gather reads the packed offsets */
+    uint32_t _reserved;
 } bsm_stage_info;
 int bsm_area_stage_plan(bsm_area* area, uint32_t max_stages, bsm_stage_info* out,
                         uint32_t* n_stages);
@@ -348,11 +354,12 @@ int bsm_batch_stage_plan(bsm_batch* batch, uint32_t max_stages, bsm_stage_info* 
 /* Plain device-to-device copy kernel of `bytes` (multiple of 16): the known-size
  * reference for scaling HBM traffic counters. */
 int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
-/* The fused synapse stage's HBM pattern without its gather: src holds four
- * streams of `bytes` (multiple of 16) back to back, read with non-temporal
- * 16-byte loads; dst receives one stream of `bytes`.  The practical bandwidth
- * ceiling for that stage's 4:1 read/write mix. */
-int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float* ms);
+/* The fused synapse stage's HBM pattern without its gather: src holds n_reads
+ * (1-4) streams of `bytes` (multiple of 16) back to back, read with
+ * non-temporal 16-byte loads; dst receives one stream of `bytes`.  The practical
+ * bandwidth ceiling for that stage's read/write mix. */
+int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, uint32_t n_reads,
+                     float* ms);
 
 /* ---- host-side synapse layout generator (replaces Synapses::grow(init=true),
  *      src/cortex/synapses.rs:454-501, entropy-seeded upstream): src slice

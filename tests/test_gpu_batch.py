@@ -159,11 +159,21 @@ def test_config3_plan_uses_fast_wide_variant(ctx):
     f1 = [p for p in plan if p["kind"] == 1]
     assert [p["stream"] for p in f1] == [0, 1, 0]   # distal tufts beside the SSC stages
     for p in f1:
-        assert p["chunk"] == 16 and p["fast"] == 1, p
-    assert sum(p["alg_bytes"] for p in f1) == sum(
-        sum(s.cells(li) * (t.syns_per_tft * 5 + t.dens_per_tft * 4 + (12 if lay.kind == 2 else 0))
-            for t in lay.tufts) + (s.cells(li) if lay.kind == 1 else 0)
-        for s in schemes for li, lay in enumerate(s.layers))
+        assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
+
+    def launch_bytes(p, pick):
+        # per synapse: slice id unless one staged slice, packed offsets, strength, state
+        syn_b = (0 if p["staged"] <= 1 else 1) + 1 + 2
+        return sum(s.cells(li) * (t.syns_per_tft * syn_b + t.dens_per_tft * 4 +
+                                  (12 if lay.kind == 2 else 0)) + (s.cells(li) if lay.kind == 1 else 0)
+                   for s in schemes for li, lay in enumerate(s.layers)
+                   for ti, t in enumerate(lay.tufts) if pick(lay, ti))
+
+    # SSC tufts, distal tufts (side stream), proximal tufts
+    assert [p["alg_bytes"] for p in f1] == [
+        launch_bytes(f1[0], lambda lay, ti: lay.kind == 1),
+        launch_bytes(f1[1], lambda lay, ti: lay.kind == 2 and ti == 1),
+        launch_bytes(f1[2], lambda lay, ti: lay.kind == 2 and ti == 0)]
     b.close()
     for a in areas:
         a.close()

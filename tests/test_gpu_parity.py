@@ -127,7 +127,7 @@ def test_cycle_config1_1000_cycles(ctx):
     # distal tuft (own layer's slices) runs on the side stream
     plan = g.stage_plan()
     assert [(p["kind"], p["stream"]) for p in plan] == [(1, 0), (2, 0), (1, 1), (1, 0), (3, 0)]
-    assert all(p["fast"] == 1 for p in plan if p["kind"] == 1)
+    assert all(p["fast"] == 1 and p["packed"] == 1 for p in plan if p["kind"] == 1)
     frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=0).frames(1000)
     run_cycles(g, o, scheme, frames, 1000, tag="c1")
     # the run did learn and fire
@@ -197,12 +197,54 @@ def test_cycle_config2_full_size(ctx):
     assert [(p["kind"], p["stream"]) for p in plan] == [(1, 0), (2, 0), (1, 1), (1, 0), (3, 0)]
     for p in plan:
         if p["kind"] == 1:
-            assert p["chunk"] == 16 and p["fast"] == 1, p
-    # algorithmic bytes of the pyramidal tuft launches: 5 B/synapse, 4 B/dendrite,
-    # 12 B/cell-tuft, one tuft each
+            assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
+    # bytes of the pyramidal tuft launches, one tuft each: per synapse the packed
+    # offsets, strength and state, plus the slice id for the distal tuft (two
+    # staged slices); 4 B/dendrite, 12 B/cell-tuft
     cells = 2 * 256 * 256
     assert [p["alg_bytes"] for p in plan if p["kind"] == 1][1:] == \
-        [cells * 512 * 5 + cells * 8 * 4 + cells * 12] * 2
+        [cells * 512 * b + cells * 8 * 4 + cells * 12 for b in (4, 3)]
+    g.close()
+    o.close()
+
+
+def test_packed_offsets_follow_writes(ctx):
+    """The gather's packed copy of the offsets follows every write: a partial
+    write across a tuft boundary, a write past |offset| 7 (that tuft falls back
+    to the unpacked reads), and a fill; each followed by cycles against the
+    oracle."""
+    s = recipes.cortical_area("pk", 32, pyr_depth=2, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                              pyr_spd=16)
+    g, o = make_pair(ctx, s)
+    rng = np.random.default_rng(12)
+    frames = inputs.SdrStream(32 * 32, 0.05, 0.5, seed=5).frames(40)
+    k = [0]
+
+    def cycles(tag, n=8):
+        run_cycles(g, o, s, frames[k[0]:k[0] + n], n, tag=tag)
+        k[0] += n
+
+    def put(name, li, arr, offset=0):
+        g.write(name, li, arr, offset)
+        o.write(name, li, arr, offset)
+
+    def packed():
+        return [(p["layer"], p["packed"]) for p in g.stage_plan() if p["kind"] == 1]
+
+    assert all(pk == 1 for _, pk in packed())
+    cycles("initial")
+    cells = s.cells(1)
+    t0 = cells * s.layers[1].tufts[0].syns_per_tft
+    put("SYN_SRC_COL_U_OFFS", 1, rng.integers(-7, 8, 300).astype(np.int8), t0 - 100)
+    assert all(pk == 1 for _, pk in packed())
+    cycles("partial write across tufts")
+    put("SYN_SRC_COL_V_OFFS", 1, rng.integers(-9, 10, 64).astype(np.int8), 200)
+    assert (1, 0) in packed() and (1, 1) in packed()   # tuft 0 unpacked, tuft 1 still packed
+    cycles("offsets past 7")
+    for name in ("SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS"):
+        g.fill(name, 0, 3)
+        o.buf(name, 0)[:] = 3
+    cycles("fill")
     g.close()
     o.close()
 

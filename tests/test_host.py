@@ -36,12 +36,13 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "bismit_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",'
                    'sizeof(bsm_slice_desc), sizeof(bsm_tuft_desc), sizeof(bsm_layer_desc),'
                    'sizeof(bsm_area_desc), offsetof(bsm_layer_desc, tufts),'
                    'offsetof(bsm_area_desc, layers), offsetof(bsm_tuft_desc, threshold),'
                    'sizeof(bsm_tuning), sizeof(bsm_stage_info),'
-                   'offsetof(bsm_stage_info, alg_bytes));return 0;}\n')
+                   'offsetof(bsm_stage_info, alg_bytes), offsetof(bsm_stage_info, packed));'
+                   'return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
@@ -50,7 +51,7 @@ def test_struct_layouts_match_header(tmp_path):
     want = [C.sizeof(L.SliceDesc), C.sizeof(L.TuftDesc), C.sizeof(L.LayerDesc),
             C.sizeof(L.AreaDesc), L.LayerDesc.tufts.offset, L.AreaDesc.layers.offset,
             L.TuftDesc.threshold.offset, C.sizeof(L.Tuning), C.sizeof(L.StageInfo),
-            L.StageInfo.alg_bytes.offset]
+            L.StageInfo.alg_bytes.offset, L.StageInfo.packed.offset]
     assert got == want
 
 
@@ -139,7 +140,9 @@ def test_footprint_config2():
     assert s.n_synapses() == 142606336
     rc, n, _ = footprint_rc(s)
     assert rc == 0
-    assert 7 * s.n_synapses() <= n < 8 * s.n_synapses()
+    # 8 B per synapse: two state buffers, static flags, slice id, v and u offsets,
+    # their packed copy and the strength; dendrites and cells add the rest
+    assert 8 * s.n_synapses() <= n < 9 * s.n_synapses()
 
 
 def test_config_sizes():
