@@ -293,7 +293,7 @@ def main():
     if not args.tile:
         # tile geometry by the project autotuner (timed on a scratch copy of the area)
         from bismit_amd.autotune import tune_area
-        tiles = [(8, 8), (4, 8), (2, 8), (2, 16), (1, 32), (4, 16)]
+        tiles = [(8, 8), (4, 8), (2, 8), (2, 16), (1, 32), (4, 16), (1, 16)]
         tuning, _ = tune_area(scheme, ctx, cycles=10,
                               cands=[dict(tile_v=tv, tile_u=tu, chunk=16, halo=6)
                                      for tv, tu in tiles])
