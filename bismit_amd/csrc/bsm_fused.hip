@@ -98,13 +98,70 @@ struct Chunk<1> {
     __device__ static inline void store(void* p, const uint32_t* o) { *BSM_G(uint8_t, p) = (uint8_t)o[0]; }
 };
 
+// The epilogue's global inputs of one tile, loaded when the tile starts so that
+// their latency hides behind the synapse loop: the first TAIL_PD dendrites and
+// TAIL_PC cell-tufts of each thread (bytes packed four to a register).  Every
+// byte belongs to this tile alone (its dendrites, its cells), so nothing else
+// writes it meanwhile.
+constexpr uint32_t TAIL_PD = 8;
+constexpr uint32_t TAIL_PC = 2;
+struct TailPre {
+    uint32_t act[TAIL_PD / 4];   // dendrite activity ratings
+    uint32_t en[TAIL_PD / 4];    // SSC soma energies (dendrite index < cells)
+    uint32_t tft[TAIL_PC];       // best_id, best_raw, best, state of a cell-tuft
+};
+
+__device__ inline uint32_t tile_cell(uint32_t ct, uint32_t tvn, uint32_t tun, uint32_t v0,
+                                     uint32_t u0, uint32_t VU, uint32_t U) {
+    const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
+    const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
+    return sl * VU + (v0 + r) * U + (u0 + cc);
+}
+
+__device__ inline void tail_prefetch(const TuftJob& J, uint32_t depth, uint32_t tvn,
+                                     uint32_t tun, uint32_t v0, uint32_t u0, uint32_t VU,
+                                     uint32_t U, uint32_t tid, uint32_t nth, TailPre& pre) {
+    const uint32_t dens = J.dens, cells_t = depth * tvn * tun, nd_t = cells_t * dens;
+    auto den_act = BSM_G(const uint8_t, J.den_act);
+    auto energy = BSM_G(const uint8_t, J.ssc_energy);
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PD / 4; k++) {
+        pre.act[k] = 0;
+        pre.en[k] = 0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PD; k++) {
+        const uint32_t x = tid + k * nth;
+        if (x < nd_t) {
+            const uint32_t ct = x / dens, dj = x - ct * dens;
+            const uint32_t dl = tile_cell(ct, tvn, tun, v0, u0, VU, U) * dens + dj;
+            const uint32_t dg = J.den_idz + dl;
+            pre.act[k / 4] |= (uint32_t)den_act[dl] << (8 * (k % 4));
+            if (J.ssc_energy != nullptr && dg < J.C)
+                pre.en[k / 4] |= (uint32_t)energy[dg] << (8 * (k % 4));
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PC; k++) {
+        pre.tft[k] = 0;
+        const uint32_t ct = tid + k * nth;
+        if (J.tft[0] != nullptr && ct < cells_t) {
+            const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
+            pre.tft[k] = (uint32_t)BSM_G(const uint8_t, J.tft[0])[cel] |
+                         ((uint32_t)BSM_G(const uint8_t, J.tft[1])[cel] << 8) |
+                         ((uint32_t)BSM_G(const uint8_t, J.tft[2])[cel] << 16) |
+                         ((uint32_t)BSM_G(const uint8_t, J.tft[3])[cel] << 24);
+        }
+    }
+}
+
 // Dendrites (den_cycle_tft, ssc_cycle on the SSC soma) and tufts (tft_cycle) of
 // one tile from the per-chunk (sum, raw) partials in LDS; ends with a barrier.
 template <int W>
 __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* acc,
                                  uint32_t stage_cap, uint32_t depth, uint32_t tvn, uint32_t tun,
                                  uint32_t v0, uint32_t u0, uint32_t VU, uint32_t U, int32_t rnd,
-                                 uint32_t tid, uint32_t nth) {
+                                 uint32_t tid, uint32_t nth, const TailPre& pre) {
     // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
     const uint32_t dens = J.dens;
     const uint32_t cells_t = depth * tvn * tun;
@@ -115,14 +172,12 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
     auto den_raw = BSM_G(uint8_t, J.den_raw);
     auto den_st = BSM_G(uint8_t, J.den_st);
     auto energy = BSM_G(uint8_t, J.ssc_energy);
-    for (uint32_t x = tid; x < nd_t; x += nth) {
+    auto den_one = [&](uint32_t x, uint32_t act0, uint32_t e0, bool have) {
         const uint32_t ct = x / dens, dj = x - ct * dens;
         const uint32_t pv = acc[x];
         uint32_t sum = pv & 0xFFFFu;
         const uint32_t raw = pv >> 16;
-        const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
-        const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
-        const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+        const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
         const uint32_t dl = cel * dens + dj;
         const uint32_t dg = J.den_idz + dl;
         const int32_t active = sum > J.thresh;
@@ -130,11 +185,12 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
         const uint32_t raw_o = min(raw >> 4, 255u);
         uint32_t st_o = min(sum >> 4, 255u);
         const int32_t seed = (int32_t)((raw + J.thresh) | dg);
-        den_act[dl] = (uint8_t)activity(den_act[dl], active, rnd, seed, DEN_ACT_DECAY);
+        den_act[dl] = (uint8_t)activity(have ? act0 : den_act[dl], active, rnd, seed,
+                                        DEN_ACT_DECAY);
         den_raw[dl] = (uint8_t)raw_o;
         if (J.ssc_energy != nullptr && dg < J.C) {
             // ssc_cycle (cl/bismit.cl:849-884) on soma index dg == dendrite dg
-            const uint32_t e = energy[dg];
+            const uint32_t e = have ? e0 : energy[dg];
             const uint32_t restless = (e > 191) & (st_o != 0);
             const uint32_t dark = e == 255;
             if (dark | restless) energy[dg] = (uint8_t)(e - 64);
@@ -145,13 +201,21 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             denc[x] = (uint8_t)raw_o;
             denc[nd_t + x] = (uint8_t)st_o;
         }
+    };
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PD; k++) {
+        const uint32_t x = tid + k * nth;
+        if (x < nd_t)
+            den_one(x, (pre.act[k / 4] >> (8 * (k % 4))) & 0xFFu,
+                    (pre.en[k / 4] >> (8 * (k % 4))) & 0xFFu, true);
     }
+    for (uint32_t x = tid + TAIL_PD * nth; x < nd_t; x += nth) den_one(x, 0u, 0u, false);
 
     // ---- tufts (tft_cycle)
     if (do_tft) {
         __syncthreads();
         const uint32_t l2 = J.l2;
-        for (uint32_t ct = tid; ct < cells_t; ct += nth) {
+        for (uint32_t ct = tid, k = 0; ct < cells_t; ct += nth, k++) {
             uint32_t best_id = 0, best_raw = 0, best = 0, asum = 0;
             for (uint32_t j = 0; j < dens; j++) {
                 const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
@@ -160,15 +224,21 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
                 asum += s;
             }
             const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
-            const uint32_t rowq = ct / tun, cc = ct - rowq * tun;
-            const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
-            const uint32_t cel = sl * VU + (v0 + r) * U + (u0 + cc);
+            const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
             auto t0 = BSM_G(uint8_t, J.tft[0]), t1 = BSM_G(uint8_t, J.tft[1]);
             auto t2 = BSM_G(uint8_t, J.tft[2]), t3 = BSM_G(uint8_t, J.tft[3]);
-            BSM_G(uint8_t, J.tft[4])[cel] = t0[cel];
-            BSM_G(uint8_t, J.tft[5])[cel] = t1[cel];
-            BSM_G(uint8_t, J.tft[6])[cel] = t2[cel];
-            BSM_G(uint8_t, J.tft[7])[cel] = t3[cel];
+            // the current values become prev_* (prefetched for the first TAIL_PC)
+            uint32_t cur = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < TAIL_PC; q++)
+                if (k == q) cur = pre.tft[q];
+            if (k >= TAIL_PC)
+                cur = (uint32_t)t0[cel] | ((uint32_t)t1[cel] << 8) | ((uint32_t)t2[cel] << 16) |
+                      ((uint32_t)t3[cel] << 24);
+            BSM_G(uint8_t, J.tft[4])[cel] = (uint8_t)cur;
+            BSM_G(uint8_t, J.tft[5])[cel] = (uint8_t)(cur >> 8);
+            BSM_G(uint8_t, J.tft[6])[cel] = (uint8_t)(cur >> 16);
+            BSM_G(uint8_t, J.tft[7])[cel] = (uint8_t)(cur >> 24);
             t0[cel] = (uint8_t)best_id;
             t1[cel] = (uint8_t)best_raw;
             t2[cel] = (uint8_t)best;
@@ -232,6 +302,8 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         const bool quirk = J.scalar_quirk != 0;
         const uint32_t nd_t = depth * tvn * tun * J.dens;
+        TailPre pre;
+        tail_prefetch(J, depth, tvn, tun, v0, u0, V * U, U, tid, nth, pre);
         for (uint32_t x = tid; x < nd_t; x += nth) acc[x] = 0u;
         const bool whole = (J.spd % W) == 0;
         const uint32_t magic = 0xFFFFFFFFu / J.spd + 1u;
@@ -369,7 +441,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 q = qa;
             }
             __syncthreads();
-            tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth);
+            tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth, pre);
             continue;
         }
 
@@ -465,7 +537,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         __syncthreads();
 
-        tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth);
+        tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth, pre);
     }
 }
 
