@@ -24,34 +24,35 @@ HBM_PEAK_GBS = 8000.0   # MI355X published HBM3E bandwidth
 ALG_BYTES_PER_SYNAPSE = 8
 ALG_BASIS = ("8 B per synapse-update: 3 B source metadata + 1 B strength + 1 B previous "
              "state + 1 B new state + 2 B flags read and written (dendrite/tuft bytes excluded)")
-def engine_basis(syn_b):
-    return (f"bytes this engine must move: {syn_b} B per synapse on this launch (source slice "
-            "id when several slices are staged, offsets packed into 1 B when |v|,|u| <= 7, "
+def engine_basis(syn_bs):
+    per = " or ".join(f"{b} B" for b in sorted(set(syn_bs)))
+    return (f"bytes this engine must move: {per} per synapse by tuft (source slice id when the "
+            "tuft's gather stages several slices, offsets packed into 1 B when |v|,|u| <= 7, "
             "strength, new state; the previous-active flag is derived from the previous state "
             "buffer) + 4 B per dendrite + 12 B per pyramidal cell-tuft")
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
-def syn_bytes(stage):
-    """Per-synapse HBM bytes of an F1 launch: the slice id unless a single staged
-    slice makes it implicit, the offsets (1 B packed, else 2 B), strength in and
-    state out."""
-    slc = 0 if (stage["fast"] and stage["staged"] <= 1) else 1
+def syn_bytes(stage, tuft):
+    """Per-synapse HBM bytes of one tuft in an F1 launch: the slice id unless the
+    tuft's gather stages a single slice (then the id is implicit and its stream
+    is not read), the offsets (1 B packed, else 2 B), strength in and state
+    out."""
+    slc = 0 if (stage["fast"] and len(tuft.src_slcs) <= 1) else 1
     return slc + (1 if stage["packed"] else 2) + 2
 
 
-def stage_bytes(scheme, li, kind, syn_b):
-    """HBM bytes this engine moves in one fused stage launch on layer li:
-    F1: per synapse syn_b (syn_bytes); per dendrite raw/state writes + activity
-        read/write (4 B); per cell-tuft 4 B read + 8 B written (pyramidal), per
-        SSC cell 1 B energy read.
-    F2/F3 are reported but not used for the roofline."""
+def stage_bytes(scheme, li, kind, stage):
+    """HBM bytes this engine moves in the F1 launches of layer li:
+    per synapse syn_bytes(stage, tuft); per dendrite raw/state writes + activity
+    read/write (4 B); per cell-tuft 4 B read + 8 B written (pyramidal), per SSC
+    cell 1 B energy read.  F2/F3 are reported but not used for the roofline."""
     lay = scheme.layers[li]
     C = scheme.cells(li)
     if kind == 1:
         b = 0
         for t in lay.tufts:
-            b += C * t.syns_per_tft * syn_b + C * t.dens_per_tft * 4
+            b += C * t.syns_per_tft * syn_bytes(stage, t) + C * t.dens_per_tft * 4
             if lay.kind == 2:
                 b += C * 12
         if lay.kind == 1:
@@ -346,22 +347,22 @@ def main():
             "how": "host frame per cycle through bsm_area_intake (synchronous copy), then "
                    "bsm_area_cycle with host-drawn rnd; Python host loop"}
     (dk, dl), (dt, alg_tot, n_launch, dstage) = max(groups.items(), key=lambda kv: kv[1][0])
-    syn_b = syn_bytes(dstage)
+    syn_bs = [syn_bytes(dstage, t) for t in scheme.layers[dl].tufts]
     # the engine's count equals the recipe's
-    assert alg_tot == stage_bytes(scheme, dl, dk, syn_b), (alg_tot, syn_b)
+    assert alg_tot == stage_bytes(scheme, dl, dk, dstage), (alg_tot, syn_bs)
     avg_s = dt / n_prof / n_launch / 1000.0
     # algorithmic bytes: ALG_BYTES_PER_SYNAPSE per synapse-update of the launch
     syn_launch = scheme.cells(dl) * sum(t.syns_per_tft for t in scheme.layers[dl].tufts) / n_launch
     alg = ALG_BYTES_PER_SYNAPSE * syn_launch
     achieved = alg / avg_s / 1e9
-    # the bytes this engine has to move for the same launch (syn_b per synapse,
+    # the bytes this engine has to move for the same launch (syn_bytes per synapse,
     # 4 B per dendrite, 12 B per pyramidal cell-tuft)
     moved = alg_tot / n_launch
     moved_gbs = moved / avg_s / 1e9
     traffic = read_traffic()
-    # the calibration kernel with the launch's read streams (syn_b - 1) and its
-    # one write stream
-    ceil = stream_ceiling(ctx, n_reads=syn_b - 1)
+    # the calibration kernel with the launch's most read streams (syn_bytes - 1) and
+    # its one write stream
+    ceil = stream_ceiling(ctx, n_reads=max(syn_bs) - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",
@@ -371,7 +372,7 @@ def main():
                 "engine_bytes": {"per_launch": moved, "achieved": round(moved_gbs, 1),
                                  "frac": round(moved_gbs / HBM_PEAK_GBS, 4),
                                  "frac_of_stream_ceiling": round(moved_gbs / ceil["gbs"], 4),
-                                 "basis": engine_basis(syn_b)},
+                                 "basis": engine_basis(syn_bs)},
                 "stream_ceiling": ceil}
 
     out = {

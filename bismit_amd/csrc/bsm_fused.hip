@@ -33,7 +33,13 @@ typedef __amdgpu_buffer_rsrc_t Rsrc;
 __device__ inline Rsrc make_rsrc(const void* base, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
-constexpr int BUF_NT = 2;  // non-temporal (streaming) cache policy
+// Non-temporal (streaming) cache policy, for the synapse streams' loads and
+// stores (new states, learned strengths): each byte is touched once per cycle,
+// so keeping it in L2 only evicts the axon windows and dendrite arrays that are
+// reused.
+// (Measured on the Config 2 pyramidal launch: the streaming store takes 6% off
+// the launch, and 11% off the 64-area batched cycle.)
+constexpr int BUF_NT = 2;
 
 template <int W>
 struct Chunk;
@@ -46,7 +52,7 @@ struct Chunk<16> {
     }
     __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
         u32x4 v = {o[0], o[1], o[2], o[3]};
-        __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, BUF_NT);
     }
     __device__ static inline void load(const void* p, uint32_t* o) {
         u32x4 x = __builtin_nontemporal_load(BSM_G(const u32x4, p));
@@ -54,7 +60,7 @@ struct Chunk<16> {
     }
     __device__ static inline void store(void* p, const uint32_t* o) {
         u32x4 v = {o[0], o[1], o[2], o[3]};
-        *BSM_G(u32x4, p) = v;
+        __builtin_nontemporal_store(v, BSM_G(u32x4, p));
     }
 };
 template <>
@@ -66,7 +72,7 @@ struct Chunk<8> {
     }
     __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
         u32x2 v = {o[0], o[1]};
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, (int)off, 0, BUF_NT);
     }
     __device__ static inline void load(const void* p, uint32_t* o) {
         u32x2 x = __builtin_nontemporal_load(BSM_G(const u32x2, p));
@@ -74,7 +80,7 @@ struct Chunk<8> {
     }
     __device__ static inline void store(void* p, const uint32_t* o) {
         u32x2 v = {o[0], o[1]};
-        *BSM_G(u32x2, p) = v;
+        __builtin_nontemporal_store(v, BSM_G(u32x2, p));
     }
 };
 template <>
@@ -84,12 +90,14 @@ struct Chunk<4> {
         o[0] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, BUF_NT);
     }
     __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
-        __builtin_amdgcn_raw_buffer_store_b32(o[0], r, (int)off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(o[0], r, (int)off, 0, BUF_NT);
     }
     __device__ static inline void load(const void* p, uint32_t* o) {
         o[0] = __builtin_nontemporal_load(BSM_G(const uint32_t, p));
     }
-    __device__ static inline void store(void* p, const uint32_t* o) { *BSM_G(uint32_t, p) = o[0]; }
+    __device__ static inline void store(void* p, const uint32_t* o) {
+        __builtin_nontemporal_store(o[0], BSM_G(uint32_t, p));
+    }
 };
 template <>
 struct Chunk<1> {
@@ -347,7 +355,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             // per tile row (slice, v): byte offset of its first cell's synapses, and
             // the window offset of that row's centre minus the first column's
             uint32_t* rowt = (uint32_t*)(lds + ((J.n_staged * WS + 15) & ~15u));
-            const int32_t sid0ws = NST > 1 ? (int32_t)(J.staged[0] * WS) : 0;
+            // a job with one staged slice in a several-slice launch reads no
+            // slice ids (its stream below is empty, so every id reads as 0)
+            const bool multi = NST > 1 && J.n_staged > 1;
+            const int32_t sid0ws = multi ? (int32_t)(J.staged[0] * WS) : 0;
             for (uint32_t rq = tid; rq < rows; rq += nth) {
                 const uint32_t sl = rq / tvn, v = v0 + (rq - sl * tvn);
                 rowt[2 * rq] = (sl * VU + v * U + u0) * spt;
@@ -358,7 +369,9 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
             // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
             const uint32_t nrec = J.C * spt;
-            const Rsrc r_slc = make_rsrc(J.slc, nrec), r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
+            // a zero-length stream: its loads return 0 without touching memory
+            const Rsrc r_slc = make_rsrc(J.slc, multi ? nrec : 0u);
+            const Rsrc r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
             const Rsrc r_u = make_rsrc(J.uofs, nrec), r_str = make_rsrc(J.str, nrec);
             // packed offsets carry v + 8 and u + 8: their bias folds into the base
             const int32_t pbias = PACKED ? 8 * nc + 8 : 0;
@@ -1116,16 +1129,60 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
         if (TV >= TU && TV > 1) TV = (TV + 1) / 2;
         else TU = (TU + 1) / 2;
     }
+    // A check-free item of under two chunks per lane spends most of its time in
+    // the per-item steps (window staging, the dendrite/tuft epilogue) rather
+    // than streaming synapses: grow such tiles (u first, since a tile row is
+    // contiguous in the synapse streams) while every job's item stays within
+    // its area, the LDS limit and the 2^16 chunk bound.  (Config 2's spiny
+    // stellate layer at a 2x16 tile: 15.2 us per launch; at 2x32: 12.6 us.)
+    if (fast) {
+        auto need = [&](uint32_t tv, uint32_t tu, uint32_t* chunks) {
+            uint32_t cap2 = 0, accb = 0, denc = 0, ch = 0;
+            for (uint32_t id : job_ids) {
+                const TuftJob& j = jobs[id];
+                cap2 = std::max(cap2, stage_bytes(j, tv, tu, true));
+                accb = std::max(accb, 4 * j.depth * tv * tu * j.dens);
+                if (j.tft[0]) denc = std::max(denc, 2 * j.depth * tv * tu * j.dens);
+                ch = std::max(ch, j.depth * tv * tu * (j.spt / std::max(W, 1u)));
+            }
+            *chunks = ch;
+            return ((cap2 + 15) & ~15u) + accb + denc;
+        };
+        uint32_t vmax = 0, umax = 0;
+        for (uint32_t id : job_ids) {
+            vmax = std::max(vmax, jobs[id].V);
+            umax = std::max(umax, jobs[id].U);
+        }
+        for (;;) {
+            uint32_t ch = 0;
+            need(TV, TU, &ch);
+            if (ch >= 2 * 256) break;
+            uint32_t ntv = TV, ntu = TU;
+            if (TU < 32 && 2 * TU <= umax) ntu = 2 * TU;
+            else if (TV < 64 && 2 * TV <= vmax) ntv = 2 * TV;
+            else break;
+            if (ntv * ntu > 1024) break;
+            uint32_t nch = 0;
+            const uint32_t nlds = need(ntv, ntu, &nch);
+            if (nlds > LDS_LIMIT || nch >= 65536u) break;
+            TV = ntv;
+            TU = ntu;
+            lds = nlds;
+            cap = 0;
+            for (uint32_t id : job_ids) cap = std::max(cap, stage_bytes(jobs[id], TV, TU, true));
+            cap = (cap + 15) & ~15u;
+        }
+    }
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
     st.alg_bytes = 0;
     // per synapse: the slice id unless one staged slice makes it implicit, the
     // offsets (1 B packed, 2 B otherwise), the strength in and the state out
-    const uint32_t syn_b = ((fast && nst <= 1) ? 0u : 1u) + (packed ? 1u : 2u) + 2u;
     for (uint32_t id : job_ids) {
         // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
         // out (pyramidal); per SSC cell its energy
         const TuftJob& j = jobs[id];
+        const uint32_t syn_b = ((fast && j.n_staged <= 1) ? 0u : 1u) + (packed ? 1u : 2u) + 2u;
         st.alg_bytes += (uint64_t)j.C * j.spt * syn_b + (uint64_t)j.C * j.dens * 4 +
                         (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0);
     }
@@ -1415,15 +1472,60 @@ int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len)
 
 // One plain cycle: every cycled layer's state parity flips once, then the
 // stages run on the area stream (and its side stream).
-int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial) {
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial,
+                const std::function<int(hipStream_t)>* intake, bool intake_first) {
     for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
+    if (intake && intake_first && (*intake)(a->stream)) return 1;
     if (enqueue_stages(a->stream, a->side, a->stages, serial,
                        [&](const Stage& st, hipStream_t s) {
                            return launch_area_stage(a, s, st, rnd_ofs);
+                       },
+                       [&](hipStream_t s) {
+                           return intake && !intake_first ? (*intake)(s) : 0;
                        }))
         return 1;
     // readback of the area's samplers, after the cycle on its stream
     return sample_after_cycle(a->sample_plan, a->stream, &a, 1, nullptr, nullptr);
+}
+
+// Slices a tuft's gather may read: its declared sources and every source id
+// its synapses hold.
+static void tuft_reads(const TuftRt& T, SliceSet& s) {
+    for (int k = 0; k < 4; k++) s.bits[k] |= T.slc_bits[k];
+    for (uint32_t k = 0; k < T.d.n_src_slcs && k < BSM_MAX_TUFT_SRCS; k++)
+        s.add_range(T.d.src_slc_ids[k], 1);
+}
+
+// Whether a cycle's side stream may start before the intake of the axon bytes
+// [off, off + len): side stages run pyramidal tufts only, so it may when no
+// pyramidal tuft reads a slice the intake writes.  The side tufts then stream
+// from the cycle's start while the intake copy runs on the area stream.
+static bool side_before_intake(const bsm_area* a, uint32_t off, uint32_t len) {
+    SliceSet in, rd;
+    for (uint32_t k = 0; k < a->n_slices; k++) {
+        const uint64_t lo = a->slices_h[k].idz;
+        const uint64_t hi = lo + (uint64_t)a->slices_h[k].v_size * a->slices_h[k].u_size;
+        if (lo < (uint64_t)off + len && off < hi) in.add_range(k, 1);
+    }
+    for (uint32_t l = 0; l < a->n_layers; l++)
+        if (!a->L[l].d.spatial)
+            for (uint32_t t = 0; t < a->L[l].d.n_tufts; t++) tuft_reads(a->L[l].t[t], rd);
+    return !in.meets(rd.bits);
+}
+
+// n_cycles plain cycles, each after its frame's intake when frames is set
+// (frame c at frames + c * stride, len bytes into the axon space at axn_offset).
+static int run_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* frames, uint64_t stride,
+                      uint32_t len, uint32_t n_cycles) {
+    const bool feed = frames && len;
+    const bool first = !feed || !side_before_intake(a, axn_offset, len);
+    for (uint32_t c = 0; c < n_cycles; c++) {
+        const std::function<int(hipStream_t)> intake = [&](hipStream_t s) {
+            return launch_intake(s, a->axn + axn_offset, frames + (size_t)c * stride, len);
+        };
+        if (cycle_fused(a, c * a->n_slots, false, feed ? &intake : nullptr, first)) return 1;
+    }
+    return 0;
 }
 
 // Upload n_cycles of rnd draws (user-given or drawn from the area stream) to
@@ -1465,9 +1567,7 @@ int bsm_area_cycle(bsm_area* a, const int32_t* rnd, uint32_t n_cycles) {
     if (a->batch_stream) return fail("area belongs to a batch; cycle it through the batch");
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (upload_rnd(a, rnd, n_cycles)) return 1;
-    for (uint32_t c = 0; c < n_cycles; c++)
-        if (cycle_fused(a, c * a->n_slots, false)) return 1;
-    return 0;
+    return run_cycles(a, 0, nullptr, 0, 0, n_cycles);
 }
 
 int bsm_cortex_cycle(bsm_area* const* areas, uint32_t n_areas, const uint8_t* const* dev_frames,
@@ -1507,14 +1607,7 @@ int bsm_area_run(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
         return fail("frame range out of the axon space");
     BSM_HIP(hipSetDevice(a->ctx->device));
     if (upload_rnd(a, nullptr, n_cycles)) return 1;
-    for (uint32_t c = 0; c < n_cycles; c++) {
-        if (dev_frames && frame_len &&
-            launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
-                          frame_len))
-            return 1;
-        if (cycle_fused(a, c * a->n_slots, false)) return 1;
-    }
-    return 0;
+    return run_cycles(a, axn_offset, dev_frames, frame_stride, frame_len, n_cycles);
 }
 
 int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_frames,
@@ -1527,13 +1620,7 @@ int bsm_area_time_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* dev_fr
     if (upload_rnd(a, nullptr, n_cycles)) return 1;
     BSM_HIP(hipStreamSynchronize(a->stream));
     BSM_HIP(hipEventRecord(a->ev0, a->stream));
-    for (uint32_t c = 0; c < n_cycles; c++) {
-        if (dev_frames && frame_len &&
-            launch_intake(a->stream, a->axn + axn_offset, dev_frames + (size_t)c * frame_stride,
-                          frame_len))
-            return 1;
-        if (cycle_fused(a, c * a->n_slots, false)) return 1;
-    }
+    if (run_cycles(a, axn_offset, dev_frames, frame_stride, frame_len, n_cycles)) return 1;
     BSM_HIP(hipEventRecord(a->ev1, a->stream));
     BSM_HIP(hipEventSynchronize(a->ev1));
     BSM_HIP(hipEventElapsedTime(ms_total, a->ev0, a->ev1));

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -187,15 +188,18 @@ int side_prepare(Side& s, const std::vector<Stage>& stages);
 void side_destroy(Side& s);
 
 // Enqueues one cycle's stages.  Side stages fork from `main` at this point and
-// record their join events; main stages wait for the side stage they name; any
-// side stage no main stage named is joined at the end, so work on `main` after
-// this call always follows every stage.  serial: all stages on `main` in order.
-template <class Launch>
+// record their join events; then pre_main(main) runs (work the main stages
+// need and the side stages do not touch, such as the cycle's intake), then the
+// main stages, each waiting for the side stage it names; any side stage no main
+// stage named is joined at the end, so work on `main` after this call always
+// follows every stage.  serial: pre_main and all stages on `main` in order.
+template <class Launch, class Pre>
 int enqueue_stages(hipStream_t main, Side& side, const std::vector<Stage>& stages, bool serial,
-                   Launch&& launch) {
+                   Launch&& launch, Pre&& pre_main) {
     bool any_side = false;
     for (const Stage& st : stages) any_side |= st.stream == 1;
     if (serial || !any_side) {
+        if (pre_main(main)) return 1;
         for (const Stage& st : stages)
             if (launch(st, main)) return 1;
         return 0;
@@ -210,6 +214,7 @@ int enqueue_stages(hipStream_t main, Side& side, const std::vector<Stage>& stage
         BSM_HIP(hipEventRecord(side.join[i], side.stream));
         joined[i] = 0;
     }
+    if (pre_main(main)) return 1;
     for (const Stage& st : stages) {
         if (st.stream == 1) continue;
         if (st.join >= 0 && (size_t)st.join < stages.size() && !joined[st.join]) {
@@ -221,6 +226,11 @@ int enqueue_stages(hipStream_t main, Side& side, const std::vector<Stage>& stage
     for (size_t i = 0; i < stages.size(); i++)
         if (!joined[i]) BSM_HIP(hipStreamWaitEvent(main, side.join[i], 0));
     return 0;
+}
+template <class Launch>
+int enqueue_stages(hipStream_t main, Side& side, const std::vector<Stage>& stages, bool serial,
+                   Launch&& launch) {
+    return enqueue_stages(main, side, stages, serial, launch, [](hipStream_t) { return 0; });
 }
 
 // Slices written during a cycle before a pyramidal layer's tufts run: the
@@ -328,7 +338,12 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
                uint32_t* n_stages);
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
                  const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
-int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial);
+// One plain cycle on the area stream (and its side stream).  intake, when
+// given, is launched on the area stream before the main stages; the side
+// stages start before it when intake_first is false.
+int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial,
+                const std::function<int(hipStream_t)>* intake = nullptr,
+                bool intake_first = true);
 // Device pointer and length of one buffer kind of the area (the current state
 // buffer for BSM_SYN_STATES; the static flag bits for BSM_SYN_FLAG_SETS).
 int resolve_buf(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len);

@@ -1,5 +1,6 @@
 """GPU parity: the HIP engine (through the C ABI) against the CPU oracle,
 bit-exact on every buffer.  Integer path: the bar is exact equality."""
+import ctypes as C
 import os
 
 import numpy as np
@@ -135,6 +136,41 @@ def test_cycle_config1_1000_cycles(ctx):
     assert np.count_nonzero(g.read("SYN_STRENGTHS", 1)) > 0
     assert np.count_nonzero(g.axon_states()[576:]) > 0
     g.close()
+
+
+def test_cycle_config1_multi_cycle_runs(ctx):
+    """Config 1 through multi-cycle calls (bsm_area_run: intake + cycle per
+    frame from device memory, rnd from the area's own stream; then
+    bsm_area_cycle of 40 cycles with caller rnd): every buffer equals the
+    oracle's run after each call, for calls of 2 to 150 cycles."""
+    scheme = recipes.config1_area()
+    g, o = make_pair(ctx, scheme, layout_seed=0)
+    # the twin area draws the same rnd stream the run calls consume
+    twin = CorticalArea(scheme, ctx, rnd_seed=0, init_layout=False)
+    off, ln = scheme.input_range()
+    frames = inputs.GlyphStream(side=24, p=0.15, hold=10, seed=0).frames(350)
+    dev = ctx.alloc(frames.nbytes)
+    ctx.upload(dev, frames)
+    done = 0
+    for k in (2, 3, 7, 50, 91, 150):
+        print("run", done, k, flush=True)
+        rnd = twin.draw_rnd(k)
+        g.run(C.c_void_p(dev.value + done * ln), ln, ln, k, offset=off)
+        for c in range(k):
+            o.intake(frames[done + c])
+            o.cycle(rnd[c * g.n_slots:(c + 1) * g.n_slots], 1)
+        done += k
+        g.sync()
+        compare(g, o, scheme, f"run to cycle {done}")
+    assert np.count_nonzero(g.read("SYN_STRENGTHS", 1)) > 0
+    rnd = twin.draw_rnd(40)
+    g.cycle(40, rnd)
+    for c in range(40):
+        o.cycle(rnd[c * g.n_slots:(c + 1) * g.n_slots], 1)
+    compare(g, o, scheme, "cycle(40)")
+    g.close()
+    twin.close()
+    ctx.free(dev)
 
 
 @pytest.mark.parametrize("tuning", [
