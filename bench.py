@@ -244,8 +244,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
-    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"),
-                    help="fixed column tile of the plain form instead of the autotuner's")
+    ap.add_argument("--autotune", action="store_true",
+                    help="plain form: tile chosen by the autotuner instead of the fixed default")
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[2, 16],
+                    help="column tile of the plain form (default 2 16: of the autotuner's "
+                         "candidates, those within 1%% of the best cycle time, it gives the "
+                         "fastest tuft launch)")
     ap.add_argument("--serial", action="store_true",
                     help="one stream per area/batch (no side stream); for profiler runs")
     ap.add_argument("--batched-only", action="store_true",
@@ -276,7 +280,7 @@ def main():
     dev = ctx.alloc(frames.nbytes)
     ctx.upload(dev, frames)
 
-    if args.tile:
+    if not args.autotune:
         tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
         area.set_tuning(**tuning)
     if args.serial:
@@ -291,7 +295,7 @@ def main():
         print("pmc probe done", flush=True)
         return
 
-    if not args.tile:
+    if args.autotune:
         # tile geometry by the project autotuner (timed on a scratch copy of the area)
         from bismit_amd.autotune import tune_area
         tiles = [(8, 8), (4, 8), (2, 8), (2, 16), (1, 32), (4, 16), (1, 16)]
