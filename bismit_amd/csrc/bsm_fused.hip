@@ -555,30 +555,48 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
 }
 
 // ------------------------------------------------------------------ F2
+// One chunk of LTP over W synapses from index i0 (prx_syns__active__mtp_ltd,
+// cl/bismit.cl:604-639, and dst_syns__active__mtpot_mtdep, :554-600): a synapse
+// whose state byte is nonzero is incremented with mask 0x7F, any other one
+// decremented with `dec_mask`; the seed is z + i with i the absolute synapse
+// index (Q5).  load() and apply() are split so that a thread can have several
+// chunks' loads in flight before it computes the first.
 template <int W>
-__device__ inline void ssc_ltp_chunk(const uint8_t* __restrict__ st, int8_t* __restrict__ str,
-                                     uint32_t z, uint32_t i0, int32_t rnd) {
-    constexpr int NB = W < 4 ? W : 4;
-    constexpr int NW = Chunk<W>::NW;
-    uint32_t ms[NW], mt[NW], out[NW];
-    Chunk<W>::load(st + i0, ms);
-    Chunk<W>::load(str + i0, mt);
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        uint32_t ow = 0;
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const uint32_t i = i0 + (uint32_t)(w * 4 + b);
-            const int32_t seed = (int32_t)(z + i);
-            int32_t s = sbyte_of(mt[w], b);
-            if (byte_of(ms[w], b)) s += rnd_inc(rnd, seed, s, 0x7F);
-            else s -= rnd_dec(rnd, seed, s, 0x1FF);
-            ow |= ((uint32_t)s & 0xFFu) << (8 * b);
-        }
-        out[w] = ow;
+struct LtpChunk {
+    static constexpr int NB = W < 4 ? W : 4;
+    static constexpr int NW = Chunk<W>::NW;
+    uint32_t ms[NW], mt[NW];
+    __device__ inline void load(const uint8_t* __restrict__ st, const int8_t* __restrict__ str,
+                                uint32_t i0) {
+        Chunk<W>::load(st + i0, ms);
+        Chunk<W>::load(str + i0, mt);
     }
-    Chunk<W>::store(str + i0, out);
-}
+    __device__ inline void apply(int8_t* __restrict__ str, uint32_t z, uint32_t i0, int32_t rnd,
+                                 uint32_t dec_mask) const {
+        uint32_t out[NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            uint32_t ow = 0;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const uint32_t i = i0 + (uint32_t)(w * 4 + b);
+                const int32_t seed = (int32_t)(z + i);
+                int32_t s = sbyte_of(mt[w], b);
+                if (byte_of(ms[w], b)) s += rnd_inc(rnd, seed, s, 0x7F);
+                else s -= rnd_dec(rnd, seed, s, dec_mask);
+                ow |= ((uint32_t)s & 0xFFu) << (8 * b);
+            }
+            out[w] = ow;
+        }
+        Chunk<W>::store(str + i0, out);
+    }
+};
+// LTP chunks each thread loads before it applies the first: four of the
+// narrow (4-byte) chunks batches of mixed dendrite lengths use, so that one
+// memory latency covers sixteen synapses; wide chunks one at a time (more in
+// flight cost registers and occupancy: measured slower).
+template <int W>
+constexpr int ltp_u() { return W <= 4 ? 4 : 1; }
 
 constexpr uint32_t F2_MAXC = 4;  // tile cells per thread (tiles up to 4 x 256 cells)
 
@@ -700,12 +718,26 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         __syncthreads();
 
         if (J.learn) {
-            const uint32_t na = n_act, spt = J.spt, CPC = spt / W;
+            // ssc_mtp_simple on the listed cells: state nonzero -> increment (0x7F),
+            // else decrement (0x1FF); ltp_u chunks per thread in flight at once (a
+            // chunk past the list loads chunk 0 again and is not applied)
+            const uint32_t na = n_act, spt = J.spt, CPC = spt / W, total = na * CPC;
             const uint8_t* __restrict__ st = J.st[p & 1];
-            for (uint32_t y = tid; y < na * CPC; y += nth) {
-                const uint32_t ci = y / CPC, k = y - ci * CPC;
-                const uint32_t z = list[ci] * spt;
-                ssc_ltp_chunk<W>(st, J.str, z, z + k * W, rnd_m);
+            constexpr int LU = ltp_u<W>();
+            for (uint32_t y0 = tid; y0 < total; y0 += LU * nth) {
+                LtpChunk<W> ch[LU];
+                uint32_t zz[LU], ii[LU];
+#pragma unroll
+                for (int j = 0; j < LU; j++) {
+                    const uint32_t y = y0 + j * nth < total ? y0 + j * nth : 0u;
+                    const uint32_t ci = y / CPC, k = y - ci * CPC;
+                    zz[j] = list[ci] * spt;
+                    ii[j] = zz[j] + k * W;
+                    ch[j].load(st, J.str, ii[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < LU; j++)
+                    if (y0 + j * nth < total) ch[j].apply(J.str, zz[j], ii[j], rnd_m, 0x1FF);
             }
         }
         __syncthreads();
@@ -713,31 +745,6 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
 }
 
 // ------------------------------------------------------------------ F3
-template <int W>
-__device__ inline void dst_ltp_chunk(const uint8_t* __restrict__ prev, int8_t* __restrict__ str,
-                                     uint32_t z, uint32_t i0, int32_t rnd) {
-    constexpr int NB = W < 4 ? W : 4;
-    constexpr int NW = Chunk<W>::NW;
-    uint32_t ms[NW], mt[NW], out[NW];
-    Chunk<W>::load(prev + i0, ms);
-    Chunk<W>::load(str + i0, mt);
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        uint32_t ow = 0;
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const uint32_t i = i0 + (uint32_t)(w * 4 + b);
-            const int32_t seed = (int32_t)(z + i);
-            int32_t s = sbyte_of(mt[w], b);
-            if (byte_of(ms[w], b)) s += rnd_inc(rnd, seed, s, 0x7F);
-            else s -= rnd_dec(rnd, seed, s, 0xFFF);
-            ow |= ((uint32_t)s & 0xFFu) << (8 * b);
-        }
-        out[w] = ow;
-    }
-    Chunk<W>::store(str + i0, out);
-}
-
 constexpr uint32_t F3_COLS = 256;
 constexpr uint32_t F3_G = 4;  // slices whose per-column inputs are loaded ahead of any store
 
@@ -870,12 +877,32 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                 const int32_t r0 = J.rnd[rnd_ofs + J.slot_mtp0];
                 const int32_t r1 = n_tufts > 1 ? J.rnd[rnd_ofs + J.slot_mtp0 + 1] : 0;
                 const int32_t r2 = n_tufts > 2 ? J.rnd[rnd_ofs + J.slot_mtp0 + 2] : 0;
-                for (uint32_t y = tid; y < nl * maxch; y += blockDim.x) {
-                    const uint32_t li = y / maxch, k = y - li * maxch;
-                    const uint32_t t = lt[li];
-                    if (k * W >= J.t_spd[t]) continue;
-                    const uint32_t z = lz[li];
-                    dst_ltp_chunk<W>(prev_st, J.str, z, z + k * W, t == 0 ? r0 : (t == 1 ? r1 : r2));
+                // tft_dst_mtp on the queued dendrites: previously active -> increment
+                // (0x7F), else decrement (0xFFF); ltp_u chunks per thread in flight
+                // at once (a chunk past a dendrite or the queue loads the queue's
+                // first chunk and is not applied)
+                const uint32_t total = nl * maxch;
+                constexpr int LU = ltp_u<W>();
+                for (uint32_t y0 = tid; y0 < total; y0 += LU * blockDim.x) {
+                    LtpChunk<W> ch[LU];
+                    uint32_t zz[LU], ii[LU], tt[LU];
+                    bool ok[LU];
+#pragma unroll
+                    for (int j = 0; j < LU; j++) {
+                        const uint32_t y = y0 + j * blockDim.x;
+                        const uint32_t li = y < total ? y / maxch : 0u;
+                        const uint32_t k = y < total ? y - li * maxch : 0u;
+                        tt[j] = lt[li];
+                        ok[j] = y < total && k * W < J.t_spd[tt[j]];
+                        zz[j] = lz[li];
+                        ii[j] = zz[j] + (ok[j] ? k * W : 0u);
+                        ch[j].load(prev_st, J.str, ii[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < LU; j++)
+                        if (ok[j])
+                            ch[j].apply(J.str, zz[j], ii[j],
+                                        tt[j] == 0 ? r0 : (tt[j] == 1 ? r1 : r2), 0xFFF);
                 }
             }
             __syncthreads();
