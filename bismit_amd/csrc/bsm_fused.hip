@@ -17,7 +17,9 @@
 // Synapse states are double-buffered: F1 writes st[par] and never reads the
 // old state; SYN_PREV_ACTIVE of the flags is "st[par^1] != 0", read by the
 // distal LTP of F3 and materialised only at readback.  F1 therefore streams
-// 3 B of metadata + 1 B strength in and 1 B state out per synapse.
+// per synapse the offsets (1 B packed, 2 B otherwise), the slice id (only when
+// the tuft's gather stages several slices) and the strength in, and the new
+// state out: 3-4 B.
 #include <algorithm>
 #include <cstring>
 
