@@ -290,8 +290,8 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
 template <int W, int NST, bool FAST, bool PACKED>
 __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                                                const WorkItem* __restrict__ items, uint32_t n_items,
-                                               uint32_t par, uint32_t rnd_ofs, uint32_t TV,
-                                               uint32_t TU, uint32_t stage_cap) {
+                                               uint32_t par, uint32_t rnd_ofs,
+                                               uint32_t stage_cap) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int NB = W < 4 ? W : 4;               // bytes per word
     constexpr int NW = Chunk<W>::NW;
@@ -302,6 +302,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         const WorkItem wi = items[it];
         const TuftJob& J = jobs[wi.job];
         const uint32_t V = J.V, U = J.U, depth = J.depth, spt = J.spt, halo = J.halo;
+        const uint32_t TV = J.tile_v, TU = J.tile_u;
         const uint32_t v0 = (uint32_t)wi.tv * TV, u0 = (uint32_t)wi.tu * TU;
         const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
         uint32_t p = par ^ J.par_xor;
@@ -1123,7 +1124,7 @@ static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fas
     return tot;
 }
 
-int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
+int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st) {
     uint32_t W = 16, nst = 0;
@@ -1158,49 +1159,55 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
         if (TV >= TU && TV > 1) TV = (TV + 1) / 2;
         else TU = (TU + 1) / 2;
     }
-    // A check-free item of under two chunks per lane spends most of its time in
-    // the per-item steps (window staging, the dendrite/tuft epilogue) rather
-    // than streaming synapses: grow such tiles (u first, since a tile row is
-    // contiguous in the synapse streams) while every job's item stays within
-    // its area, the LDS limit and the 2^16 chunk bound.  (Config 2's spiny
-    // stellate layer at a 2x16 tile: 15.2 us per launch; at 2x32: 12.6 us.)
+    // Every job takes the tile (TV, TU) ...
+    for (uint32_t id : job_ids) {
+        jobs[id].tile_v = TV;
+        jobs[id].tile_u = TU;
+    }
+    // ... except that a check-free item of under two chunks per lane spends
+    // most of its time in the per-item steps (window staging, the dendrite/tuft
+    // epilogue) rather than streaming synapses: such a job's tile grows (u
+    // first, since a tile row is contiguous in the synapse streams) while it
+    // stays within its area, the LDS limit and the 2^16 chunk bound.  Jobs of a
+    // launch may differ by 12x in synapses per column (a batch of areas), so
+    // each job grows on its own.  (Config 2's spiny stellate layer at a 2x16
+    // tile: 15.2 us per launch; at 2x32: 12.6 us.)
+    auto job_lds = [&](const TuftJob& j, uint32_t tv, uint32_t tu) {
+        return ((stage_bytes(j, tv, tu, fast) + 15) & ~15u) + 4 * j.depth * tv * tu * j.dens +
+               (j.tft[0] ? 2 * j.depth * tv * tu * j.dens : 0u);
+    };
     if (fast) {
-        auto need = [&](uint32_t tv, uint32_t tu, uint32_t* chunks) {
-            uint32_t cap2 = 0, accb = 0, denc = 0, ch = 0;
-            for (uint32_t id : job_ids) {
-                const TuftJob& j = jobs[id];
-                cap2 = std::max(cap2, stage_bytes(j, tv, tu, true));
-                accb = std::max(accb, 4 * j.depth * tv * tu * j.dens);
-                if (j.tft[0]) denc = std::max(denc, 2 * j.depth * tv * tu * j.dens);
-                ch = std::max(ch, j.depth * tv * tu * (j.spt / std::max(W, 1u)));
-            }
-            *chunks = ch;
-            return ((cap2 + 15) & ~15u) + accb + denc;
-        };
-        uint32_t vmax = 0, umax = 0;
         for (uint32_t id : job_ids) {
-            vmax = std::max(vmax, jobs[id].V);
-            umax = std::max(umax, jobs[id].U);
+            TuftJob& j = jobs[id];
+            const uint32_t per_col = j.depth * (j.spt / std::max(W, 1u));
+            for (;;) {
+                if (per_col * j.tile_v * j.tile_u >= 2 * 256) break;
+                uint32_t ntv = j.tile_v, ntu = j.tile_u;
+                if (ntu < 32 && 2 * ntu <= j.U) ntu *= 2;
+                else if (ntv < 64 && 2 * ntv <= j.V) ntv *= 2;
+                else break;
+                if (ntv * ntu > 1024 || per_col * ntv * ntu >= 65536u ||
+                    job_lds(j, ntv, ntu) > LDS_LIMIT)
+                    break;
+                j.tile_v = ntv;
+                j.tile_u = ntu;
+            }
         }
-        for (;;) {
-            uint32_t ch = 0;
-            need(TV, TU, &ch);
-            if (ch >= 2 * 256) break;
-            uint32_t ntv = TV, ntu = TU;
-            if (TU < 32 && 2 * TU <= umax) ntu = 2 * TU;
-            else if (TV < 64 && 2 * TV <= vmax) ntv = 2 * TV;
-            else break;
-            if (ntv * ntu > 1024) break;
-            uint32_t nch = 0;
-            const uint32_t nlds = need(ntv, ntu, &nch);
-            if (nlds > LDS_LIMIT || nch >= 65536u) break;
-            TV = ntv;
-            TU = ntu;
-            lds = nlds;
-            cap = 0;
-            for (uint32_t id : job_ids) cap = std::max(cap, stage_bytes(jobs[id], TV, TU, true));
-            cap = (cap + 15) & ~15u;
+    }
+    // the launch's LDS: the largest window (+ row table), sums and tuft bytes
+    {
+        uint32_t accb = 0, denc = 0;
+        cap = 0;
+        for (uint32_t id : job_ids) {
+            const TuftJob& j = jobs[id];
+            const uint32_t tiles = j.tile_v * j.tile_u;
+            cap = std::max(cap, stage_bytes(j, j.tile_v, j.tile_u, fast));
+            accb = std::max(accb, 4 * j.depth * tiles * j.dens);
+            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * tiles * j.dens);
         }
+        cap = (cap + 15) & ~15u;
+        lds = cap + accb + denc;
+        if (lds > LDS_LIMIT) return fail("fused tuft kernel does not fit in LDS");
     }
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
@@ -1217,7 +1224,7 @@ int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_i
     }
     for (uint32_t id : job_ids) {
         const TuftJob& j = jobs[id];
-        uint32_t ntv = (j.V + TV - 1) / TV, ntu = (j.U + TU - 1) / TU;
+        uint32_t ntv = (j.V + j.tile_v - 1) / j.tile_v, ntu = (j.U + j.tile_u - 1) / j.tile_u;
         if (ntv > 65535 || ntu > 65535 || id > 65535) return fail("too many tiles");
         for (uint32_t a = 0; a < ntv; a++)
             for (uint32_t b = 0; b < ntu; b++)
@@ -1408,14 +1415,14 @@ int build_plan(bsm_area* a) {
 #define F1_CASE(w, n)                                                                        \
     case (w * 8 + n):                                                                        \
         kf_tuft<w, n, false, false><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
-                                                                       par, rnd_ofs, st.tv, \
-                                                                       st.tu, st.stage_cap);\
+                                                                       par, rnd_ofs,         \
+                                                                       st.stage_cap);        \
         break;
 #define F1_FAST_CASE(w, n, pk)                                                               \
     case ((w * 8 + n) * 2 + pk):                                                             \
         kf_tuft<w, n, true, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
-                                                                        par, rnd_ofs, st.tv,\
-                                                                        st.tu, st.stage_cap);\
+                                                                        par, rnd_ofs,        \
+                                                                        st.stage_cap);       \
         break;
 
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,

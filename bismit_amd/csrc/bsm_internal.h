@@ -58,6 +58,7 @@ struct TuftJob {
     // host-side planning copies of the staged slices' geometry
     uint16_t st_vsize[4], st_usize[4];
     uint8_t st_vsc[4], st_usc[4];
+    uint32_t tile_v, tile_u;  // the job's column tile (set by the F1 planner)
 };
 
 // F2 work unit: SSC layer post-processing (inhibition + SSC LTP).
@@ -315,7 +316,7 @@ uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w);
 uint32_t ssc_chunk(const bsm_area* a, uint32_t l);
 // Appends the items of one stage and fills its geometry; jobs[j] indexes the
 // job arrays.  Returns 0 or an error.
-int plan_f1(const std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
+int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st);
 int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_ids, const Tuning& tn,
