@@ -209,6 +209,33 @@ def test_cycle_scalar_quirk_nonspatial(ctx):
     g.close()
 
 
+@pytest.mark.parametrize("shape", [
+    # rectangular area, the widest dendrites (256 synapses, the limit of the
+    # packed 16-bit sums) beside 4-synapse ones, two and three slices
+    dict(side=24, u_side=72, ssc_depth=2, pyr_depth=3, ssc_dens=2, ssc_spd=4, pyr_dens=8,
+         pyr_spd=256),
+    # an area of one 8 x 8 tile, three spatial slices
+    dict(side=8, ssc_depth=3, pyr_depth=1, ssc_dens=3, ssc_spd=8, pyr_dens=2, pyr_spd=12),
+])
+def test_cycle_extreme_shapes(ctx, shape):
+    """Planner edge cases against the oracle: per-job tiles where one tuft's
+    item holds a fraction of a chunk per lane (its tile grows, bounded by the
+    area) and another's would exceed the LDS or the chunk bound (its tile
+    shrinks), on rectangular and minimal areas; 12 cycles, every buffer after
+    every cycle."""
+    print("shape", shape, flush=True)
+    scheme = recipes.cortical_area("x", **shape)
+    g, o = make_pair(ctx, scheme)
+    plan = g.stage_plan()
+    print(plan, flush=True)
+    assert all(p["lds_bytes"] <= 64 * 1024 for p in plan)
+    frames = inputs.SdrStream(scheme.input_range()[1], 0.08, 0.5, seed=3).frames(12)
+    run_cycles(g, o, scheme, frames, 12, tag=str(shape))
+    assert np.count_nonzero(g.read("SYN_STRENGTHS", 0)) > 0
+    g.close()
+    o.close()
+
+
 def test_cycle_settings(ctx):
     """bypass_inhib (passthrough with rnd 0) and disable_learning."""
     for st in (L.SET_BYPASS_INHIB, L.SET_DISABLE_LEARNING, L.SET_DISABLE_PYRS):
