@@ -26,19 +26,22 @@ ALG_BASIS = ("8 B per synapse-update: 3 B source metadata + 1 B strength + 1 B p
              "state + 1 B new state + 2 B flags read and written (dendrite/tuft bytes excluded)")
 def engine_basis(syn_bs):
     per = " or ".join(f"{b} B" for b in sorted(set(syn_bs)))
-    return (f"bytes this engine must move: {per} per synapse by tuft (source slice id when the "
-            "tuft's gather stages several slices, offsets packed into 1 B when |v|,|u| <= 7, "
+    return (f"bytes this engine must move: {per} per synapse by tuft (a 2-bit staged-slice code "
+            "when the tuft's gather stages several slices, offsets packed into 1 B when |v|,|u| <= 7, "
             "strength, new state; the previous-active flag is derived from the previous state "
             "buffer) + 4 B per dendrite + 12 B per pyramidal cell-tuft")
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
 def syn_bytes(stage, tuft):
-    """Per-synapse HBM bytes of one tuft in an F1 launch: the slice id unless the
-    tuft's gather stages a single slice (then the id is implicit and its stream
-    is not read), the offsets (1 B packed, else 2 B), strength in and state
-    out."""
-    slc = 0 if (stage["fast"] and len(tuft.src_slcs) <= 1) else 1
+    """Per-synapse HBM bytes of one tuft in an F1 launch: the source slice (the
+    check-free gather reads a 2-bit staged-slice code when it stages several
+    slices and nothing for one; the checked gather the 1 B id), the offsets
+    (1 B packed, else 2 B), strength in and state out."""
+    if stage["fast"]:
+        slc = 0 if len(tuft.src_slcs) <= 1 else 0.25
+    else:
+        slc = 1
     return slc + (1 if stage["packed"] else 2) + 2
 
 
@@ -366,7 +369,7 @@ def main():
     traffic = read_traffic()
     # the calibration kernel with the launch's most read streams (syn_bytes - 1) and
     # its one write stream
-    ceil = stream_ceiling(ctx, n_reads=max(syn_bs) - 1)
+    ceil = stream_ceiling(ctx, n_reads=int(max(syn_bs)) - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": f"{kinds[dk]} ({scheme.layers[dl].name})",

@@ -153,6 +153,7 @@ static void plan_area(bsm_area* a, ArenaPlan& ap) {
         ap.add(&L.syn_v, ns);
         ap.add(&L.syn_u, ns);
         ap.add(&L.syn_vu, ns);
+        ap.add(&L.syn_sb, L.n_sb);
         ap.add(&L.syn_str, ns);
         ap.add(&L.den_states, nd);
         ap.add(&L.den_raw, nd);
@@ -173,13 +174,15 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
         LayerRt& L = a->L[l];
         L.d = d->layers[l];
         L.cells = (uint32_t)L.d.depth * d->v_size * d->u_size;
-        uint64_t ns = 0, nd = 0;
+        uint64_t ns = 0, nd = 0, nsb = 0;
         for (uint32_t t = 0; t < L.d.n_tufts; t++) {
             TuftRt& T = L.t[t];
             T.d = L.d.tufts[t];
             T.spt = T.d.dens_per_tft * T.d.syns_per_den;
             T.syn_idz = ns;
             T.den_idz = nd;
+            T.sb_idz = nsb;
+            nsb += ((uint64_t)L.cells * T.spt + 3) / 4;
             // zero-filled buffers: slice 0, offsets 0
             T.reach_v = T.reach_u = T.reach_max = 0;
             T.slc_bits[0] = 1;
@@ -189,6 +192,7 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
         }
         L.n_syn = ns;
         L.n_den = nd;
+        L.n_sb = nsb;
         L.n_celtft = (uint64_t)L.cells * L.d.n_tufts;
         L.cur = 0;
         L.slot0 = slot;
@@ -226,6 +230,24 @@ __global__ __launch_bounds__(256) void k_pack_vu(const int8_t* __restrict__ v,
         vu[i] = (uint8_t)((((v[i] + 8) & 15) << 4) | ((u[i] + 8) & 15));
 }
 
+// Staged-slice codes for the gather: byte b of a tuft holds (id - base) & 3 of
+// its synapses 4b..4b+3 (tuft-relative; bits 2k..2k+1 for synapse 4b+k), exact
+// when every id lies in [base, base + 4) (the planner reads them only then).
+// Bytes [b0, b1) of one tuft of n synapses.
+__global__ __launch_bounds__(256) void k_pack_sb(const uint8_t* __restrict__ slc,
+                                                 uint8_t* __restrict__ sb, uint64_t n,
+                                                 uint64_t b0, uint64_t b1, uint32_t base) {
+    for (uint64_t b = b0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < b1;
+         b += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t code = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint64_t i = 4 * b + k;
+            if (i < n) code |= ((slc[i] - base) & 3u) << (2 * k);
+        }
+        sb[b] = (uint8_t)code;
+    }
+}
+
 int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len) {
     LayerRt& L = a->L[layer];
     if (offset >= L.n_syn || !len) return 0;
@@ -233,6 +255,20 @@ int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, si
     const uint32_t grid = (uint32_t)std::min<uint64_t>((len + 255) / 256, 8192);
     k_pack_vu<<<grid, 256, 0, s>>>(L.syn_v + offset, L.syn_u + offset, L.syn_vu + offset, len);
     BSM_HIP(hipGetLastError());
+    // the codes of every tuft the range touches, in whole four-synapse groups
+    for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+        const TuftRt& T = L.t[t];
+        const uint64_t n = (uint64_t)L.cells * T.spt;
+        const uint64_t lo = std::max<uint64_t>(offset, T.syn_idz);
+        const uint64_t hi = std::min<uint64_t>(offset + len, T.syn_idz + n);
+        if (lo >= hi) continue;
+        uint32_t staged[4];
+        const uint32_t base = stage_slices(a, T, staged) ? staged[0] : 0u;
+        const uint64_t b0 = (lo - T.syn_idz) / 4, b1 = (hi - T.syn_idz + 3) / 4;
+        const uint32_t g = (uint32_t)std::min<uint64_t>((b1 - b0 + 255) / 256, 8192);
+        k_pack_sb<<<g, 256, 0, s>>>(L.syn_slc + T.syn_idz, L.syn_sb + T.sb_idz, n, b0, b1, base);
+        BSM_HIP(hipGetLastError());
+    }
     return 0;
 }
 
@@ -625,11 +661,11 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
         *p.first = a->arena + off;
         off += align_up(p.second ? p.second : 1, 256);
     }
-    // Buffers start zero-filled like Buffer::builder().fill_val(0); the packed
-    // offsets of v = u = 0 are 0x88.
+    // Buffers start zero-filled like Buffer::builder().fill_val(0); the derived
+    // gather streams follow from the zero offsets and slice ids.
     BSM_HIP(hipMemsetAsync(a->arena, 0, ap.total, a->stream));
     for (uint32_t l = 0; l < a->n_layers; l++)
-        if (a->L[l].n_syn) BSM_HIP(hipMemsetAsync(a->L[l].syn_vu, 0x88, a->L[l].n_syn, a->stream));
+        if (launch_pack_vu(a->stream, a, l, 0, a->L[l].n_syn)) return 1;
     BSM_HIP(hipMemcpyAsync(a->slices_d, a->slices_h.data(), sizeof(SliceRt) * a->n_slices,
                            hipMemcpyHostToDevice, a->stream));
     BSM_HIP(hipStreamSynchronize(a->stream));
@@ -815,7 +851,8 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
     }
     BSM_HIP(hipMemcpy(p + offset, src, len, hipMemcpyHostToDevice));
     layout_stats_update(a, buf, layer, offset, len, (const uint8_t*)src, -1);
-    if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS) {
+    if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS ||
+        buf == BSM_SYN_SRC_SLC_IDS) {
         if (launch_pack_vu(a->stream, a, layer, offset, len)) return 1;
         BSM_HIP(hipStreamSynchronize(a->stream));
     }
@@ -838,7 +875,8 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
         BSM_HIP(hipMemsetAsync(p, val, n, a->stream));
         // a whole-buffer fill replaces the summary of every tuft of the layer
         if (buf != BSM_AXN_STATES) layout_stats_update(a, buf, layer, 0, n, nullptr, val);
-        if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS)
+        if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS ||
+            buf == BSM_SYN_SRC_SLC_IDS)
             if (launch_pack_vu(a->stream, a, layer, 0, n)) return 1;
     }
     BSM_HIP(hipStreamSynchronize(a->stream));

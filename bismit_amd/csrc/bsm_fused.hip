@@ -17,9 +17,9 @@
 // Synapse states are double-buffered: F1 writes st[par] and never reads the
 // old state; SYN_PREV_ACTIVE of the flags is "st[par^1] != 0", read by the
 // distal LTP of F3 and materialised only at readback.  F1 therefore streams
-// per synapse the offsets (1 B packed, 2 B otherwise), the slice id (only when
-// the tuft's gather stages several slices) and the strength in, and the new
-// state out: 3-4 B.
+// per synapse the offsets (1 B packed, 2 B otherwise), a 2-bit staged-slice
+// code (only when the tuft's gather stages several slices) and the strength
+// in, and the new state out: 3-3.25 B.
 #include <algorithm>
 #include <cstring>
 
@@ -107,6 +107,14 @@ struct Chunk<1> {
     __device__ static inline void load(const void* p, uint32_t* o) { o[0] = *BSM_G(const uint8_t, p); }
     __device__ static inline void store(void* p, const uint32_t* o) { *BSM_G(uint8_t, p) = (uint8_t)o[0]; }
 };
+
+// The 2-bit staged-slice codes of a W-synapse chunk: W / 4 bytes at byte `off`.
+template <int W>
+__device__ inline uint32_t load_codes(Rsrc r, uint32_t off) {
+    if constexpr (W == 16) return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, BUF_NT);
+    else if constexpr (W == 8) return __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, BUF_NT);
+    else return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, BUF_NT);
+}
 
 // The epilogue's global inputs of one tile, loaded when the tile starts so that
 // their latency hides behind the synapse loop: the first TAIL_PD dendrites and
@@ -359,21 +367,21 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             // the window offset of that row's centre minus the first column's
             uint32_t* rowt = (uint32_t*)(lds + ((J.n_staged * WS + 15) & ~15u));
             // a job with one staged slice in a several-slice launch reads no
-            // slice ids (its stream below is empty, so every id reads as 0)
+            // slice codes (its stream below is empty, so every code reads as 0)
             const bool multi = NST > 1 && J.n_staged > 1;
-            const int32_t sid0ws = multi ? (int32_t)(J.staged[0] * WS) : 0;
             for (uint32_t rq = tid; rq < rows; rq += nth) {
                 const uint32_t sl = rq / tvn, v = v0 + (rq - sl * tvn);
                 rowt[2 * rq] = (sl * VU + v * U + u0) * spt;
                 rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * nc +
-                                              umid - b0 - sid0ws);
+                                              umid - b0);
             }
             __syncthreads();
             const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
             // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
             const uint32_t nrec = J.C * spt;
             // a zero-length stream: its loads return 0 without touching memory
-            const Rsrc r_slc = make_rsrc(J.slc, multi ? nrec : 0u);
+            // staged-slice codes: 2 bits per synapse (spt is a multiple of W >= 4)
+            const Rsrc r_slc = make_rsrc(J.sb, multi ? nrec / 4 : 0u);
             const Rsrc r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
             const Rsrc r_u = make_rsrc(J.uofs, nrec), r_str = make_rsrc(J.str, nrec);
             // packed offsets carry v + 8 and u + 8: their bias folds into the base
@@ -393,7 +401,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                 off0 = (rem - __umul24(cc, CPT)) * W;
                 so = rowt[2 * rowq] + rem * W;
                 lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
-                Chunk<W>::load_b(r_slc, so, ms);
+                if constexpr (NST > 1) ms[0] = load_codes<W>(r_slc, so >> 2);
                 Chunk<W>::load_b(r_v, so, mv);   // packed: (v + 8) << 4 | (u + 8) per byte
                 if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, mu);
                 Chunk<W>::load_b(r_str, so, mt);
@@ -420,11 +428,12 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                     int32_t ad1 = __builtin_amdgcn_sdot4((int32_t)p01, coefB, lbp, false);
                     int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lbp, false);
                     int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lbp, false);
-                    if (NST > 1) {
-                        ad0 += (int32_t)__umul24(byte_of(ms[w], 0), WS);
-                        ad1 += (int32_t)__umul24(byte_of(ms[w], 1), WS);
-                        ad2 += (int32_t)__umul24(byte_of(ms[w], 2), WS);
-                        ad3 += (int32_t)__umul24(byte_of(ms[w], 3), WS);
+                    if (NST > 1) {  // window of the staged slice: code x WS
+                        const uint32_t cw = ms[0] >> (8 * w);
+                        ad0 += (int32_t)__umul24(cw & 3u, WS);
+                        ad1 += (int32_t)__umul24((cw >> 2) & 3u, WS);
+                        ad2 += (int32_t)__umul24((cw >> 4) & 3u, WS);
+                        ad3 += (int32_t)__umul24((cw >> 6) & 3u, WS);
                     }
                     const uint32_t x = (uint32_t)lds[ad0] | ((uint32_t)lds[ad1] << 8) |
                                        ((uint32_t)lds[ad2] << 16) | ((uint32_t)lds[ad3] << 24);
@@ -937,6 +946,24 @@ uint32_t tuft_chunk(const bsm_area* a, uint32_t l, uint32_t t, uint32_t max_w) {
 
 uint32_t ssc_chunk(const bsm_area* a, uint32_t l) { return pick_w(a->L[l].t[0].spt, 0, 0, 16); }
 
+// Up to 4 distinct source slices, in the tuft's order, sorted; with the scalar
+// quirk only slices without mids (where char and int offsets agree).
+uint32_t stage_slices(const bsm_area* a, const TuftRt& T, uint32_t* staged) {
+    const bool quirk = (T.spt % 4) != 0;
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < T.d.n_src_slcs && n < 4; k++) {
+        const uint32_t sid = T.d.src_slc_ids[k];
+        bool dup = false;
+        for (uint32_t m = 0; m < n; m++) dup |= staged[m] == sid;
+        if (dup) continue;
+        const SliceRt& s = a->slices_h[sid];
+        if (quirk && (s.v_mid || s.u_mid)) continue;
+        staged[n++] = sid;
+    }
+    std::sort(staged, staged + n);
+    return n;
+}
+
 TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, const int32_t* rnd) {
     LayerRt& L = a->L[l];
     TuftRt& T = L.t[t];
@@ -947,6 +974,7 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     j.vofs = L.syn_v + T.syn_idz;
     j.uofs = L.syn_u + T.syn_idz;
     j.vu = L.syn_vu + T.syn_idz;
+    j.sb = L.syn_sb + T.sb_idz;
     j.str = L.syn_str + T.syn_idz;
     j.st[0] = L.syn_state[0] + T.syn_idz;
     j.st[1] = L.syn_state[1] + T.syn_idz;
@@ -972,21 +1000,8 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     j.scalar_quirk = (T.spt % 4) != 0;
     j.halo = tn.halo;
     j.par_xor = 0;
-    // Stage up to 4 distinct source slices; with the scalar quirk only slices
-    // without mids (where char and int offsets agree).
-    uint32_t n = 0;
-    for (uint32_t k = 0; k < T.d.n_src_slcs && n < 4; k++) {
-        uint32_t sid = T.d.src_slc_ids[k];
-        bool dup = false;
-        for (uint32_t m = 0; m < n; m++) dup |= j.staged[m] == sid;
-        if (dup) continue;
-        const SliceRt& s = a->slices_h[sid];
-        if (j.scalar_quirk && (s.v_mid || s.u_mid)) continue;
-        j.staged[n] = sid;
-        n++;
-    }
+    const uint32_t n = stage_slices(a, T, j.staged);
     j.n_staged = n;
-    std::sort(j.staged, j.staged + n);
     for (uint32_t k = 0; k < n; k++) {
         const SliceRt& s = a->slices_h[j.staged[k]];
         j.st_vsize[k] = s.v_size;
@@ -1212,14 +1227,15 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
     st.alg_bytes = 0;
-    // per synapse: the slice id unless one staged slice makes it implicit, the
-    // offsets (1 B packed, 2 B otherwise), the strength in and the state out
+    // per synapse, in quarter bytes: the slice (fast: a 2-bit code when several
+    // slices are staged, nothing for one; else the 1 B id), the offsets (1 B
+    // packed, 2 B otherwise), the strength in and the state out
     for (uint32_t id : job_ids) {
         // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
         // out (pyramidal); per SSC cell its energy
         const TuftJob& j = jobs[id];
-        const uint32_t syn_b = ((fast && j.n_staged <= 1) ? 0u : 1u) + (packed ? 1u : 2u) + 2u;
-        st.alg_bytes += (uint64_t)j.C * j.spt * syn_b + (uint64_t)j.C * j.dens * 4 +
+        const uint32_t syn_q = (fast ? (j.n_staged <= 1 ? 0u : 1u) : 4u) + (packed ? 4u : 8u) + 8u;
+        st.alg_bytes += (uint64_t)j.C * j.spt * syn_q / 4 + (uint64_t)j.C * j.dens * 4 +
                         (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0);
     }
     for (uint32_t id : job_ids) {

@@ -32,6 +32,8 @@ struct TuftJob {
     const int8_t* vofs;
     const int8_t* uofs;
     const uint8_t* vu;     // packed offsets: (v + 8) << 4 | (u + 8), valid when |v|, |u| <= 7
+    const uint8_t* sb;     // staged-slice codes: (slice id - staged[0]) & 3, 2 bits per
+                           // synapse, four to a byte (low bits first); valid when fast
     const int8_t* str;
     uint8_t* st[2];        // tuft base in the two state buffers; st[par] is written
     uint8_t* den_raw;      // tuft dendrite base
@@ -117,6 +119,7 @@ struct TuftRt {
     bsm_tuft_desc d;
     uint32_t spt;
     uint64_t syn_idz, den_idz;
+    uint64_t sb_idz;            // first byte of the tuft's staged-slice codes (LayerRt::syn_sb)
     // Host-side summary of the tuft's synapse sources, kept conservative on
     // every write (never shrinks): the largest |v/u offset| and the set of
     // source slice ids present.  Lets the planner prove that a staged LDS
@@ -132,6 +135,7 @@ struct LayerRt {
     bsm_layer_desc d;
     uint32_t cells;
     uint64_t n_syn, n_den, n_celtft;
+    uint64_t n_sb;           // bytes of syn_sb: ceil(synapses / 4) per tuft
     TuftRt t[BSM_MAX_TUFTS];
     uint8_t* syn_state[2];   // double-buffered synapse states
     int cur;                 // index of the current state buffer
@@ -140,6 +144,7 @@ struct LayerRt {
     int8_t* syn_v;
     int8_t* syn_u;
     uint8_t* syn_vu;         // packed copy of the offsets for the gather (see TuftJob::vu)
+    uint8_t* syn_sb;         // staged-slice codes of each tuft for the gather (see TuftJob::sb)
     int8_t* syn_str;
     uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;
     uint8_t* tft[8];
@@ -326,9 +331,13 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
 int build_plan(bsm_area* a);
 int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out);
 int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len);
-// Refreshes the packed offsets of synapses [offset, offset + len) of a layer from
-// its v and u offset arrays (after any write to them).
+// Refreshes the gather's derived streams of synapses [offset, offset + len) of a
+// layer after any write to their offsets or slice ids: the packed offsets, and
+// the staged-slice codes of every four-synapse group the range touches.
 int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len);
+// The source slices the fused gather stages for a tuft (at most 4, sorted; with
+// the scalar quirk only slices without mids).  Depends on the area's scheme only.
+uint32_t stage_slices(const bsm_area* a, const TuftRt& T, uint32_t* staged);
 constexpr uint32_t PACK_REACH = 7;   // |offset| bound of the packed form
 // Enqueues the deterministic regrowth of every tuft of the area on stream s and
 // widens the tufts' source summaries; *widened reports whether a summary grew

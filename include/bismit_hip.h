@@ -339,8 +339,9 @@ typedef struct bsm_stage_info {
                             the spatial layers' stages, joined before the layer's
                             pyramidal post stage) */
     uint64_t alg_bytes;  /* gather+dendrites+tufts stages: HBM bytes this launch must
-                            move: per synapse the source slice id (only for tufts
-                            whose gather stages several slices, or a checked gather), the offsets
+                            move: per synapse the source slice (a 2-bit staged-slice code
+                            for tufts whose gather stages several slices, the 1 B id in
+                            a checked gather), the offsets
                             (1 B packed, else 2 B), the strength and the new state; 4 B
                             per dendrite, 12 B per pyramidal cell-tuft, 1 B per SSC
                             cell; 0 for other stages */

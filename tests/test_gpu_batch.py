@@ -162,10 +162,10 @@ def test_config3_plan_uses_fast_wide_variant(ctx):
         assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
 
     def launch_bytes(p, pick):
-        # per synapse: slice id unless the tuft stages one slice, packed offsets,
-        # strength, state
+        # per synapse: a 2-bit staged-slice code unless the tuft stages one slice,
+        # packed offsets, strength, state
         def syn_b(t):
-            return (0 if len(t.src_slcs) <= 1 else 1) + 1 + 2
+            return (0 if len(t.src_slcs) <= 1 else 0.25) + 1 + 2
         return sum(s.cells(li) * (t.syns_per_tft * syn_b(t) + t.dens_per_tft * 4 +
                                   (12 if lay.kind == 2 else 0)) + (s.cells(li) if lay.kind == 1 else 0)
                    for s in schemes for li, lay in enumerate(s.layers)

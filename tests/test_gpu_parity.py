@@ -262,11 +262,11 @@ def test_cycle_config2_full_size(ctx):
         if p["kind"] == 1:
             assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
     # bytes of the pyramidal tuft launches, one tuft each: per synapse the packed
-    # offsets, strength and state, plus the slice id for the distal tuft (two
-    # staged slices); 4 B/dendrite, 12 B/cell-tuft
+    # offsets, strength and state, plus a 2-bit staged-slice code for the distal
+    # tuft (two staged slices); 4 B/dendrite, 12 B/cell-tuft
     cells = 2 * 256 * 256
     assert [p["alg_bytes"] for p in plan if p["kind"] == 1][1:] == \
-        [cells * 512 * b + cells * 8 * 4 + cells * 12 for b in (4, 3)]
+        [cells * 512 * b + cells * 8 * 4 + cells * 12 for b in (3.25, 3)]
     g.close()
     o.close()
 
@@ -308,6 +308,36 @@ def test_packed_offsets_follow_writes(ctx):
         g.fill(name, 0, 3)
         o.buf(name, 0)[:] = 3
     cycles("fill")
+    g.close()
+    o.close()
+
+
+def test_slice_codes_follow_writes(ctx):
+    """The gather's 2-bit staged-slice codes follow writes of the slice ids: a
+    write at an odd offset and length across a tuft boundary (partial groups of
+    four synapses at both ends), then a fill; each followed by cycles against
+    the oracle."""
+    s = recipes.cortical_area("sc", 32, pyr_depth=2, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                              pyr_spd=16)
+    g, o = make_pair(ctx, s)
+    rng = np.random.default_rng(21)
+    frames = inputs.SdrStream(32 * 32, 0.05, 0.5, seed=6).frames(24)
+    tufts = s.layers[1].tufts
+    assert any(len(set(t.src_slcs)) > 1 for t in tufts)
+    run_cycles(g, o, s, frames[:8], 8, tag="initial")
+    cells = s.cells(1)
+    t0 = cells * tufts[0].syns_per_tft
+    lo, n = t0 - 101, 203
+    ids = np.array([rng.choice(tufts[0 if i < t0 else 1].src_slcs) for i in range(lo, lo + n)],
+                   dtype=np.uint8)
+    g.write("SYN_SRC_SLC_IDS", 1, ids, lo)
+    o.write("SYN_SRC_SLC_IDS", 1, ids, lo)
+    assert all(p["packed"] == 1 for p in g.stage_plan() if p["kind"] == 1)
+    run_cycles(g, o, s, frames[8:16], 8, tag="partial write across tufts")
+    val = int(tufts[1].src_slcs[-1])
+    g.fill("SYN_SRC_SLC_IDS", 1, val)
+    o.buf("SYN_SRC_SLC_IDS", 1)[:] = val
+    run_cycles(g, o, s, frames[16:24], 8, tag="fill")
     g.close()
     o.close()
 
