@@ -249,10 +249,13 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--autotune", action="store_true",
                     help="plain form: tile chosen by the autotuner instead of the fixed default")
-    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[2, 16],
-                    help="column tile of the plain form (default 2 16: of the autotuner's "
-                         "candidates, those within 1%% of the best cycle time, it gives the "
-                         "fastest tuft launch)")
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[1, 32],
+                    help="column tile of the plain form (default 1 32: with the one-stream "
+                         "schedule 112.8 us per cycle against 114.5 us for 2x16 on the side "
+                         "stream, 115.0 us for 2x16 on one stream)")
+    ap.add_argument("--schedule", choices=("one", "side"), default="one",
+                    help="plain form: one stream (default), or the independent pyramidal "
+                         "tuft on a side stream")
     ap.add_argument("--serial", action="store_true",
                     help="one stream per area/batch (no side stream); for profiler runs")
     ap.add_argument("--batched-only", action="store_true",
@@ -286,8 +289,8 @@ def main():
     if not args.autotune:
         tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
         area.set_tuning(**tuning)
-    if args.serial:
-        area.set_tuning(overlap=2)
+    plain_overlap = 2 if (args.serial or args.schedule == "one") else 1
+    area.set_tuning(overlap=plain_overlap)
     if args.pmc_probe:
         n = 256 << 20
         a, b = ctx.alloc(n), ctx.alloc(n)
@@ -319,7 +322,7 @@ def main():
 
     # roofline of the dominant kernel: HIP events around every launch, launched one
     # at a time on the one-stream schedule (each layer's tufts in one launch, as
-    # in the committed rocprof summaries); the timed runs above use two streams
+    # in the committed rocprof summaries), whatever schedule the timed runs used
     n_prof = 50
     schedule = [(p["kind"], p["stream"]) for p in area.stage_plan()]
     area.set_tuning(overlap=2)
@@ -338,7 +341,7 @@ def main():
         g[0] += t
         g[1] += p["alg_bytes"]
         g[2] += 1
-    area.set_tuning(overlap=2 if args.serial else 1)
+    area.set_tuning(overlap=plain_overlap)
     # the host-buffer boundary: each cycle's frame handed over from host memory
     # (bsm_area_intake of a host frame, then bsm_area_cycle), PCIe included
     n_host = min(200, args.steps)
