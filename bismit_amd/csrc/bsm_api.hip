@@ -535,6 +535,15 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
     return 0;
 }
 
+// Buffer I/O on an area inside a batch first waits for the batch's enqueued
+// cycles.  bsm_batch_sync also copies a dynamic batch's per-layer state
+// parities (kept on the device between masked cycles) into the areas' host
+// copies, so the state buffer resolved afterwards is the current one.
+static int batch_settle(bsm_area* a) {
+    if (!a->batch) return 0;
+    return bsm_batch_sync(a->batch);
+}
+
 }  // namespace bsm
 
 using namespace bsm;
@@ -685,6 +694,7 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->items_d = nullptr;
     a->plan_dirty = true;
     a->batch_stream = nullptr;
+    a->batch = nullptr;
     a->layout_epoch = 0;
     a->sampler_gen = 0;
     a->sample_plan = nullptr;
@@ -804,13 +814,13 @@ int bsm_area_buf_len(bsm_area* a, int buf, uint32_t layer, size_t* len) {
 
 int bsm_area_read(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len, void* dst) {
     if (!a || (!dst && len)) return fail("null arg");
+    if (batch_settle(a)) return 1;
     uint8_t* p;
     size_t n;
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("read range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
     BSM_HIP(hipStreamSynchronize(a->stream));
-    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
     if (len == 0) return 0;
     BSM_HIP(hipMemcpy(dst, p + offset, len, hipMemcpyDeviceToHost));
     if (buf == BSM_SYN_FLAG_SETS) {
@@ -828,13 +838,13 @@ int bsm_area_read(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t le
 int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t len,
                    const void* src) {
     if (!a || (!src && len)) return fail("null arg");
+    if (batch_settle(a)) return 1;
     uint8_t* p;
     size_t n;
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("write range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
     BSM_HIP(hipStreamSynchronize(a->stream));
-    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
     if (len == 0) return 0;
     if (buf == BSM_SYN_FLAG_SETS) {
         LayerRt& L = a->L[layer];
@@ -861,11 +871,11 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
 
 int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
     if (!a) return fail("null area");
+    if (batch_settle(a)) return 1;
     uint8_t* p;
     size_t n;
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     BSM_HIP(hipSetDevice(a->ctx->device));
-    if (a->batch_stream) BSM_HIP(hipStreamSynchronize(a->batch_stream));
     if (buf == BSM_SYN_FLAG_SETS) {
         LayerRt& L = a->L[layer];
         BSM_HIP(hipMemsetAsync(p, val & ~SYN_PREV_ACTIVE, n, a->stream));

@@ -452,6 +452,7 @@ int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, 
         io.rnd = a->rnd_d;
         io.n_slots = a->n_slots;
         a->batch_stream = b->stream;
+        a->batch = b;
         b->io_parts = std::max(b->io_parts, (uint32_t)((a->axn_len + COPY_SEG - 1) / COPY_SEG));
     }
     BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -530,6 +531,7 @@ int bsm_batch_destroy(bsm_batch* b) {
     for (bsm_area* a : b->areas) {
         a->plan_dirty = true;
         a->batch_stream = nullptr;
+        a->batch = nullptr;
     }
     delete b;
     return 0;

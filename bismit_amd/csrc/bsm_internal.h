@@ -301,6 +301,8 @@ struct bsm_area {
     bool plan_dirty;
     hipEvent_t ev0, ev1;
     hipStream_t batch_stream;  // non-null while the area belongs to a batch
+    bsm_batch* batch;          // that batch (buffer I/O syncs it first, which also brings
+                               // a dynamic batch's device-side state parities to the host)
     uint32_t layout_epoch;     // bumped whenever the synapse source summary changes
     // readback (samplers): the area's open samplers, the generation of that
     // set, the plan serving the plain form, and the number of cycles done

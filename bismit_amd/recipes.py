@@ -234,13 +234,14 @@ def config2_area():
     return cortical_area("c2", 256, pyr_dens=8, pyr_spd=64)
 
 
-def config3_areas(n=64, seed=0):
+def config3_areas(n=64, seed=0, sides=(32, 192)):
     """64 uneven areas: sides log-uniform in [32, 192] rounded to multiples of 8,
-    1-3 slices per layer, 16-64 synapses per dendrite (multiples of 4)."""
+    1-3 slices per layer, 16-64 synapses per dendrite (multiples of 4).  `sides`
+    narrows the side range (the same recipe at oracle-checkable sizes)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        side = int(8 * round(math.exp(rng.uniform(math.log(32), math.log(192))) / 8))
+        side = int(8 * round(math.exp(rng.uniform(math.log(sides[0]), math.log(sides[1]))) / 8))
         side = max(8, side)
         sd = int(rng.integers(1, 4))
         pd = int(rng.integers(1, 4))

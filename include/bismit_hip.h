@@ -52,7 +52,8 @@ enum bsm_control_kind {
     BSM_CTRL_PYR_OUTPUT = 3
 };
 /* CorticalAreaSettings switches (src/cortex/cortical_area.rs:253-338).
- * Regrowth is always off in this engine (host-side, entropy-seeded upstream). */
+ * Regrowth is not a cycle setting here: the caller schedules it with
+ * bsm_area_regrow / bsm_batch_regrow (every 201 cycles upstream). */
 enum bsm_settings {
     BSM_SET_BYPASS_INHIB = 1u << 0,
     BSM_SET_DISABLE_PYRS = 1u << 1,
@@ -167,7 +168,10 @@ int bsm_area_syn_count(bsm_area* area, uint64_t* n);
 int bsm_area_slice_idz(bsm_area* area, uint32_t slc, uint32_t* idz);
 
 /* ---- buffer I/O (replaces Buffer read/write/fill and the test traits,
- *      src/cortex/synapses.rs:536-549, src/cortex/axon_space.rs:650-668) ---- */
+ *      src/cortex/synapses.rs:536-549, src/cortex/axon_space.rs:650-668).
+ *      On an area inside a batch these first wait for the batch's enqueued
+ *      cycles (bsm_batch_sync), so they see the state after the last cycle
+ *      enqueued, static or masked. ---- */
 int bsm_area_buf_len(bsm_area* area, int buf, uint32_t layer, size_t* len);
 int bsm_area_read(bsm_area* area, int buf, uint32_t layer, size_t offset, size_t len,
                   void* host_dst);

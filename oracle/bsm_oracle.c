@@ -256,9 +256,8 @@ int orc_buf_ptr(orc_area* a, int buf, uint32_t layer, void** ptr, size_t* len) {
  * offset, which wraps.  The host uses the scalar kernel only when
  * syns_per_tft % 4 != 0 (src/cortex/synapses.rs:261-268).  A slice id beyond
  * the table is undefined upstream; here it reads as 0. */
-static inline uint8_t axn_state(const orc_area* a, uint32_t slc, int32_t v_id, int8_t v_ofs,
-                                int32_t u_id, int8_t u_ofs, int scalar_quirk) {
-    if (slc >= a->n_slices) return 0;
+static inline uint32_t axn_index(const orc_area* a, uint32_t slc, int32_t v_id, int8_t v_ofs,
+                                 int32_t u_id, int8_t u_ofs, int scalar_quirk, int* safe) {
     const bsm_slice_desc* s = &a->slc[slc];
     int32_t vsz = (int32_t)s->v_size, usz = (int32_t)s->u_size;
     int32_t vo, uo;
@@ -272,9 +271,27 @@ static inline uint8_t axn_state(const orc_area* a, uint32_t slc, int32_t v_id, i
     int32_t vs = (int32_t)(((uint32_t)v_id * s->v_scale) >> SLC_SCL_COEFF_L2);
     int32_t us = (int32_t)(((uint32_t)u_id * s->u_scale) >> SLC_SCL_COEFF_L2);
     int32_t vv = vs + vo, uu = us + uo;
-    int safe = (vv >= 0) & (vv < vsz) & (uu >= 0) & (uu < usz);
-    if (!safe) return 0;
-    return a->axn[a->idz[slc] + (uint32_t)(vv * usz + uu)];
+    *safe = (vv >= 0) & (vv < vsz) & (uu >= 0) & (uu < usz);
+    return a->idz[slc] + (uint32_t)(vv * usz + uu);
+}
+
+static inline uint8_t axn_state(const orc_area* a, uint32_t slc, int32_t v_id, int8_t v_ofs,
+                                int32_t u_id, int8_t u_ofs, int scalar_quirk) {
+    if (slc >= a->n_slices) return 0;
+    int safe;
+    uint32_t idx = axn_index(a, slc, v_id, v_ofs, u_id, u_ofs, scalar_quirk, &safe);
+    return safe ? a->axn[idx] : 0;
+}
+
+uint8_t orc_axn_state(orc_area* a, uint32_t slc, uint32_t v_id, int8_t v_ofs, uint32_t u_id,
+                      int8_t u_ofs, int32_t scalar_quirk, uint32_t* idx, int32_t* safe) {
+    int sf = 0;
+    uint32_t i = 0;
+    if (slc < a->n_slices)
+        i = axn_index(a, slc, (int32_t)v_id, v_ofs, (int32_t)u_id, u_ofs, scalar_quirk, &sf);
+    if (idx) *idx = i;
+    if (safe) *safe = sf;
+    return sf ? a->axn[i] : 0;
 }
 
 /* axn_idx_3d_unsafe with zero offsets (the cell's own axon), scalar form:

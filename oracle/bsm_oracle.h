@@ -39,6 +39,14 @@ int orc_slice_idz(orc_area* a, uint32_t slc, uint32_t* idz);
 void orc_set_threads(int n);
 int orc_rnd_slots(orc_area* a, uint32_t* n);
 
+/* axn_state_3d_safe (cl/bismit.cl:397-418) of one synapse source: the state
+ * the gather reads for a cell at (v_id, u_id) with offsets (v_ofs, u_ofs) into
+ * slice slc; scalar_quirk selects the scalar index form (char-wrapped mids,
+ * cl/bismit.cl:323-363) over the vec4 form (int mids, :366-391).  *idx gets the
+ * axon index (also when unsafe) and *safe the bounds test.  Test probe. */
+uint8_t orc_axn_state(orc_area* a, uint32_t slc, uint32_t v_id, int8_t v_ofs, uint32_t u_id,
+                      int8_t u_ofs, int32_t scalar_quirk, uint32_t* idx, int32_t* safe);
+
 /* kernels (same argument meaning as the bsm_k_* entry points) */
 int orc_k_syn_flags(orc_area* a, uint32_t layer);
 int orc_k_syn_cycle(orc_area* a, uint32_t layer, uint32_t tuft);

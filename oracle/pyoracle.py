@@ -64,6 +64,9 @@ def load():
         for n in ["inhib_simple", "inhib_passthrough"]:
             getattr(lib, "orc_k_" + n).argtypes = [P, C.c_uint32, C.c_int32]
         lib.orc_k_ssc_mtp.argtypes = [P, C.c_uint32, C.c_uint32]
+        lib.orc_axn_state.restype = C.c_uint8
+        lib.orc_axn_state.argtypes = [P, C.c_uint32, C.c_uint32, C.c_int8, C.c_uint32, C.c_int8,
+                                      C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
         lib.orc_area_cycle.argtypes = [P, C.POINTER(C.c_int32), C.c_uint32]
         lib.orc_area_regrow.argtypes = [P, C.c_uint64, C.c_uint64, C.c_int32]
         lib.orc_retina.argtypes = [P, C.c_uint32, C.c_uint32, P]
@@ -165,6 +168,14 @@ class OracleArea:
         """Deterministic regrowth of every synapse with strength <= -25."""
         if self.lib.orc_area_regrow(self.h, seed, epoch, reach):
             raise RuntimeError("orc_area_regrow failed")
+
+    def axn_state(self, slc, v_id, v_ofs, u_id, u_ofs, scalar_quirk):
+        """(state, axon index, safe) the gather reads for one synapse source."""
+        idx = C.c_uint32()
+        safe = C.c_int32()
+        st = self.lib.orc_axn_state(self.h, slc, v_id, v_ofs, u_id, u_ofs, scalar_quirk,
+                                    C.byref(idx), C.byref(safe))
+        return int(st), idx.value, safe.value
 
     def k(self, name, *args):
         rc = getattr(self.lib, "orc_k_" + name)(self.h, *args)

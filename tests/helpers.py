@@ -40,6 +40,39 @@ def load_layout(area_like, scheme, layout):
         area_like.write("SYN_SRC_COL_U_OFFS", li, u)
 
 
+def make_pair(ctx, scheme, layout_seed=0, reach=6, rnd_seed=0):
+    """A device area and an oracle area of the same scheme and synapse layout."""
+    from bismit_amd.cortex import CorticalArea
+    from oracle import pyoracle as P
+    g = CorticalArea(scheme, ctx, rnd_seed=rnd_seed, init_layout=False)
+    o = P.OracleArea(scheme)
+    lay = host_layout(scheme, layout_seed, reach)
+    load_layout(g, scheme, lay)
+    load_layout(o, scheme, lay)
+    return g, o
+
+
+def run_cycles(g, o, scheme, frames, n, rng_seed=0, every=1, tag=""):
+    """n plain cycles on both sides with the device area's own rnd draws;
+    every buffer compared after every `every` cycles and after the last."""
+    rnd = g.draw_rnd(n)
+    for c in range(n):
+        if frames is not None:
+            g.intake(frames[c])
+            o.intake(frames[c])
+        r = rnd[c * g.n_slots:(c + 1) * g.n_slots]
+        g.cycle(1, r)
+        o.cycle(r, 1)
+        if (c + 1) % every == 0 or c == n - 1:
+            compare(g, o, scheme, f"{tag} cycle {c}")
+
+
+def area_rnd(seed, n_slots, n_cycles):
+    """The draws an area created with rnd_seed=seed consumes over n_cycles
+    (its xorshift128 stream), cycle-major, as int32."""
+    return L.xorshift128(seed, n_slots * n_cycles).view(np.int32).reshape(n_cycles, n_slots)
+
+
 def compare(gpu, orc, scheme, tag="", synapses=True):
     """Assert every buffer of the area is identical; return the list compared."""
     names = []

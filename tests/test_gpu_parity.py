@@ -11,20 +11,11 @@ from bismit_amd import inputs, recipes
 from bismit_amd.cortex import CorticalArea
 from oracle import pyoracle as P
 
-from helpers import compare, host_layout, load_layout, layer_bufs
+from helpers import compare, layer_bufs, make_pair, run_cycles
 
 pytestmark = pytest.mark.gpu
 
 P.set_threads(min(16, os.cpu_count() or 1))
-
-
-def make_pair(ctx, scheme, layout_seed=0, reach=6, rnd_seed=0):
-    g = CorticalArea(scheme, ctx, rnd_seed=rnd_seed, init_layout=False)
-    o = P.OracleArea(scheme)
-    lay = host_layout(scheme, layout_seed, reach)
-    load_layout(g, scheme, lay)
-    load_layout(o, scheme, lay)
-    return g, o
 
 
 def randomize(g, o, scheme, rng, strengths=True):
@@ -104,19 +95,6 @@ def test_kernels_one_by_one(ctx, which):
             o.k(name, *args)
             compare(g, o, scheme, f"{which} after {name}{args}")
     g.close()
-
-
-def run_cycles(g, o, scheme, frames, n, rng_seed=0, every=1, tag=""):
-    rnd = g.draw_rnd(n)
-    for c in range(n):
-        if frames is not None:
-            g.intake(frames[c])
-            o.intake(frames[c])
-        r = rnd[c * g.n_slots:(c + 1) * g.n_slots]
-        g.cycle(1, r)
-        o.cycle(r, 1)
-        if (c + 1) % every == 0 or c == n - 1:
-            compare(g, o, scheme, f"{tag} cycle {c}")
 
 
 def test_cycle_config1_1000_cycles(ctx):

@@ -713,3 +713,186 @@ def test_retina_against_a_convolution():
     want = np.where(off, np.maximum(avg - (img + 48), 0), np.maximum(img - (avg + 48), 0))
     assert np.array_equal(out, want.ravel().astype(np.uint8))
     o.close()
+
+
+# ------------------------------------------ axon index forms (scalar vs vec4)
+
+
+def test_axn_state_scalar_equals_vec4_on_spatial_slices():
+    """Reference invariant (src/tests/kernels.rs:21-91 with the test kernels of
+    cl/tests.cl:15-85): the scalar and vec4 axon index functions agree on every
+    spatial slice (mid 0), for every cell and every offset in [-8, 8] -- on a
+    local slice (scale 16) and on a slice mapped from a smaller area (scale 6)."""
+    sch = recipes.cortical_area("q", 16, input_side=6, ssc_dens=1, ssc_spd=4, pyr_dens=1,
+                                pyr_spd=4, pyr_depth=1, distal=False)
+    assert [s.v_scale for s in sch.slices] == [6, 16, 16]
+    o = P.OracleArea(sch)
+    rng = np.random.default_rng(30)
+    axn = o.buf("AXN_STATES")
+    axn[:] = rng.integers(1, 256, axn.size).astype(np.uint8)
+    n_safe = n_unsafe = 0
+    for slc in range(3):
+        for v in range(16):
+            for u in range(16):
+                for vo in range(-8, 9):
+                    for uo in range(-8, 9):
+                        a = o.axn_state(slc, v, vo, u, uo, 0)
+                        b = o.axn_state(slc, v, vo, u, uo, 1)
+                        assert a == b, (slc, v, u, vo, uo, a, b)
+                        n_safe += a[2]
+                        n_unsafe += 1 - a[2]
+    assert n_safe > 0 and n_unsafe > 0
+
+
+def test_axn_state_char_wrapped_mid_hand_cases():
+    """Non-spatial slice of 230 x 230 (scale 0, mid 115): the scalar index adds
+    the mid into the char offset (cl/bismit.cl:331-332), which wraps past 127,
+    while vec4 adds in int (:373-374).  Expected indices worked by hand:
+    row * 230 + col, e.g. 127 * 230 + 110 = 29320; 115 + 13 = 128 wraps to
+    -128 in char (off the slice)."""
+    sch = recipes.cortical_area("w", 16, ssc_dens=1, ssc_spd=4, pyr_dens=1, pyr_spd=4,
+                                pyr_depth=1, distal=False)
+    sch.slices[0] = recipes.SliceScheme.nonspatial(230, 230)
+    o = P.OracleArea(sch)
+    axn = o.buf("AXN_STATES")
+    axn[:230 * 230] = (np.arange(230 * 230) % 255 + 1).astype(np.uint8)
+    # (v_ofs, u_ofs) -> (int-form index or None when off the slice,
+    #                    scalar-form index or None)
+    cases = {
+        (0, 0): (26565, 26565),           # (115, 115)
+        (12, -5): (29320, 29320),         # (127, 110): no wrap yet
+        (13, 0): (29555, None),           # int (128, 115); char -128
+        (20, 0): (31165, None),           # int (135, 115); char -121
+        (114, 0): (52785, None),          # int (229, 115); char -27
+        (0, 13): (26578, None),           # u wraps the same way
+        (-115, -115): (0, 0),             # (0, 0)
+        (-116, 0): (None, None),          # row -1 in both forms
+        (-128, 0): (None, None),          # -13 in both forms
+        (127, 0): (None, None),           # int 242 >= 230; char -14
+    }
+    for (vo, uo), want in cases.items():
+        for quirk in (0, 1):
+            for v_id, u_id in ((0, 0), (5, 9), (15, 15)):   # scale 0: the cell position drops out
+                st, idx, safe = o.axn_state(0, v_id, vo, u_id, uo, quirk)
+                w = want[quirk]
+                if w is None:
+                    assert safe == 0 and st == 0, (vo, uo, quirk)
+                else:
+                    assert safe == 1 and idx == w and st == w % 255 + 1, (vo, uo, quirk, idx)
+
+
+# ---------------------------------------------------------- tft_set_syn_flags
+
+
+def test_syn_flags_every_byte_and_state():
+    """tft_set_syn_flags (cl/syns.cl:23-33): for every flag byte, with the
+    synapse's previous state 0 and not 0, only bit 4 (SYN_PREV_ACTIVE, 0x10)
+    changes, and it ends up set exactly when the state is not 0; states are
+    untouched."""
+    sch = tiny_area(pyr_spd=8, dens=2)          # layer 1: 512 cells x 2 tufts x 16 synapses
+    o = P.OracleArea(sch)
+    fl = o.buf("SYN_FLAG_SETS", 1)
+    st = o.buf("SYN_STATES", 1)
+    assert fl.size >= 1024
+    fl[:] = 0
+    st[:] = 0
+    fl[:512] = np.tile(np.arange(256), 2)
+    st[256:512] = np.array([1, 128, 255, 200] * 64, dtype=np.uint8)
+    st_before = st.copy()
+    o.k("syn_flags", 1)
+    for i in range(512):
+        byte, active = i % 256, i >= 256
+        want = (byte | 0x10) if active else (byte & 0xEF)
+        assert fl[i] == want, (i, fl[i], want)
+    assert np.array_equal(st, st_before)
+    assert np.all(fl[512:] == 0)
+
+
+# ------------------------------------------- inhib_passthrough and pyr_output
+
+
+def test_inhib_passthrough_is_inhib_simple_of_radius_zero():
+    """inhib_passthrough (cl/control.cl:194-217) writes each cell's soma to its
+    own axon, unmasked, and rates activity with seed soma | cell and decay 768.
+    inhib_simple at radius 0 (pinned by brute force above) tests a cell only
+    against itself (distance 0: s*9 >= s*8), so it must give identical axons
+    and activities for the same rnd."""
+    side = 16
+    rng = np.random.default_rng(40)
+    sch_p = tiny_area(side=side, radius=4)
+    sch_s = tiny_area(side=side, radius=0)
+    op, os_ = P.OracleArea(sch_p), P.OracleArea(sch_s)
+    soma = rng.choice(np.array([0, 0, 0, 1, 50, 200, 255], np.uint8), side * side)
+    act = rng.integers(0, 256, side * side).astype(np.uint8)
+    axn0 = rng.integers(0, 256, op.buf("AXN_STATES").size).astype(np.uint8)
+    for o in (op, os_):
+        o.buf("AXN_STATES")[:] = axn0
+        o.buf("DEN_STATES", 0)[:side * side] = soma     # the SSC soma aliases dendrite states
+        o.buf("SOMA_ACTIVITIES", 0)[:] = act
+    for rnd in (0, -12345, 987654321):
+        op.k("inhib_passthrough", 0, rnd)
+        os_.k("inhib_simple", 0, rnd)
+        assert np.array_equal(op.read("AXN_STATES"), os_.read("AXN_STATES"))
+        assert np.array_equal(op.read("SOMA_ACTIVITIES", 0), os_.read("SOMA_ACTIVITIES", 0))
+    # closed form: the layer's axon slice is the soma, every other axon untouched
+    idz, _ = sch_p.slice_idzs()
+    got = op.read("AXN_STATES")
+    assert np.array_equal(got[idz[1]:idz[1] + side * side], soma)
+    keep = np.ones(got.size, bool)
+    keep[idz[1]:idz[1] + side * side] = False
+    assert np.array_equal(got[keep], axn0[keep])
+    # activities move by at most one step per call and only up where the soma fires
+    o1 = P.OracleArea(sch_p)
+    o1.buf("DEN_STATES", 0)[:side * side] = soma
+    o1.buf("SOMA_ACTIVITIES", 0)[:] = act
+    o1.k("inhib_passthrough", 0, 777)
+    d = o1.read("SOMA_ACTIVITIES", 0).astype(int) - act.astype(int)
+    assert np.all(np.abs(d) <= 1)
+    assert np.all(d[soma == 0] <= 0)
+
+
+def test_pyr_output_writes_the_soma_unmasked():
+    """pyr_output (cl/control.cl:298-314): every slice of the layer's axons
+    equals its soma, including zeros, and nothing else changes.  With one slice
+    it equals inhib_intra_column (pinned by the reset-quirk cases above), whose
+    only slice is always the column's best."""
+    rng = np.random.default_rng(41)
+    for depth in (1, 2, 3):
+        sch = tiny_area(pyr_depth=depth)
+        o = P.OracleArea(sch)
+        axn0 = rng.integers(0, 256, o.buf("AXN_STATES").size).astype(np.uint8)
+        o.buf("AXN_STATES")[:] = axn0
+        soma = rng.choice(np.array([0, 0, 3, 90, 190], np.uint8), sch.cells(1))
+        o.buf("SOMA_STATES", 1)[:] = soma
+        o.k("pyr_output", 1)
+        off, n = sch.layer_axn_range(1)
+        got = o.read("AXN_STATES")
+        assert np.array_equal(got[off:off + n], soma)
+        keep = np.ones(got.size, bool)
+        keep[off:off + n] = False
+        assert np.array_equal(got[keep], axn0[keep])
+        if depth == 1:
+            o2 = P.OracleArea(sch)
+            o2.buf("AXN_STATES")[:] = axn0
+            o2.buf("SOMA_STATES", 1)[:] = soma
+            o2.k("inhib_intra_column", 1)
+            assert np.array_equal(o2.read("AXN_STATES"), got)
+
+
+# ------------------------------------------- activity rating: decrement rule
+
+
+def test_activity_rating_decrement_tests_the_updated_rating(lib):
+    """update_activity_rating (cl/bismit.cl:531-540) decrements on the rating
+    as just incremented.  From a = 0 with the cell active and decay 40000, the
+    result is 1 with probability (255/256) * (1 - 40000/65536) = 0.3886 when
+    the decrement tests the updated rating, and 255/256 = 0.9961 if it tested
+    the old one (0 never decrements).  2^16 seeds: sigma = 0.0019."""
+    n = 1 << 16
+    ones = sum(lib.orc_update_activity_rating(0, 1, 4242, s, 40000) for s in range(n))
+    p = ones / n
+    assert abs(p - (255 / 256) * (1 - 40000 / 65536)) < 0.012, p
+    # and from a = 1 inactive the rate is decay/65536 whatever a was before
+    # (no increment can happen): 2^16 trials at 40000 expect 40000 decrements
+    dec = sum(1 - lib.orc_update_activity_rating(1, 0, 4242, s, 40000) for s in range(n))
+    assert abs(dec - 40000) < 600, dec
