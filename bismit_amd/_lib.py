@@ -107,6 +107,7 @@ SIGNATURES = [
     ("bsm_area_buf_len", C.c_int, [_P, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)]),
     ("bsm_area_read", C.c_int, [_P, C.c_int, C.c_uint32, C.c_size_t, C.c_size_t, _P]),
     ("bsm_area_intake_filtered", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P, C.c_int]),
+    ("bsm_area_intake_filtered_noalloc", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P]),
     ("bsm_k_retina", C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
     ("bsm_area_set_smoother", C.c_int, [_P, C.c_uint32, C.c_int32]),
     ("bsm_k_smooth", C.c_int, [_P, C.c_uint32]),

@@ -575,6 +575,10 @@ int bsm_ctx_create(int device, bsm_ctx** out) {
         delete c;
         return hip_fail(e, "hipGetDeviceProperties");
     }
+    if (filter_scratch_setup(device)) {
+        delete c;
+        return 1;
+    }
     *out = c;
     return 0;
 }

@@ -20,11 +20,36 @@
 // race-free launches, one per group, in two launches.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bsm_internal.h"
 
 namespace bsm {
+
+// Staging of bsm_area_intake_filtered_noalloc: a file-scope device array, so
+// that filtering a host frame allocates nothing.  Each device holds its own copy
+// for the life of the process.  Host code never names the array in a launch or
+// copy: it uses the device address hipGetSymbolAddress returned for the device
+// (scratch_ptr), taken once per device by filter_scratch_setup.
+static __device__ __align__(256) uint8_t scratch[FILTER_SCRATCH_MAX];
+static std::mutex scratch_mu;
+static uint8_t* scratch_ptr[MAX_DEVICES] = {};
+
+int filter_scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return fail("device index beyond the staging table");
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (scratch_ptr[device]) return 0;
+    int cur = -1;
+    BSM_HIP(hipGetDevice(&cur));
+    if (cur != device) return fail("filter_scratch_setup: device is not current");
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail("hipGetSymbolAddress(scratch) returned null");
+    scratch_ptr[device] = (uint8_t*)p;
+    return 0;
+}
 
 constexpr int RETINA_THRESHOLD = 48;
 constexpr int RETINA_SMALL_R = 3;
@@ -298,6 +323,32 @@ int bsm_k_retina(bsm_area* a, uint32_t slc, uint32_t depth, const uint8_t* dev_s
     if (check_retina(a, slc, depth, &V, &U)) return 1;
     BSM_HIP(hipSetDevice(a->ctx->device));
     return launch_retina(a, slc, depth, V, U, dev_src);
+}
+
+int bsm_area_intake_filtered_noalloc(bsm_area* a, int filter, uint32_t slc, uint32_t depth,
+                                     const uint8_t* src) {
+    if (!a || !src) return fail("null arg");
+    if (a->batch_stream) return fail("area belongs to a batch");
+    if (filter != BSM_FILTER_RETINA) return fail("unknown filter");
+    uint32_t V, U;
+    if (check_retina(a, slc, depth, &V, &U)) return 1;
+    const size_t bytes = (size_t)depth * V * U;
+    if (bytes > FILTER_SCRATCH_MAX)
+        return fail("filtered frame of " + std::to_string(bytes) +
+                    " bytes exceeds the static staging maximum of " +
+                    std::to_string(FILTER_SCRATCH_MAX) + " bytes");
+    const int dev = a->ctx->device;
+    BSM_HIP(hipSetDevice(dev));
+    uint8_t* stage = (dev >= 0 && dev < MAX_DEVICES) ? scratch_ptr[dev] : nullptr;
+    if (!stage) return fail("static staging is not set up on this device");
+    // work still queued on the area may read its axons; the staging copy is
+    // rewritten whole, so nothing in it needs clearing
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    BSM_HIP(hipMemcpy(stage, src, bytes, hipMemcpyHostToDevice));
+    if (launch_retina(a, slc, depth, V, U, stage)) return 1;
+    // the next call on this device may overwrite the staging copy
+    BSM_HIP(hipStreamSynchronize(a->stream));
+    return 0;
 }
 
 int bsm_area_intake_filtered(bsm_area* a, int filter, uint32_t slc, uint32_t depth,

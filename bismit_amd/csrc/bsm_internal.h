@@ -371,6 +371,13 @@ void samplers_detach(bsm_area* a);
 void frame_pool_destroy(bsm_ctx* c);
 // Activity smoother and sensory filter (bsm_filters.hip).
 constexpr uint32_t SMOOTH_GROUPS = 6;
+// Static staging of bsm_area_intake_filtered_noalloc (bytes per device) and the
+// devices whose staging address is kept.
+constexpr size_t FILTER_SCRATCH_MAX = BSM_FILTER_STATIC_MAX;
+constexpr int MAX_DEVICES = 64;
+// Takes the device address of the static staging array on `device`, which must
+// be current; once per device, thread-safe.
+int filter_scratch_setup(int device);
 int launch_smoother(bsm_area* a, uint32_t layer, hipStream_t s);
 void smoother_free(LayerRt& L);
 // A stage of the area's plain plan on stream s (smoother stages included).

@@ -196,6 +196,16 @@ enum bsm_filter_kind { BSM_FILTER_RETINA = 1 };
  * copied to a device staging buffer). */
 int bsm_area_intake_filtered(bsm_area* area, int filter, uint32_t slc, uint32_t depth,
                              const uint8_t* src, int src_is_device);
+/* bsm_area_intake_filtered from a host frame without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (code under graph capture, tight serving loops).  The frame is staged in a
+ * static device array of BSM_FILTER_STATIC_MAX bytes per device (held until the
+ * process ends) instead of the area's lazily allocated staging buffer; larger
+ * frames are refused with an error.  Completes on return.  Calls that overlap
+ * on one device (from several host threads) race on the shared staging array. */
+#define BSM_FILTER_STATIC_MAX (4u << 20)
+int bsm_area_intake_filtered_noalloc(bsm_area* area, int filter, uint32_t slc, uint32_t depth,
+                                     const uint8_t* src);
 /* The retina kernel alone on a device frame (cl/filters.cl:34-100). */
 int bsm_k_retina(bsm_area* area, uint32_t slc, uint32_t depth, const uint8_t* dev_src);
 /* Attaches an ActivitySmoother control layer with field radius `radius` to

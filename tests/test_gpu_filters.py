@@ -12,7 +12,7 @@ from bismit_amd.cortex import Batch, CorticalArea
 from oracle import pyoracle as P
 
 from helpers import compare
-from test_gpu_parity import make_pair
+from helpers import make_pair
 
 pytestmark = pytest.mark.gpu
 
