@@ -35,6 +35,7 @@ BUF_NAMES = [
 ]
 BUF = {n: i for i, n in enumerate(BUF_NAMES)}
 FILTER_RETINA = 1
+FRAME_SDR = 1 << 63          # bsm_batch_cycle_masked: the offset names an SDR entry
 SIGNED_BUFS = {"SYN_STRENGTHS", "SYN_SRC_COL_V_OFFS", "SYN_SRC_COL_U_OFFS"}
 
 
@@ -106,6 +107,14 @@ SIGNATURES = [
     ("bsm_area_slice_idz", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("bsm_area_buf_len", C.c_int, [_P, C.c_int, C.c_uint32, C.POINTER(C.c_size_t)]),
     ("bsm_area_read", C.c_int, [_P, C.c_int, C.c_uint32, C.c_size_t, C.c_size_t, _P]),
+    ("bsm_sdr_create", C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(_P)]),
+    ("bsm_sdr_destroy", C.c_int, [_P]),
+    ("bsm_sdr_info", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("bsm_sdr_next", C.c_int, [_P, _P]),
+    ("bsm_sdr_frames", C.c_int, [_P, C.c_uint32, _P]),
+    ("bsm_sdr_next_packed", C.c_int, [_P, C.c_uint32, _P, _P]),
+    ("bsm_sdr_next_indices", C.c_int, [_P, _P]),
+    ("bsm_sdr_next_packed_sdr", C.c_int, [_P, C.c_uint32, _P, _P]),
     ("bsm_area_intake_filtered", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P, C.c_int]),
     ("bsm_area_intake_filtered_noalloc", C.c_int, [_P, C.c_int, C.c_uint32, C.c_uint32, _P]),
     ("bsm_k_retina", C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
@@ -166,6 +175,10 @@ SIGNATURES = [
     ("bsm_xorshift128_fill", C.c_int, [C.c_uint64, C.POINTER(C.c_uint32), C.c_uint64]),
     ("bsm_batch_create", C.c_int, [_P, C.POINTER(_P), C.c_uint32, C.POINTER(_P)]),
     ("bsm_batch_create_ex", C.c_int, [_P, C.POINTER(_P), C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    ("bsm_batch_add_areas", C.c_int, [_P, C.POINTER(_P), C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32)]),
+    ("bsm_batch_remove_areas", C.c_int, [_P, C.POINTER(_P), C.c_uint32]),
+    ("bsm_batch_busy_ms", C.c_int, [_P, C.POINTER(C.c_double)]),
     ("bsm_batch_set_input_ranges", C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("bsm_batch_cycle_masked", C.c_int, [_P, _U8P, _P, C.POINTER(C.c_uint64), C.c_uint64,
                                          C.c_int]),

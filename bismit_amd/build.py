@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libbismit_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["bsm_api.hip", "bsm_kernels.hip", "bsm_fused.hip", "bsm_batch.hip", "bsm_sample.hip",
-           "bsm_filters.hip"]
+           "bsm_filters.hip", "bsm_inputs.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

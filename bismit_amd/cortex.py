@@ -311,23 +311,52 @@ class Batch:
         self.lib = self.ctx.lib
         self.areas = list(areas)
         self.dynamic = dynamic
-        arr = (C.c_void_p * len(areas))(*[a.h.value for a in areas])
+        arr = (C.c_void_p * max(1, len(areas)))(*[a.h.value for a in areas])
         self.h = C.c_void_p()
         L.check(self.lib.bsm_batch_create_ex(self.ctx.h, arr, len(areas),
                                              L.BATCH_DYNAMIC if dynamic else 0, C.byref(self.h)))
-        if dynamic:
+        if dynamic and areas:
             n = len(areas)
             rng = [a.scheme.input_range() for a in areas]
             of = (C.c_uint32 * n)(*[r[0] for r in rng])
             ln = (C.c_uint32 * n)(*[r[1] for r in rng])
             L.check(self.lib.bsm_batch_set_input_ranges(self.h, of, ln))
 
+    def add(self, areas):
+        """Dynamic batches: the areas join (after the enqueued cycles); they take
+        their frames into their schemes' input ranges."""
+        areas = list(areas)
+        if any(a.scheme.input_filter for a in areas):
+            raise ValueError("areas with a sensory filter take their input in the plain form")
+        n = len(areas)
+        if not n:
+            return
+        arr = (C.c_void_p * n)(*[a.h.value for a in areas])
+        rng = [a.scheme.input_range() for a in areas]
+        of = (C.c_uint32 * n)(*[r[0] for r in rng])
+        ln = (C.c_uint32 * n)(*[r[1] for r in rng])
+        L.check(self.lib.bsm_batch_add_areas(self.h, arr, n, of, ln))
+        self.areas.extend(areas)
+
+    def remove(self, areas):
+        """Dynamic batches: the areas leave (after the enqueued cycles) and
+        continue in the plain form; the others keep their order."""
+        areas = list(areas)
+        if not areas:
+            return
+        arr = (C.c_void_p * len(areas))(*[a.h.value for a in areas])
+        L.check(self.lib.bsm_batch_remove_areas(self.h, arr, len(areas)))
+        gone = set(id(a) for a in areas)
+        self.areas = [a for a in self.areas if id(a) not in gone]
+
     def cycle_masked(self, active, frames, frame_ofs, graph=True):
         """One shared cycle of the areas with active[i] != 0; frames is one packed
-        uint8 array, area i's frame at frame_ofs[i] (-1: none)."""
+        uint8 array, area i's frame at frame_ofs[i] (-1: none; a uint64 offset
+        with L.FRAME_SDR set names an SDR entry: uint32 count, then indices)."""
         active = np.ascontiguousarray(active, dtype=np.uint8)
         frames = np.ascontiguousarray(frames, dtype=np.uint8)
-        fo = np.ascontiguousarray(frame_ofs, dtype=np.int64).view(np.uint64)
+        fo = np.asarray(frame_ofs)
+        fo = np.ascontiguousarray(fo if fo.dtype == np.uint64 else fo.astype(np.int64).view(np.uint64))
         L.check(self.lib.bsm_batch_cycle_masked(
             self.h, L.u8ptr(active), L.vptr(frames) if frames.size else None,
             fo.ctypes.data_as(C.POINTER(C.c_uint64)), frames.size, 1 if graph else 0))

@@ -76,6 +76,24 @@ __global__ __launch_bounds__(256) void kb_prologue(const AreaIo* __restrict__ io
     const uint8_t* src = nullptr;
     if (A.dyn) {
         const uint64_t fo = *A.frame_ofs;
+        if (A.len && fo != ~0ull && (fo & BSM_FRAME_SDR)) {
+            // SDR entry: a count, then that many axon indices of the input range
+            // (checked on the host); this block clears its segment, then sets
+            // the active axons that fall in it
+            const uint32_t* e = (const uint32_t*)(A.stage + (fo & ~BSM_FRAME_SDR));
+            const uint32_t lo = part * COPY_SEG;
+            if (lo >= A.len) return;
+            const uint32_t seg = min(COPY_SEG, A.len - lo);
+            block_zero(A.dst + lo, seg, threadIdx.x, blockDim.x);
+            __syncthreads();
+            const uint32_t n = e[0];
+            auto d8 = BSM_G(uint8_t, A.dst);
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+                const uint32_t x = e[1 + i] - lo;
+                if (x < seg) d8[lo + x] = 255;
+            }
+            return;
+        }
         if (A.len && fo != ~0ull) src = A.stage + fo;
     } else if (A.len && A.n_frames) {
         src = A.frames + (size_t)(counter[par & 1] % A.n_frames) * A.stride;
@@ -142,6 +160,14 @@ struct bsm_batch {
     uint32_t tract_parts = 1;   // tract blocks per tract (longest tract / COPY_SEG)
     std::vector<uint32_t> epochs;   // areas' layout epochs the plan was made for
     bsm::SamplePlan* sample_plan = nullptr;   // samplers of the batched areas
+    // dynamic batches: each area's input range (kept across membership changes)
+    std::vector<uint32_t> in_ofs, in_len;
+    size_t area_cap = 0;        // areas the per-area device arrays hold
+    // device time of masked cycles: events around each cycle, one pair per
+    // transfer slot, added up once the slot comes round again
+    hipEvent_t busy_ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    bool busy_rec[2] = {false, false};
+    double busy_ms = 0.0;
 };
 
 namespace bsm {
@@ -343,6 +369,7 @@ static bool layouts_changed(const bsm_batch* b) {
 }
 
 static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = nullptr) {
+    if (b->areas.empty()) return 0;   // a dynamic batch between its last and next member
     if (!b->dirty && layouts_changed(b)) {
         // a layout write may have changed which gather path is exact: re-plan
         // against the areas' current parities
@@ -382,17 +409,19 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = 
 // Device transfer buffer of a dynamic batch: [act n | pad to 16][frame_ofs n][frames].
 static size_t xfer_hdr(size_t n) { return ((n + 15) & ~(size_t)15) + 8 * n; }
 
-static int grow_xfer(bsm_batch* b, size_t frame_bytes) {
+static int grow_xfer(bsm_batch* b, size_t frame_bytes, bool repoint = false) {
     const size_t n = b->areas.size();
     const size_t need = xfer_hdr(n) + frame_bytes;
-    if (need <= b->xfer_cap) return 0;
-    const size_t cap = std::max(need, b->xfer_cap * 2);
-    BSM_HIP(hipStreamSynchronize(b->stream));
-    if (b->xfer_d) BSM_HIP(hipFree(b->xfer_d));
-    b->xfer_d = nullptr;
-    BSM_HIP(hipMalloc((void**)&b->xfer_d, cap));
-    BSM_HIP(hipMemset(b->xfer_d, 0, cap));
-    b->xfer_cap = cap;
+    if (need <= b->xfer_cap && !repoint) return 0;
+    if (need > b->xfer_cap) {
+        const size_t cap = std::max(need, b->xfer_cap * 2);
+        BSM_HIP(hipStreamSynchronize(b->stream));
+        if (b->xfer_d) BSM_HIP(hipFree(b->xfer_d));
+        b->xfer_d = nullptr;
+        BSM_HIP(hipMalloc((void**)&b->xfer_d, cap));
+        BSM_HIP(hipMemset(b->xfer_d, 0, cap));
+        b->xfer_cap = cap;
+    }
     const size_t a0 = (n + 15) & ~(size_t)15;
     for (size_t i = 0; i < n; i++) {
         b->io[i].act = b->xfer_d + i;
@@ -401,6 +430,95 @@ static int grow_xfer(bsm_batch* b, size_t frame_bytes) {
     }
     if (b->io_d)
         BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Adds the device time of the masked cycle last recorded in transfer slot k.
+static int busy_harvest(bsm_batch* b, int k) {
+    if (!b->busy_rec[k]) return 0;
+    BSM_HIP(hipEventSynchronize(b->busy_ev[k][1]));
+    float ms = 0.f;
+    BSM_HIP(hipEventElapsedTime(&ms, b->busy_ev[k][0], b->busy_ev[k][1]));
+    b->busy_ms += ms;
+    b->busy_rec[k] = false;
+    return 0;
+}
+
+static int check_joinable(bsm_batch* b, bsm_area* a, const std::vector<bsm_area*>& others) {
+    if (!a || a->ctx != b->ctx) return fail("area not on this context");
+    if (a->batch_stream) return fail("area already belongs to a batch");
+    for (bsm_area* o : others)
+        if (o == a) return fail("duplicate area in batch");
+    if ((a->axn_len + COPY_SEG - 1) / COPY_SEG > 65535)
+        return fail("axon space too large for the batch prologue");
+    for (uint32_t l = 0; l < a->n_layers; l++)
+        if (a->L[l].smooth_n) return fail("areas with an activity smoother cycle in the plain form");
+    return 0;
+}
+
+// Loads the per-area device state of the batch from its areas' host state: the
+// xorshift128 words, the dynamic parity words, the AreaIo entries (which keep the
+// static inputs or the dynamic input ranges) and the transfer layout.  Device
+// arrays only grow.  The batch stream must be idle; the plan becomes stale.
+static int load_areas(bsm_batch* b, const std::vector<AreaIo>& keep_io) {
+    const size_t n = b->areas.size();
+    if (n > b->area_cap) {
+        const size_t cap = std::max(n, b->area_cap * 2);
+        (void)hipFree(b->rng_d);
+        (void)hipFree(b->dyn_d);
+        (void)hipFree(b->io_d);
+        b->rng_d = nullptr;
+        b->dyn_d = nullptr;
+        b->io_d = nullptr;
+        BSM_HIP(hipMalloc((void**)&b->rng_d, sizeof(uint32_t) * 4 * cap));
+        if (b->dynamic) BSM_HIP(hipMalloc((void**)&b->dyn_d, sizeof(uint32_t) * BSM_MAX_LAYERS * cap));
+        BSM_HIP(hipMalloc((void**)&b->io_d, sizeof(AreaIo) * cap));
+        b->area_cap = cap;
+    }
+    std::vector<uint32_t> rng(4 * n);
+    std::vector<uint32_t> dw(b->dynamic ? (size_t)BSM_MAX_LAYERS * n : 0, 0);
+    b->io.assign(n, AreaIo{});
+    b->io_parts = 1;
+    for (size_t i = 0; i < n; i++) {
+        bsm_area* a = b->areas[i];
+        if (ensure_rnd(a, std::max<uint32_t>(a->n_slots, 1))) return 1;
+        std::memcpy(&rng[4 * i], a->rng, sizeof(a->rng));
+        AreaIo& io = b->io[i];
+        if (i < keep_io.size()) io = keep_io[i];
+        io.rng = b->rng_d + 4 * i;
+        io.rnd = a->rnd_d;
+        io.n_slots = a->n_slots;
+        b->io_parts = std::max(b->io_parts, (uint32_t)((a->axn_len + COPY_SEG - 1) / COPY_SEG));
+        if (b->dynamic) {
+            uint32_t mask = 0;
+            for (uint32_t l = 0; l < a->n_layers; l++) {
+                const bool spatial = a->L[l].d.spatial != 0;
+                const bool on = spatial ? !(a->settings & BSM_SET_DISABLE_SSCS)
+                                        : !(a->settings & BSM_SET_DISABLE_PYRS);
+                dw[i * BSM_MAX_LAYERS + l] = (uint32_t)a->L[l].cur;
+                if (on) mask |= 1u << l;
+            }
+            io.layer_mask = mask;
+            io.dyn = b->dyn_d + i * BSM_MAX_LAYERS;
+            io.dst = a->axn + b->in_ofs[i];
+            io.len = b->in_len[i];
+        }
+    }
+    if (n) {
+        BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+        if (b->dynamic)
+            BSM_HIP(hipMemcpy(b->dyn_d, dw.data(), dw.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice));
+    }
+    // dynamic batches: point the entries at the transfer buffer (uploads io_d)
+    if (b->dynamic) {
+        if (grow_xfer(b, 1 << 16, true)) return 1;
+    } else if (n) {
+        BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n, hipMemcpyHostToDevice));
+    }
+    free_graphs(b);
+    b->dirty = true;
     return 0;
 }
 
@@ -417,77 +535,109 @@ int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm
 int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, uint32_t flags,
                         bsm_batch** out) {
     if (!ctx || !out || (!areas && n_areas)) return fail("null arg");
-    if (n_areas == 0 || n_areas > 65535) return fail("n_areas must be in [1, 65535]");
-    for (uint32_t i = 0; i < n_areas; i++) {
-        if (!areas[i] || areas[i]->ctx != ctx) return fail("area not on this context");
-        if (areas[i]->batch_stream) return fail("area already belongs to a batch");
-        for (uint32_t k = 0; k < i; k++)
-            if (areas[k] == areas[i]) return fail("duplicate area in batch");
-        if ((areas[i]->axn_len + COPY_SEG - 1) / COPY_SEG > 65535)
-            return fail("axon space too large for the batch prologue");
-        for (uint32_t l = 0; l < areas[i]->n_layers; l++)
-            if (areas[i]->L[l].smooth_n)
-                return fail("areas with an activity smoother cycle in the plain form");
-    }
+    const bool dynamic = (flags & BSM_BATCH_DYNAMIC) != 0;
+    if ((n_areas == 0 && !dynamic) || n_areas > 65535)
+        return fail("n_areas must be in [1, 65535] ([0, 65535] for dynamic batches)");
     BSM_HIP(hipSetDevice(ctx->device));
     bsm_batch* b = new bsm_batch();
     b->ctx = ctx;
-    b->areas.assign(areas, areas + n_areas);
+    b->dynamic = dynamic;
+    for (uint32_t i = 0; i < n_areas; i++) {
+        if (check_joinable(b, areas[i], b->areas)) {
+            delete b;
+            return 1;
+        }
+        b->areas.push_back(areas[i]);
+    }
     BSM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     BSM_HIP(hipEventCreate(&b->ev0));
     BSM_HIP(hipEventCreate(&b->ev1));
-    BSM_HIP(hipMalloc((void**)&b->rng_d, sizeof(uint32_t) * 4 * n_areas));
     BSM_HIP(hipMalloc((void**)&b->counter_d, 2 * sizeof(uint32_t)));
     BSM_HIP(hipMemset(b->counter_d, 0, 2 * sizeof(uint32_t)));
-    std::vector<uint32_t> rng(4 * n_areas);
-    b->io.resize(n_areas);
-    for (uint32_t i = 0; i < n_areas; i++) {
-        bsm_area* a = areas[i];
+    for (int i = 0; i < 2; i++) {
+        BSM_HIP(hipEventCreateWithFlags(&b->xfer_ev[i], hipEventDisableTiming));
+        BSM_HIP(hipEventCreate(&b->busy_ev[i][0]));
+        BSM_HIP(hipEventCreate(&b->busy_ev[i][1]));
+    }
+    for (bsm_area* a : b->areas) {
         BSM_HIP(hipStreamSynchronize(a->stream));
-        if (ensure_rnd(a, std::max<uint32_t>(a->n_slots, 1))) return 1;
-        std::memcpy(&rng[4 * i], a->rng, sizeof(a->rng));
-        AreaIo& io = b->io[i];
-        io = AreaIo{};
-        io.rng = b->rng_d + 4 * i;
-        io.rnd = a->rnd_d;
-        io.n_slots = a->n_slots;
         a->batch_stream = b->stream;
         a->batch = b;
-        b->io_parts = std::max(b->io_parts, (uint32_t)((a->axn_len + COPY_SEG - 1) / COPY_SEG));
     }
-    BSM_HIP(hipMemcpy(b->rng_d, rng.data(), rng.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    for (int i = 0; i < 2; i++) BSM_HIP(hipEventCreateWithFlags(&b->xfer_ev[i], hipEventDisableTiming));
-    if (flags & BSM_BATCH_DYNAMIC) {
-        b->dynamic = true;
-        std::vector<uint32_t> dw((size_t)BSM_MAX_LAYERS * n_areas, 0);
-        for (uint32_t i = 0; i < n_areas; i++) {
-            bsm_area* a = areas[i];
-            uint32_t mask = 0;
-            for (uint32_t l = 0; l < a->n_layers; l++) {
-                const bool spatial = a->L[l].d.spatial != 0;
-                const bool on = spatial ? !(a->settings & BSM_SET_DISABLE_SSCS)
-                                        : !(a->settings & BSM_SET_DISABLE_PYRS);
-                dw[(size_t)i * BSM_MAX_LAYERS + l] = (uint32_t)a->L[l].cur;
-                if (on) mask |= 1u << l;
-            }
-            b->io[i].layer_mask = mask;
-        }
-        BSM_HIP(hipMalloc((void**)&b->dyn_d, dw.size() * sizeof(uint32_t)));
-        BSM_HIP(hipMemcpy(b->dyn_d, dw.data(), dw.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-        for (uint32_t i = 0; i < n_areas; i++) b->io[i].dyn = b->dyn_d + (size_t)i * BSM_MAX_LAYERS;
-        if (grow_xfer(b, 1 << 16)) return 1;
-    }
-    BSM_HIP(hipMalloc((void**)&b->io_d, sizeof(AreaIo) * n_areas));
-    BSM_HIP(hipMemcpy(b->io_d, b->io.data(), sizeof(AreaIo) * n_areas, hipMemcpyHostToDevice));
-    b->dirty = true;
+    b->in_ofs.assign(n_areas, 0);
+    b->in_len.assign(n_areas, 0);
+    if (load_areas(b, {})) return 1;
     *out = b;
     return 0;
+}
+
+int bsm_batch_add_areas(bsm_batch* b, bsm_area* const* areas, uint32_t n,
+                        const uint32_t* axn_offset, const uint32_t* len) {
+    if (!b || (n && (!areas || !axn_offset || !len))) return fail("null arg");
+    if (!b->dynamic) return fail("areas join dynamic batches only");
+    if (b->areas.size() + n > 65535) return fail("a batch holds at most 65535 areas");
+    for (uint32_t i = 0; i < n; i++) {
+        std::vector<bsm_area*> others(b->areas);
+        others.insert(others.end(), areas, areas + i);
+        if (check_joinable(b, areas[i], others)) return 1;
+        if (axn_offset[i] > areas[i]->axn_len || len[i] > areas[i]->axn_len - axn_offset[i])
+            return fail("input range out of the axon space of joining area " + std::to_string(i));
+    }
+    if (!n) return 0;
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    // the enqueued cycles finish and the members' device state comes back to
+    // their host copies, which load_areas uploads again with the new members
+    if (bsm_batch_sync(b)) return 1;
+    for (uint32_t i = 0; i < n; i++) {
+        bsm_area* a = areas[i];
+        BSM_HIP(hipStreamSynchronize(a->stream));
+        b->areas.push_back(a);
+        b->in_ofs.push_back(axn_offset[i]);
+        b->in_len.push_back(len[i]);
+        a->batch_stream = b->stream;
+        a->batch = b;
+    }
+    return load_areas(b, {});
+}
+
+int bsm_batch_remove_areas(bsm_batch* b, bsm_area* const* areas, uint32_t n) {
+    if (!b || (n && !areas)) return fail("null arg");
+    if (!b->dynamic) return fail("areas leave dynamic batches only");
+    std::vector<char> drop(b->areas.size(), 0);
+    for (uint32_t i = 0; i < n; i++) {
+        size_t k = 0;
+        while (k < b->areas.size() && b->areas[k] != areas[i]) k++;
+        if (k == b->areas.size()) return fail("area to remove is not in the batch");
+        drop[k] = 1;
+    }
+    if (!n) return 0;
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    if (bsm_batch_sync(b)) return 1;
+    std::vector<bsm_area*> keep;
+    std::vector<uint32_t> ko, kl;
+    for (size_t k = 0; k < b->areas.size(); k++) {
+        bsm_area* a = b->areas[k];
+        if (drop[k]) {
+            a->plan_dirty = true;
+            a->batch_stream = nullptr;
+            a->batch = nullptr;
+            continue;
+        }
+        keep.push_back(a);
+        ko.push_back(b->in_ofs[k]);
+        kl.push_back(b->in_len[k]);
+    }
+    b->areas.swap(keep);
+    b->in_ofs.swap(ko);
+    b->in_len.swap(kl);
+    return load_areas(b, {});
 }
 
 int bsm_batch_sync(bsm_batch* b) {
     if (!b) return fail("null batch");
     BSM_HIP(hipSetDevice(b->ctx->device));
     BSM_HIP(hipStreamSynchronize(b->stream));
+    if (b->areas.empty()) return 0;
     // the areas' host rnd streams continue where the device streams stopped
     std::vector<uint32_t> rng(4 * b->areas.size());
     BSM_HIP(hipMemcpy(rng.data(), b->rng_d, rng.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -520,6 +670,8 @@ int bsm_batch_destroy(bsm_batch* b) {
     for (int i = 0; i < 2; i++) {
         if (b->xfer_h[i]) (void)hipHostFree(b->xfer_h[i]);
         (void)hipEventDestroy(b->xfer_ev[i]);
+        (void)hipEventDestroy(b->busy_ev[i][0]);
+        (void)hipEventDestroy(b->busy_ev[i][1]);
     }
     (void)hipFree(b->rng_d);
     (void)hipFree(b->counter_d);
@@ -651,6 +803,11 @@ int bsm_batch_set_input_ranges(bsm_batch* b, const uint32_t* axn_offset, const u
         bsm_area* a = b->areas[i];
         if (axn_offset[i] > a->axn_len || len[i] > a->axn_len - axn_offset[i])
             return fail("input range out of the axon space of area " + std::to_string(i));
+    }
+    for (size_t i = 0; i < b->areas.size(); i++) {
+        bsm_area* a = b->areas[i];
+        b->in_ofs[i] = axn_offset[i];
+        b->in_len[i] = len[i];
         b->io[i].dst = a->axn + axn_offset[i];
         b->io[i].len = len[i];
     }
@@ -664,10 +821,29 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     if (!b->dynamic) return fail("masked cycles need a dynamic batch");
     BSM_HIP(hipSetDevice(b->ctx->device));
     const size_t n = b->areas.size();
-    for (size_t i = 0; frame_ofs && i < n; i++)
-        if (frame_ofs[i] != ~0ull && (frame_ofs[i] > frame_bytes ||
-                                      b->io[i].len > frame_bytes - frame_ofs[i]))
+    for (size_t i = 0; frame_ofs && i < n; i++) {
+        const uint64_t fo = frame_ofs[i];
+        if (fo == ~0ull) continue;
+        if (fo & BSM_FRAME_SDR) {
+            // every index is checked here: the prologue writes where they point
+            const uint64_t at = fo & ~BSM_FRAME_SDR;
+            if ((at & 3) || at > frame_bytes || frame_bytes - at < 4)
+                return fail("SDR entry of area " + std::to_string(i) +
+                            " misaligned or out of the packed buffer");
+            const uint32_t* e = (const uint32_t*)(frames + at);
+            const uint32_t cnt = e[0];
+            if ((frame_bytes - at - 4) / 4 < cnt)
+                return fail("SDR entry of area " + std::to_string(i) + " runs past the buffer");
+            const uint32_t len = b->io[i].len;
+            for (uint32_t k = 0; k < cnt; k++)
+                if (e[1 + k] >= len)
+                    return fail("SDR index of area " + std::to_string(i) +
+                                " beyond its input range");
+        } else if (fo > frame_bytes || b->io[i].len > frame_bytes - fo) {
             return fail("frame of area " + std::to_string(i) + " out of the packed buffer");
+        }
+    }
+    if (n == 0) return 0;
     if (grow_xfer(b, frame_bytes)) return 1;
     if (b->dirty && batch_plan(b)) return 1;
     if (use_graph && !b->exec[0] && capture(b)) return 1;
@@ -675,6 +851,7 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     const int k = b->xfer_next;
     b->xfer_next ^= 1;
     BSM_HIP(hipEventSynchronize(b->xfer_ev[k]));
+    if (busy_harvest(b, k)) return 1;
     if (b->xfer_h_cap[k] < total) {
         if (b->xfer_h[k]) BSM_HIP(hipHostFree(b->xfer_h[k]));
         b->xfer_h[k] = nullptr;
@@ -687,10 +864,24 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     uint64_t* fo = (uint64_t*)(h + ((n + 15) & ~(size_t)15));
     for (size_t i = 0; i < n; i++) fo[i] = frame_ofs ? frame_ofs[i] : ~0ull;
     if (frame_bytes) std::memcpy(h + hdr, frames, frame_bytes);
+    BSM_HIP(hipEventRecord(b->busy_ev[k][0], b->stream));
     BSM_HIP(hipMemcpyAsync(b->xfer_d, h, total, hipMemcpyHostToDevice, b->stream));
     BSM_HIP(hipEventRecord(b->xfer_ev[k], b->stream));
     if (run_cycles(b, 1, use_graph)) return 1;
-    return sample_after_cycle(b->sample_plan, b->stream, b->areas.data(), n, active, b->dyn_d);
+    if (sample_after_cycle(b->sample_plan, b->stream, b->areas.data(), n, active, b->dyn_d))
+        return 1;
+    BSM_HIP(hipEventRecord(b->busy_ev[k][1], b->stream));
+    b->busy_rec[k] = true;
+    return 0;
+}
+
+int bsm_batch_busy_ms(bsm_batch* b, double* ms) {
+    if (!b || !ms) return fail("null arg");
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    for (int k = 0; k < 2; k++)
+        if (busy_harvest(b, k)) return 1;
+    *ms = b->busy_ms;
+    return 0;
 }
 
 int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,

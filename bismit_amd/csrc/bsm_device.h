@@ -141,6 +141,20 @@ __device__ inline void block_copy(uint8_t* __restrict__ dst, const uint8_t* __re
     for (; i < len; i += nth) d8[i] = s8[i];
 }
 
+// Block-cooperative zero fill of len bytes (16-byte stores over the aligned middle).
+__device__ inline void block_zero(uint8_t* __restrict__ dst, uint32_t len, uint32_t tid,
+                                  uint32_t nth) {
+    auto d8 = BSM_G(uint8_t, dst);
+    const uint32_t head = (uint32_t)((16u - (reinterpret_cast<uintptr_t>(dst) & 15u)) & 15u);
+    const uint32_t h = head < len ? head : len;
+    for (uint32_t i = tid; i < h; i += nth) d8[i] = 0;
+    const uint32_t nv = (len - h) >> 4;
+    auto d = BSM_G(u32x4, dst + h);
+    const u32x4 z = {0u, 0u, 0u, 0u};
+    for (uint32_t i = tid; i < nv; i += nth) d[i] = z;
+    for (uint32_t i = h + (nv << 4) + tid; i < len; i += nth) d8[i] = 0;
+}
+
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int b) { return (w >> (8 * b)) & 0xFFu; }
 __device__ __forceinline__ int32_t sbyte_of(uint32_t w, int b) {
     return (int32_t)(int8_t)(uint8_t)((w >> (8 * b)) & 0xFFu);

@@ -2,6 +2,7 @@
 input generators, src/encode/, src/subcortex/input_generator.rs, are replaced by
 these seeded generators).  Every frame is a u8 array of 0/255 axon states.
 """
+import ctypes as C
 import math
 
 import numpy as np
@@ -29,50 +30,63 @@ class GlyphStream:
 class SdrStream:
     """Sparse distributed frames over n_axons: exactly round(density*n) active
     (255) axons, with `overlap` of the previous frame's active set carried into
-    the next frame and the rest drawn fresh."""
+    the next frame (a uniform subset) and the rest drawn fresh from outside it.
+    The stream runs in the engine library (bsm_sdr_*, xorshift128 seeded by
+    splitmix64(seed)), so that the serving loop fills every stepped session's
+    frame with one call (next_packed)."""
 
     def __init__(self, n_axons, density=0.02, overlap=0.5, seed=0):
-        self.n = n_axons
-        self.k = max(1, int(round(density * n_axons)))
-        self.keep = int(round(overlap * self.k))
-        if self.n - self.k < self.k - self.keep:
-            raise ValueError("density too high for the requested overlap")
-        self.rng = np.random.default_rng(seed)
-        self.active = self.rng.choice(self.n, self.k, replace=False)
-        self.member = np.zeros(self.n, dtype=bool)   # membership of self.active
-        self.member[self.active] = True
-
-    def next_active(self):
-        """The current active set; the next one keeps `keep` of it (a uniform
-        random subset) and draws the rest uniformly from outside it."""
-        kept = self.active[self.rng.permutation(self.k)[:self.keep]]
-        need = self.k - self.keep
-        fresh = np.empty(0, dtype=np.int64)
-        taken = self.member.copy()
-        # rejection sampling against the previous set and the draws so far
-        while fresh.size < need:
-            cand = self.rng.integers(0, self.n, 2 * (need - fresh.size) + 8)
-            cand = cand[~taken[cand]]
-            _, first = np.unique(cand, return_index=True)
-            cand = cand[np.sort(first)][:need - fresh.size]
-            taken[cand] = True
-            fresh = np.concatenate([fresh, cand])
-        cur = self.active
-        self.member[cur] = False
-        self.active = np.concatenate([kept, fresh])
-        self.member[self.active] = True
-        return cur
+        self.lib = L.load()
+        self.h = C.c_void_p()
+        L.check(self.lib.bsm_sdr_create(int(n_axons), float(density), float(overlap),
+                                        C.c_uint64(int(seed) & (2**64 - 1)), C.byref(self.h)))
+        n, k = C.c_uint32(), C.c_uint32()
+        L.check(self.lib.bsm_sdr_info(self.h, C.byref(n), C.byref(k)))
+        self.n, self.k = n.value, k.value
 
     def next_frame(self):
-        out = np.zeros(self.n, dtype=np.uint8)
-        out[self.next_active()] = 255
+        out = np.empty(self.n, dtype=np.uint8)
+        L.check(self.lib.bsm_sdr_next(self.h, L.vptr(out)))
         return out
 
     def frames(self, n_cycles):
-        out = np.zeros((n_cycles, self.n), dtype=np.uint8)
-        for c in range(n_cycles):
-            out[c, self.next_active()] = 255
+        out = np.empty((n_cycles, self.n), dtype=np.uint8)
+        if n_cycles:
+            L.check(self.lib.bsm_sdr_frames(self.h, n_cycles, L.vptr(out)))
         return out
+
+    def next_indices(self):
+        """The next frame as its active axon indices (the frame next_frame
+        would return, as 255 at these indices)."""
+        out = np.empty(self.k, dtype=np.uint32)
+        L.check(self.lib.bsm_sdr_next_indices(self.h, L.vptr(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.bsm_sdr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def next_packed(streams, buf, offsets, sdr=False):
+    """The next frame of each stream, stream i's written at buf[offsets[i]:]:
+    dense n bytes, or with sdr=True an SDR entry of bsm_batch_cycle_masked
+    (uint32 k, then the k active indices; offsets 4-byte aligned)."""
+    if not streams:
+        return
+    hs = np.array([s.h.value for s in streams], dtype=np.uint64)
+    ofs = np.ascontiguousarray(offsets, dtype=np.uint64)
+    assert buf.dtype == np.uint8 and buf.flags.c_contiguous
+    size = (lambda s: 4 * (s.k + 1)) if sdr else (lambda s: s.n)
+    assert all(int(o) + size(s) <= buf.size for o, s in zip(ofs, streams))
+    fn = streams[0].lib.bsm_sdr_next_packed_sdr if sdr else streams[0].lib.bsm_sdr_next_packed
+    L.check(fn(L.vptr(hs), len(streams), L.vptr(buf), L.vptr(ofs)))
 
 
 class SymbolStream:

@@ -158,6 +158,38 @@ def recv_many(rxs, wait_for_frame=True):
     return [Frame(outs[i], cyc[i]) if got[i] else None for i in range(n)]
 
 
+class ReceiverSet:
+    """Several receivers received together into buffers of their own, reused
+    call after call: one engine call per recv, with the handle and destination
+    arrays built once (a server receiving every resident session's output each
+    cycle)."""
+
+    def __init__(self, rxs):
+        self.rxs = list(rxs)
+        n = len(self.rxs)
+        self.outs = [np.empty(rx.len, dtype=rx.dtype) for rx in self.rxs]
+        self._hs = np.array([rx.h.value for rx in self.rxs], dtype=np.uint64)
+        self._ds = np.array([o.ctypes.data for o in self.outs], dtype=np.uint64)
+        self._ls = np.array([rx.len for rx in self.rxs], dtype=np.uint64)
+        self.cycles = np.zeros(n, dtype=np.uint64)
+        self.got = np.zeros(n, dtype=np.int32)
+
+    def recv(self, wait_for_frame=False):
+        """Receives into self.outs; self.got[i] / self.cycles[i] say which got a
+        frame and its cycle.  Without wait_for_frame, receivers with nothing
+        pending or in flight get none; frames in flight are waited for."""
+        if not self.rxs:
+            return self.got
+        lib = self.rxs[0].lib
+        L.check(lib.bsm_sampler_recv_many(
+            C.cast(L.vptr(self._hs), C.POINTER(C.c_void_p)), len(self.rxs),
+            1 if wait_for_frame else 0, C.cast(L.vptr(self._ds), C.POINTER(C.c_void_p)),
+            C.cast(L.vptr(self._ls), C.POINTER(C.c_uint64)),
+            C.cast(L.vptr(self.cycles), C.POINTER(C.c_uint64)),
+            C.cast(L.vptr(self.got), C.POINTER(C.c_int))))
+        return self.got
+
+
 def create(area, kind: SamplerKind, buffer_kind=SamplerBufferKind.SINGLE, backpressure=False):
     """A sampler on `area` (CorticalArea::sampler, src/cortex/cortical_area.rs:1017-1284)."""
     if buffer_kind != SamplerBufferKind.SINGLE:

@@ -1,41 +1,54 @@
 """Serving loop: continuous batching of sessions' sensory frames into shared
 cycles on one GPU.
 
-Each session owns a cortex of 1-3 areas resident in HBM for its lifetime and
-streams 2%-sparse SDR frames at its own rate.  The server
-  - admits waiting sessions while their HBM footprint fits the budget (and at
-    most `max_sessions` are resident),
-  - at every shared cycle steps exactly the areas whose session has a frame
-    due, through one dynamic batch (one persistent launch per stage, idle areas
-    masked on the device; optionally captured as a hipGraph),
-  - reads every stepped area's output axons back through a sampler (one gather
-    kernel and one copy per cycle for all sessions), hands them to the session
-    (`on_output`), and records per-frame latency = the frame's output readable
-    on the host - the frame's arrival time, and aggregate synapse-updates/s,
+Each session owns a cortex of 1-3 areas resident in HBM for its lifetime.  Its
+client streams 2%-sparse SDR frames at the session's rate from the moment the
+session arrives: frame k exists at arrival + k / rate.  The server
+  - queues arriving sessions and admits them in arrival order while their HBM
+    footprint fits the budget, their compute demand (synapses x frame rate)
+    fits `target_util` of the GPU's synapse-update capacity, and the resident
+    sessions keep up with their clients (90th percentile lag below `max_lag_s`);
+    a session still waiting after `max_wait_s` is rejected (its client gives up),
+  - sets admitted sessions up on worker threads while the loop keeps cycling,
+    and adds them to / removes finished ones from one dynamic batch between
+    cycles (bsm_batch_add_areas / bsm_batch_remove_areas: no batch rebuild),
+  - at every shared cycle steps exactly the sessions with a frame produced and
+    not yet stepped (one per session; a session behind its client steps every
+    cycle until it has caught up), sending the frames as SDR entries (active
+    indices, scattered on the device), one persistent launch per stage with
+    idle areas masked on the device,
+  - generates the next cycle's frames while the GPU runs the current one, then
+    receives every stepped area's output axons through its sampler (one gather
+    kernel and one copy per cycle for all sessions) and hands them to the session,
   - regrows a session's weak synapses after every `regrow_every` of its cycles
     (the reference's CorticalArea::cycle regrows every 201 cycles,
-    src/cortex/cortical_area.rs:773-776,792-806), all due sessions in one masked call.
-Sessions that arrive while others run join at the next admission epoch; their
-frames overlap on the device with everyone else's.  Each area keeps its own rnd
-stream, state parity and input, so its result is the same as a plain replay of
-the session alone (tests/test_gpu_batch.py checks it).
+    src/cortex/cortical_area.rs:773-776,792-806), all due sessions in one call.
+Frame latency is measured from the frame's production at the client to its
+output frame readable on the host, so it includes the admission wait, the
+set-up and any catch-up.  Each area keeps its own rnd stream, state parity and
+input, so its result is the same as a plain replay of the session alone (the
+GPU tests check it against the oracle).
 
 The reference has no serving mode; its single-threaded flywheel event loop
 (src/flywheel.rs:235-377) is the nearest analogue.
 """
-import math
+import ctypes
 import threading
 import time
+from collections import deque
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import List
 
 import numpy as np
 
+from . import _lib as L
 from . import recipes
 from .cortex import Batch, CorticalArea, footprint
-from .inputs import SdrStream
-from .sampler import SamplerKind, recv_many
+from .inputs import SdrStream, next_packed
+from .sampler import Frame, ReceiverSet, SamplerKind
+
+NO_FRAME = np.uint64(2**64 - 1)
 
 
 @dataclass
@@ -83,9 +96,10 @@ def output_layer(scheme):
 
 
 class Session:
-    def __init__(self, spec: SessionSpec, ctx, t_admit, keep_frames=False, sample_outputs=True):
+    def __init__(self, spec: SessionSpec, ctx, t_arrival, sample_outputs=True):
         self.spec = spec
-        self.t0 = t_admit      # reset when the session joins the batch
+        self.t_arr = t_arrival      # on the server clock
+        self.t_join = None          # joined the batch
         self.schemes = [area_scheme(s, f"s{spec.sid}a{i}") for i, s in enumerate(spec.sides)]
         self.areas = []
         for i, sc in enumerate(self.schemes):
@@ -93,17 +107,22 @@ class Session:
                                            layout_seed=spec.seed + 104729 * i, reach=6))
         self.streams = [SdrStream(sc.input_range()[1], 0.02, 0.5, seed=spec.seed + i)
                         for i, sc in enumerate(self.schemes)]
-        self.next = 0                      # next frame index
+        self.entry_bytes = [4 * (st.k + 1) for st in self.streams]
+        self.next = 0                      # next frame to step
         self.done = 0                      # frames completed
         self.synapses = sum(sc.n_synapses() for sc in self.schemes)
-        self.log = [] if keep_frames else None
         # per-cycle output readback (no backpressure: a slow consumer skips frames)
         self.outputs = [a.sampler(SamplerKind.Axons(output_layer(sc)))
                         for a, sc in zip(self.areas, self.schemes)] if sample_outputs else []
         self.last_outputs = [None] * len(self.areas)
+        self.pos = np.zeros(0, np.int64)   # the areas' positions in the batch
+        self.rx = np.zeros(0, np.int64)    # their samplers' positions in the receiver set
+        self.fp = 0
+        self.demand = 0.0
 
-    def due(self, k):
-        return self.t0 + k / self.spec.rate_hz
+    def produced(self, k):
+        """When the client produces frame k."""
+        return self.t_arr + k / self.spec.rate_hz
 
     def regrow_seed(self, k):
         """Regrowth seed of area k (its epochs count the session's regrowths)."""
@@ -113,6 +132,8 @@ class Session:
         return self.done >= self.spec.n_frames
 
     def close(self):
+        for st in self.streams:
+            st.close()
         for rx in self.outputs:
             rx.close()
         for a in self.areas:
@@ -121,256 +142,347 @@ class Session:
 
 @dataclass
 class ServeStats:
-    latencies: list = field(default_factory=list)
+    latencies: list = field(default_factory=list)      # frame production -> output on host
+    service: list = field(default_factory=list)        # max(production, join) -> output
+    admit_waits: list = field(default_factory=list)
     cycles: int = 0
     synapse_updates: float = 0.0
     gpu_busy_s: float = 0.0
     wall_s: float = 0.0
-    admitted: int = 0
-    completed: int = 0
     arrived: int = 0
+    admitted: int = 0
+    rejected: int = 0
+    completed: int = 0
     max_resident: int = 0
-    rebuilds: int = 0
-    mean_active_areas: float = 0.0
-    rebuild_s: list = field(default_factory=list)
+    max_queue: int = 0
+    queue_area: float = 0.0       # integral of the queue length over time
+    resident_area: float = 0.0
+    active_areas: int = 0
+    membership_changes: int = 0
+    membership_s: list = field(default_factory=list)
     setup_s: list = field(default_factory=list)
+    config: dict = field(default_factory=dict)
 
     def summary(self):
-        lat = np.array(self.latencies) if self.latencies else np.zeros(1)
-        return {
+        def pct(x, q):
+            return float(np.percentile(np.array(x), q) * 1e3) if x else 0.0
+        w = max(self.wall_s, 1e-9)
+        out = {
             "frames": len(self.latencies),
-            "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "synapse_updates_per_s": self.synapse_updates / max(self.wall_s, 1e-9),
+            "p50_ms": pct(self.latencies, 50),
+            "p99_ms": pct(self.latencies, 99),
+            "latency_from": "frame produced at the client (arrival + k / rate) to its output "
+                            "frame readable on the host: admission wait, set-up and catch-up "
+                            "included",
+            "service_p50_ms": pct(self.service, 50),
+            "service_p99_ms": pct(self.service, 99),
+            "admission_wait_p50_ms": pct(self.admit_waits, 50),
+            "admission_wait_p99_ms": pct(self.admit_waits, 99),
+            "synapse_updates_per_s": self.synapse_updates / w,
+            "gpu_busy_frac": self.gpu_busy_s / w,
             "shared_cycles": self.cycles,
+            "mean_active_areas_per_cycle": self.active_areas / max(self.cycles, 1),
             "sessions_arrived": self.arrived,
             "sessions_admitted": self.admitted,
+            "sessions_rejected": self.rejected,
             "sessions_completed": self.completed,
+            "queue_len_mean": self.queue_area / w,
+            "queue_len_max": self.max_queue,
+            "resident_sessions_mean": self.resident_area / w,
             "max_resident_sessions": self.max_resident,
-            "batch_rebuilds": self.rebuilds,
-            "mean_active_areas_per_cycle": self.mean_active_areas,
-            "wall_s": self.wall_s,
-            "rebuild_ms_max": max(self.rebuild_s, default=0.0) * 1e3,
-            "rebuild_ms_mean": float(np.mean(self.rebuild_s) * 1e3) if self.rebuild_s else 0.0,
+            "membership_changes": self.membership_changes,
+            "membership_ms_mean": float(np.mean(self.membership_s) * 1e3) if self.membership_s else 0.0,
+            "membership_ms_max": max(self.membership_s, default=0.0) * 1e3,
             "session_setup_ms_mean": float(np.mean(self.setup_s) * 1e3) if self.setup_s else 0.0,
+            "wall_s": self.wall_s,
         }
+        out.update(self.config)
+        return out
+
+
+class _Cycle:
+    """A shared cycle prepared or in flight."""
+    __slots__ = ("sessions", "produced", "ofs", "total", "syn", "n_areas")
 
 
 class Server:
-    def __init__(self, ctx, hbm_budget_bytes=96 << 30, max_sessions=64, epoch_s=0.05,
-                 use_graph=False, keep_frames=False, on_retire=None, setup_workers=2,
-                 regrow_every=201, regrow_reach=6, sample_outputs=True, on_output=None):
+    def __init__(self, ctx, hbm_budget_bytes=None, max_sessions=None, epoch_s=0.02,
+                 use_graph=False, on_retire=None, setup_workers=4, regrow_every=201,
+                 regrow_reach=6, sample_outputs=True, on_output=None, max_wait_s=1.0,
+                 capacity=1.0e12, target_util=0.85, max_lag_s=0.1):
         self.ctx = ctx
         # Masked cycles are enqueued directly by default: the batch changes with
-        # every admission epoch, and capturing plus instantiating its two graphs
-        # (about 4 ms, on the frame path) costs more latency than the ~25 us of
-        # launches per cycle a graph saves (p99 14.4 -> 11.7 ms, same p50, at
-        # 200 sessions/s).
+        # most admission epochs, and capturing plus instantiating its two graphs
+        # costs more latency than the launches per cycle a graph saves.
+        self.use_graph = use_graph
         self.sample_outputs = sample_outputs
-        self.on_output = on_output            # on_output(session, [Frame per area])
+        self.on_output = on_output            # on_output(session, [Frame per area]); the data
+                                              # stays valid until the session's next cycle
+        self.on_retire = on_retire
         self.regrow_every = regrow_every      # 0: regrowth off
         self.regrow_reach = regrow_reach
+        if hbm_budget_bytes is None:
+            hbm_budget_bytes = int(0.75 * ctx.info()["mem_bytes"])
         self.budget = hbm_budget_bytes
-        self.max_sessions = max_sessions
+        self.max_sessions = max_sessions      # None: HBM and compute decide
         self.epoch_s = epoch_s
-        self.use_graph = use_graph
-        self.keep_frames = keep_frames
-        self.on_retire = on_retire
+        self.max_wait_s = max_wait_s          # None: sessions wait as long as it takes
+        self.capacity = capacity              # synapse-updates/s the GPU sustains (hint)
+        self.target_util = target_util
+        self.max_lag_s = max_lag_s
+        self.setup_workers = setup_workers
         self.resident = []
         self.used = 0
+        self.demand = 0.0
         self.batch = None
-        # sessions are set up (areas allocated and laid out on the device) by
-        # worker threads while the loop keeps cycling; the ctypes calls release
-        # the interpreter lock, and each area works on its own stream
-        self.setup_workers = setup_workers
+        self.recvset = ReceiverSet([])
         self._ready = []
         self._ready_lock = threading.Lock()
+        self._per_side = {}
+        self._buf = np.empty(1 << 20, np.uint8)
+
+    # ------------------------------------------------------------ admission
+    def _side(self, side):
+        """(footprint bytes, synapses) of one session area of this side."""
+        if side not in self._per_side:
+            sc = area_scheme(side, "f")
+            self._per_side[side] = (footprint(sc), sc.n_synapses())
+        return self._per_side[side]
 
     def footprint(self, spec):
-        return sum(footprint(area_scheme(s, "f")) for s in spec.sides)
+        return sum(self._side(s)[0] for s in spec.sides)
 
-    def _rebuild(self):
-        if self.batch is not None:
-            self.batch.close()
-            self.batch = None
-        areas = [a for s in self.resident for a in s.areas]
-        if areas:
-            self.batch = Batch(areas, self.ctx, dynamic=True)
+    def session_demand(self, spec):
+        return sum(self._side(s)[1] for s in spec.sides) * spec.rate_hz
 
-    def _deliver(self, active_sessions):
-        """Every stepped area's output frame of this cycle, to its session."""
-        rxs = [rx for s in active_sessions for rx in s.outputs]
-        frames = recv_many(rxs, wait_for_frame=False)
-        k = 0
-        for s in active_sessions:
-            got = frames[k:k + len(s.outputs)]
-            k += len(s.outputs)
-            if any(f is None for f in got):
-                raise RuntimeError(f"session {s.spec.sid}: output frame missing")
-            s.last_outputs = got
-            if self.on_output:
-                self.on_output(s, got)
+    def _keeping_up(self, now):
+        """Resident sessions (joined over half a second ago) keep up with their
+        clients: 90th percentile of how long their next frame has waited."""
+        lags = [now - s.produced(s.next) for s in self.resident
+                if s.next < s.spec.n_frames and s.t_join is not None and now - s.t_join > 0.5]
+        return not lags or float(np.percentile(lags, 90)) <= self.max_lag_s
 
-    def _regrow_due(self, active_set):
-        """Regrowth of the active sessions whose cycle about to complete is a
-        multiple of regrow_every (it belongs to that cycle, as upstream)."""
-        n = len(self.batch.areas)
-        which = np.zeros(n, np.uint8)
-        seeds = [0] * n
-        epochs = [0] * n
-        idx = 0
-        for s in self.resident:
-            c = s.done + 1
-            if id(s) in active_set and c % self.regrow_every == 0:
-                for k in range(len(s.areas)):
-                    which[idx + k] = 1
-                    seeds[idx + k] = s.regrow_seed(k)
-                    epochs[idx + k] = c // self.regrow_every - 1
-            idx += len(s.areas)
-        if which.any():
-            self.batch.regrow(seeds, epochs, self.regrow_reach, which)
+    def _admit(self, waiting, setting_up, pool, st, now):
+        if not self._keeping_up(now):
+            return
+        while waiting:
+            spec, t_arr = waiting[0]
+            if self.max_sessions is not None and \
+                    len(self.resident) + len(setting_up) >= self.max_sessions:
+                break
+            fp = self.footprint(spec)
+            d = self.session_demand(spec)
+            if self.used + fp > self.budget:
+                break
+            if self.resident and self.demand + d > self.target_util * self.capacity:
+                break
+            waiting.popleft()
+            self.used += fp
+            self.demand += d
+            st.admit_waits.append(now - t_arr)
+            setting_up.append(pool.submit(self._setup, spec, t_arr, fp, d, st))
+            st.admitted += 1
 
-    def _setup(self, spec, fp, st):
+    def _setup(self, spec, t_arr, fp, d, st):
         t0 = time.perf_counter()
-        sess = Session(spec, self.ctx, 0.0, self.keep_frames, self.sample_outputs)
+        sess = Session(spec, self.ctx, t_arr, self.sample_outputs)
         sess.fp = fp
+        sess.demand = d
         st.setup_s.append(time.perf_counter() - t0)
         with self._ready_lock:
             self._ready.append(sess)
 
+    # ------------------------------------------------------------ membership
+    def _membership(self, setting_up, st, now):
+        """Between cycles: finished sessions leave the batch, set-up ones join."""
+        for f in [f for f in setting_up if f.done()]:
+            f.result()           # re-raise a set-up error here
+            setting_up.remove(f)
+        retired = [s for s in self.resident if s.finished()]
+        with self._ready_lock:
+            ready, self._ready = self._ready, []
+        if not retired and not ready:
+            return
+        t0 = time.perf_counter()
+        if retired:
+            for s in retired:
+                s.last_outputs = [Frame(self.recvset.outs[i].copy(), s.done) for i in s.rx]
+                if self.on_retire:
+                    self.on_retire(s)
+            self.batch.remove([a for s in retired for a in s.areas])
+            gone = set(id(s) for s in retired)
+            self.resident = [s for s in self.resident if id(s) not in gone]
+            for s in retired:
+                self.used -= s.fp
+                self.demand -= s.demand
+                s.close()
+                st.completed += 1
+        if ready:
+            self.batch.add([a for s in ready for a in s.areas])
+            for s in ready:
+                s.t_join = now
+            self.resident.extend(ready)
+        pos = {id(a): i for i, a in enumerate(self.batch.areas)}
+        rxs = []
+        for s in self.resident:
+            s.pos = np.array([pos[id(a)] for a in s.areas], np.int64)
+            s.rx = np.arange(len(rxs), len(rxs) + len(s.outputs), dtype=np.int64)
+            rxs.extend(s.outputs)
+        self.recvset = ReceiverSet(rxs)
+        st.membership_s.append(time.perf_counter() - t0)
+        st.membership_changes += 1
+        st.max_resident = max(st.max_resident, len(self.resident))
+
+    # ------------------------------------------------------------ cycles
+    def _prepare(self, now):
+        """The next shared cycle: every resident session with a frame produced
+        and not yet stepped; its frames generated into the staging buffer."""
+        act = [s for s in self.resident if s.next < s.spec.n_frames and s.produced(s.next) <= now]
+        if not act:
+            return None
+        sizes = [b for s in act for b in s.entry_bytes]
+        ofs = np.zeros(len(sizes), np.uint64)
+        if len(sizes) > 1:
+            ofs[1:] = np.cumsum(sizes[:-1], dtype=np.uint64)
+        total = int(ofs[-1]) + sizes[-1]
+        if self._buf.size < total:
+            self._buf = np.empty(max(total, 2 * self._buf.size), np.uint8)
+        next_packed([st for s in act for st in s.streams], self._buf, ofs, sdr=True)
+        c = _Cycle()
+        c.sessions = act
+        c.produced = [s.produced(s.next) for s in act]
+        c.ofs = ofs
+        c.total = total
+        c.syn = sum(s.synapses for s in act)
+        c.n_areas = len(sizes)
+        for s in act:
+            s.next += 1
+        return c
+
+    def _enqueue(self, c):
+        n = len(self.batch.areas)
+        mask = np.zeros(n, np.uint8)
+        fo = np.full(n, NO_FRAME, np.uint64)
+        pos = np.concatenate([s.pos for s in c.sessions])
+        mask[pos] = 1
+        fo[pos] = c.ofs | np.uint64(L.FRAME_SDR)
+        self.batch.cycle_masked(mask, self._buf[:c.total], fo, graph=self.use_graph)
+
+    def _finish(self, c, st, clock):
+        """Receives the cycle's output frames, hands them out, records latency
+        and regrows the sessions whose cycle count is due."""
+        got = self.recvset.recv(wait_for_frame=False)
+        if not self.sample_outputs:
+            self.batch.sync()
+        t = clock()
+        for s, prod in zip(c.sessions, c.produced):
+            if self.sample_outputs:
+                if not all(got[i] for i in s.rx):
+                    raise RuntimeError(f"session {s.spec.sid}: output frame missing")
+                if self.on_output:
+                    self.on_output(s, [Frame(self.recvset.outs[i], int(self.recvset.cycles[i]))
+                                       for i in s.rx])
+            s.done += 1
+            st.latencies.append(t - prod)
+            st.service.append(t - max(prod, s.t_join))
+        st.cycles += 1
+        st.synapse_updates += c.syn
+        st.active_areas += c.n_areas
+        if self.regrow_every:
+            due = [s for s in c.sessions if s.done % self.regrow_every == 0]
+            if due:
+                n = len(self.batch.areas)
+                which = np.zeros(n, np.uint8)
+                seeds = [0] * n
+                epochs = [0] * n
+                for s in due:
+                    for k, p in enumerate(s.pos):
+                        which[p] = 1
+                        seeds[p] = s.regrow_seed(k)
+                        epochs[p] = s.done // self.regrow_every - 1
+                self.batch.regrow(seeds, epochs, self.regrow_reach, which)
+
+    # ------------------------------------------------------------ loop
     def run(self, specs, wall_limit_s=30.0, time_scale=1.0):
-        """Serve `specs` (arrival times in seconds, compressed by time_scale) until
-        every admitted session finished or wall_limit_s passed."""
+        """Serve `specs` (arrival times in seconds, scaled by time_scale) until
+        every session is done or rejected, or wall_limit_s passed."""
         with ThreadPoolExecutor(max_workers=self.setup_workers) as pool:
             return self._run(specs, wall_limit_s, time_scale, pool)
 
     def _run(self, specs, wall_limit_s, time_scale, pool):
         st = ServeStats()
-        pending = sorted(specs, key=lambda s: s.arrival)
-        waiting = []
-        setting_up = []           # futures of sessions being set up
+        st.config = {"hbm_budget_gb": self.budget / 2**30, "capacity_hint": self.capacity,
+                     "target_util": self.target_util, "max_wait_s": self.max_wait_s,
+                     "max_lag_ms": self.max_lag_s * 1e3, "max_sessions": self.max_sessions}
+        self.batch = Batch([], self.ctx, dynamic=True)
+        pending = deque(sorted(specs, key=lambda s: s.arrival))
+        waiting = deque()
+        setting_up = []
         t_start = time.perf_counter()
         clock = lambda: time.perf_counter() - t_start
         last_epoch = -1e9
-        act_sum = 0
+        t_last = 0.0
+        inflight = None
         while True:
             now = clock()
+            st.queue_area += len(waiting) * (now - t_last)
+            st.resident_area += len(self.resident) * (now - t_last)
+            t_last = now
             if now > wall_limit_s:
                 break
             while pending and pending[0].arrival * time_scale <= now:
-                waiting.append(pending.pop(0))
+                spec = pending.popleft()
+                waiting.append((spec, spec.arrival * time_scale))
                 st.arrived += 1
-            changed = False
+            if self.max_wait_s is not None:
+                while waiting and now - waiting[0][1] > self.max_wait_s:
+                    waiting.popleft()
+                    st.rejected += 1
+            st.max_queue = max(st.max_queue, len(waiting))
             if now - last_epoch >= self.epoch_s:
                 last_epoch = now
-                retired = [s for s in self.resident if s.finished()]
-                if retired:
-                    # the batch releases every area before retired ones are freed
-                    if self.batch is not None:
-                        self.batch.close()
-                        self.batch = None
-                    for s in retired:
-                        if self.on_retire:
-                            self.on_retire(s)
-                        self.used -= s.fp
-                        s.close()
-                        st.completed += 1
-                    self.resident = [s for s in self.resident if not s.finished()]
-                    changed = True
-                # admission: HBM is reserved now, the set-up runs in the background
-                while waiting and len(self.resident) + len(setting_up) < self.max_sessions:
-                    fp = self.footprint(waiting[0])
-                    if self.used + fp > self.budget:
-                        break
-                    spec = waiting.pop(0)
-                    self.used += fp
-                    setting_up.append(pool.submit(self._setup, spec, fp, st))
-                    st.admitted += 1
-                # sessions whose set-up finished join the batch
-                for f in [f for f in setting_up if f.done()]:
-                    f.result()           # re-raise a set-up error here
-                    setting_up.remove(f)
+                self._admit(waiting, setting_up, pool, st, now)
+            # the next cycle's frames are made while the GPU runs the current one
+            nxt = self._prepare(now)
+            if inflight is not None:
+                self._finish(inflight, st, clock)
+                inflight = None
+            self._membership(setting_up, st, clock())
+            if nxt is not None:
+                self._enqueue(nxt)
+                inflight = nxt
+                continue
+            if not self.resident and not pending and not waiting and not setting_up:
                 with self._ready_lock:
-                    ready, self._ready = self._ready, []
-                for sess in ready:
-                    sess.t0 = clock()
-                    self.resident.append(sess)
-                    changed = True
-            if changed:
-                t_rb = time.perf_counter()
-                self._rebuild()
-                st.rebuild_s.append(time.perf_counter() - t_rb)
-                st.rebuilds += 1
-                st.max_resident = max(st.max_resident, len(self.resident))
-            if not self.resident:
-                if not pending and not waiting and not setting_up:
-                    break
-                nxt = pending[0].arrival * time_scale if pending else now + 1e-3
-                time.sleep(max(0.0, min(nxt - now, 0.01)))
-                continue
-            # frames due now
-            now = clock()
-            active_sessions = [s for s in self.resident
-                               if s.next < s.spec.n_frames and s.due(s.next) <= now]
-            if not active_sessions:
-                nxt = min(s.due(s.next) for s in self.resident if s.next < s.spec.n_frames) \
-                    if any(s.next < s.spec.n_frames for s in self.resident) else now + 1e-3
-                time.sleep(max(0.0, min(nxt - now, 0.005)))
-                continue
-            n = len(self.batch.areas)
-            mask = np.zeros(n, dtype=np.uint8)
-            ofs = np.full(n, -1, dtype=np.int64)
-            chunks, pos, idx, due = [], 0, 0, []
-            syn = 0
-            active_set = set(id(s) for s in active_sessions)
-            for s in self.resident:
-                if id(s) in active_set:
-                    for k, a in enumerate(s.areas):
-                        fr = s.streams[k].next_frame()
-                        if s.log is not None:
-                            s.log.append((k, s.next))
-                        mask[idx + k] = 1
-                        ofs[idx + k] = pos
-                        chunks.append(fr)
-                        pos += fr.size
-                    due.append(s.due(s.next))
-                    s.next += 1
-                    syn += s.synapses
-                idx += len(s.areas)
-            frames = np.concatenate(chunks) if chunks else np.zeros(0, np.uint8)
-            t0 = clock()
-            self.batch.cycle_masked(mask, frames, ofs, graph=self.use_graph)
-            if self.regrow_every:
-                self._regrow_due(active_set)
-            if self.sample_outputs:
-                # the frames' event follows the cycle on the batch stream, so
-                # receiving them is the wait for the cycle
-                self._deliver(active_sessions)
-            else:
-                self.batch.sync()
-            t1 = clock()
-            st.gpu_busy_s += t1 - t0
-            for s in active_sessions:
-                s.done += 1
-            st.latencies.extend(t1 - d for d in due)
-            st.cycles += 1
-            act_sum += int(mask.sum())
-            st.synapse_updates += syn
+                    if not self._ready:
+                        break
+            # idle: sleep until the next frame or arrival is due (bounded)
+            wake = [s.produced(s.next) for s in self.resident if s.next < s.spec.n_frames]
+            if pending:
+                wake.append(pending[0].arrival * time_scale)
+            nxt_t = min(wake) if wake else now + 1e-3
+            time.sleep(max(0.0, min(nxt_t - clock(), 0.002)))
+        if inflight is not None:
+            self._finish(inflight, st, clock)
         st.wall_s = clock()
-        st.mean_active_areas = act_sum / max(st.cycles, 1)
+        busy = ctypes.c_double()
+        L.check(self.ctx.lib.bsm_batch_busy_ms(self.batch.h, ctypes.byref(busy)))
+        st.gpu_busy_s = busy.value / 1000.0
         # drain: sessions still being set up are closed once ready
         for f in setting_up:
             f.result()
         with self._ready_lock:
-            self.resident.extend(self._ready)
-            self._ready = []
-        if self.batch is not None:
-            self.batch.close()
-            self.batch = None
+            late, self._ready = self._ready, []
         for s in self.resident:
-            if self.on_retire and s.finished():
-                self.on_retire(s)
-        for s in self.resident:
+            if s.finished():
+                s.last_outputs = [Frame(self.recvset.outs[i].copy(), s.done) for i in s.rx]
+                if self.on_retire:
+                    self.on_retire(s)
+        self.batch.close()
+        self.batch = None
+        for s in self.resident + late:
             s.close()
         self.resident = []
+        self.recvset = ReceiverSet([])
         return st
+

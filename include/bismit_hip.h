@@ -401,6 +401,29 @@ int bsm_area_init_layout(bsm_area* area, uint64_t seed, int32_t reach);
  * Completes on return.  Areas inside a batch regrow through bsm_batch_regrow. */
 int bsm_area_regrow(bsm_area* area, uint64_t seed, uint64_t epoch, int32_t reach);
 
+/* ---- synthetic input streams of the workloads (host only; the reference's
+ *      encoders and input generators, src/encode/, src/subcortex/
+ *      input_generator.rs, are not part of this engine).  An SDR stream over
+ *      n_axons keeps exactly k = round(density * n_axons) axons at 255; each
+ *      next frame keeps round(overlap * k) of the previous active axons (a
+ *      uniform subset) and draws the rest uniformly from outside the previous
+ *      set, from xorshift128 seeded by splitmix64(seed). ---- */
+typedef struct bsm_sdr bsm_sdr;
+int bsm_sdr_create(uint32_t n_axons, double density, double overlap, uint64_t seed,
+                   bsm_sdr** out);
+int bsm_sdr_destroy(bsm_sdr* s);
+int bsm_sdr_info(bsm_sdr* s, uint32_t* n_axons, uint32_t* k);
+/* Writes the current frame (n_axons bytes) and advances the stream. */
+int bsm_sdr_next(bsm_sdr* s, uint8_t* frame);
+int bsm_sdr_frames(bsm_sdr* s, uint32_t n_frames, uint8_t* frames);
+/* The next frame of each of n streams, stream i's at buf + ofs[i]. */
+int bsm_sdr_next_packed(bsm_sdr* const* s, uint32_t n, uint8_t* buf, const uint64_t* ofs);
+/* The current frame as its k active axon indices (in draw order), then advance. */
+int bsm_sdr_next_indices(bsm_sdr* s, uint32_t* idx);
+/* The next frame of each of n streams as an SDR entry of bsm_batch_cycle_masked
+ * (uint32 k, then k indices) at buf + ofs[i] (4-byte aligned). */
+int bsm_sdr_next_packed_sdr(bsm_sdr* const* s, uint32_t n, uint8_t* buf, const uint64_t* ofs);
+
 /* ---- deterministic rnd schedule helpers ---- */
 uint64_t bsm_splitmix64(uint64_t* state);
 /* xorshift128 (Marsaglia 2003) stream seeded from splitmix64(seed). */
@@ -414,14 +437,30 @@ int bsm_batch_create(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, bsm
 enum bsm_batch_flags { BSM_BATCH_DYNAMIC = 1u };
 int bsm_batch_create_ex(bsm_ctx* ctx, bsm_area* const* areas, uint32_t n_areas, uint32_t flags,
                         bsm_batch** out);
+/* Dynamic batches: areas join and leave between cycles (continuous batching).
+ * Both wait for the batch's enqueued cycles; the next cycle re-plans.  Joining
+ * areas bring their input ranges; leaving areas continue in the plain form with
+ * their state as the batch left it.  A dynamic batch may be created empty. */
+int bsm_batch_add_areas(bsm_batch* b, bsm_area* const* areas, uint32_t n,
+                        const uint32_t* axn_offset, const uint32_t* len);
+int bsm_batch_remove_areas(bsm_batch* b, bsm_area* const* areas, uint32_t n);
 /* Dynamic batches: axon range each area's frame is copied to. */
 int bsm_batch_set_input_ranges(bsm_batch* b, const uint32_t* axn_offset, const uint32_t* len);
 /* Dynamic batches: one shared cycle of the areas with active[i] != 0.  frames is a
  * host buffer of frame_bytes; area i's frame starts at frame_ofs[i] (UINT64_MAX:
- * no intake).  Returns after enqueueing (asynchronous); bsm_batch_query polls. */
+ * no intake).  An offset with BSM_FRAME_SDR set names an SDR entry instead of a
+ * dense frame: at (offset & ~BSM_FRAME_SDR), 4-byte aligned, a uint32 count n and
+ * n uint32 axon indices relative to the area's input range (each below its
+ * length); the input range becomes 0 except those axons, which become 255.
+ * Returns after enqueueing (asynchronous); bsm_batch_query polls. */
+#define BSM_FRAME_SDR (1ull << 63)
 int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* frames,
                            const uint64_t* frame_ofs, uint64_t frame_bytes, int use_graph);
 int bsm_batch_query(bsm_batch* b, int* done);
+/* Dynamic batches: device time (HIP events on the batch stream) of every masked
+ * cycle so far, from its frame upload to its samplers' gather; waits for the
+ * last two. */
+int bsm_batch_busy_ms(bsm_batch* b, double* ms);
 /* Regrowth of batched areas (the batch's form of bsm_area_regrow, same draws):
  * area i regrows with seeds[i] and epochs[i] when which is null or which[i] is
  * nonzero, so each area of a dynamic batch keeps its own 201-cycle schedule

@@ -319,14 +319,14 @@ def test_serving_sessions_match_solo_replay(ctx):
         assert all(f.cycle == sess.done + 1 for f in frames)
 
     every = 25
-    srv = Server(ctx, hbm_budget_bytes=1 << 30, max_sessions=4, epoch_s=0.01,
+    srv = Server(ctx, hbm_budget_bytes=1 << 30, max_sessions=4, epoch_s=0.01, max_wait_s=None,
                  on_retire=on_retire, regrow_every=every, on_output=on_output)
     st = srv.run(specs, wall_limit_s=60.0)
     summ = st.summary()
     print(summ, flush=True)
     assert summ["sessions_completed"] == len(specs)
     assert summ["max_resident_sessions"] <= 4
-    assert summ["batch_rebuilds"] > 2
+    assert summ["membership_changes"] > 2
     assert n_out[0] == summ["frames"]      # one delivery per session-cycle
     from bismit_amd.serve import area_scheme, output_layer
     for sid, (spec, done, bufs, outs) in finals.items():

@@ -303,7 +303,7 @@ def test_serving_sessions_at_config5_sizes_match_oracle(ctx):
             for a, sc in zip(sess.areas, sess.schemes)])
 
     every = 20
-    srv = Server(ctx, hbm_budget_bytes=8 << 30, max_sessions=3, epoch_s=0.01,
+    srv = Server(ctx, hbm_budget_bytes=8 << 30, max_sessions=3, epoch_s=0.01, max_wait_s=None,
                  on_retire=on_retire, regrow_every=every)
     st = srv.run(specs, wall_limit_s=60.0)
     summ = st.summary()
@@ -337,3 +337,99 @@ def test_serving_sessions_at_config5_sizes_match_oracle(ctx):
 def helpers_bufs(scheme, li):
     from helpers import layer_bufs
     return layer_bufs(scheme, li)
+
+
+def test_dynamic_batch_membership_and_sdr_intake_match_oracle(ctx):
+    """Continuous batching through bsm_batch_add_areas / bsm_batch_remove_areas:
+    a dynamic batch created empty; areas join at cycles 0 and 6, one leaves at
+    cycle 12, runs two plain cycles on its own (its rnd stream continues where
+    the batch left it) and joins again at cycle 18.  Frames alternate between
+    dense entries and SDR entries (count + active indices, scattered on the
+    device).  Each area against its own oracle area after every cycle."""
+    schemes = [recipes.cortical_area(f"m{i}", side, ssc_depth=sd, pyr_depth=pd, ssc_spd=spd,
+                                     pyr_spd=pspd)
+               for i, (side, sd, pd, spd, pspd) in enumerate(
+                   [(24, 1, 2, 16, 20), (40, 2, 1, 32, 24), (32, 1, 3, 20, 16),
+                    (48, 3, 2, 24, 36), (16, 1, 1, 8, 12)])]
+    seeds = [400 + i for i in range(len(schemes))]
+    gs, os_, streams = [], [], []
+    for i, s in enumerate(schemes):
+        g = CorticalArea(s, ctx, rnd_seed=seeds[i], init_layout=False)
+        o = P.OracleArea(s)
+        lay = host_layout(s, 80 + i)
+        load_layout(g, s, lay)
+        load_layout(o, s, lay)
+        gs.append(g)
+        os_.append(o)
+        streams.append(inputs.SdrStream(s.input_range()[1], 0.03, 0.5, seed=90 + i))
+    n = 24
+    rnds = [area_rnd(seeds[i], gs[i].n_slots, 64) for i in range(len(schemes))]
+    done = [0] * len(schemes)
+    rng = np.random.default_rng(52)
+    b = Batch([], ctx, dynamic=True)
+    members = []
+
+    def step_oracle(i, dense):
+        os_[i].intake(dense)
+        os_[i].cycle(rnds[i][done[i]], 1)
+        done[i] += 1
+
+    for c in range(n):
+        if c == 0:
+            b.add([gs[0], gs[1], gs[2]])
+            members = [0, 1, 2]
+        elif c == 6:
+            b.add([gs[3], gs[4]])
+            members = [0, 1, 2, 3, 4]
+        elif c == 12:
+            b.remove([gs[1]])
+            members = [0, 2, 3, 4]
+            for _ in range(2):
+                idx = streams[1].next_indices()
+                dense = np.zeros(streams[1].n, np.uint8)
+                dense[idx] = 255
+                gs[1].intake(dense)
+                gs[1].cycle(1)
+                step_oracle(1, dense)
+            compare(gs[1], os_[1], schemes[1], "area 1 plain while out of the batch")
+        elif c == 18:
+            b.add([gs[1]])
+            members = [0, 2, 3, 4, 1]
+        assert [id(a) for a in b.areas] == [id(gs[i]) for i in members]
+        mask = (rng.random(len(members)) < 0.75).astype(np.uint8)
+        ofs = np.full(len(members), np.uint64(2**64 - 1), np.uint64)
+        buf = bytearray()
+        dense_of = {}
+        for j in np.flatnonzero(mask):
+            i = members[j]
+            idx = streams[i].next_indices()
+            dense = np.zeros(streams[i].n, np.uint8)
+            dense[idx] = 255
+            dense_of[i] = dense
+            if (c + j) % 2:
+                pad = (-len(buf)) % 4
+                buf += bytes(pad)
+                ofs[j] = np.uint64(len(buf)) | np.uint64(L.FRAME_SDR)
+                buf += np.concatenate([[idx.size], idx]).astype(np.uint32).tobytes()
+            else:
+                ofs[j] = len(buf)
+                buf += dense.tobytes()
+        b.cycle_masked(mask, np.frombuffer(bytes(buf), np.uint8), ofs, graph=bool(c % 3 == 0))
+        for j, i in enumerate(members):
+            if mask[j]:
+                step_oracle(i, dense_of[i])
+        for i in range(len(schemes)):
+            compare(gs[i], os_[i], schemes[i], f"membership area {i} cycle {c}")
+    assert min(done) >= 8, done
+    # an SDR index beyond the input range is refused on the host, before any launch
+    bad = np.array([1, schemes[0].input_range()[1]], np.uint32).tobytes()
+    ofs = np.full(len(members), np.uint64(2**64 - 1), np.uint64)
+    ofs[0] = np.uint64(L.FRAME_SDR)
+    mask = np.zeros(len(members), np.uint8)
+    mask[0] = 1
+    with pytest.raises(L.BsmError, match="beyond its input range"):
+        b.cycle_masked(mask, np.frombuffer(bad, np.uint8), ofs, graph=False)
+    compare(gs[0], os_[0], schemes[0], "after the refused cycle")
+    b.close()
+    for g in gs:
+        g.close()
