@@ -163,6 +163,7 @@ struct bsm_batch {
     // dynamic batches: each area's input range (kept across membership changes)
     std::vector<uint32_t> in_ofs, in_len;
     size_t area_cap = 0;        // areas the per-area device arrays hold
+    size_t cap_t = 0, cap_s = 0, cap_p = 0, cap_i = 0;   // plan table capacities (bytes)
     // device time of masked cycles: events around each cycle, one pair per
     // transfer slot, added up once the slot comes round again
     hipEvent_t busy_ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -297,20 +298,25 @@ static int batch_plan(bsm_batch* b) {
         }
     }
     if (side_prepare(b->side, b->stages)) return 1;
-    (void)hipFree(b->tjobs_d);
-    (void)hipFree(b->sjobs_d);
-    (void)hipFree(b->pjobs_d);
-    (void)hipFree(b->items_d);
-    auto up = [&](void** dst, const void* src, size_t bytes) -> int {
-        *dst = nullptr;
-        BSM_HIP(hipMalloc(dst, bytes ? bytes : 16));
+    // the tables only grow (membership changes re-plan often); the batch stream
+    // is idle whenever a plan is made, so they are rewritten in place
+    BSM_HIP(hipStreamSynchronize(b->stream));
+    auto up = [&](void** dst, size_t& cap, const void* src, size_t bytes) -> int {
+        if (bytes > cap || !*dst) {
+            (void)hipFree(*dst);
+            *dst = nullptr;
+            const size_t c = std::max<size_t>({bytes, 2 * cap, 256});
+            BSM_HIP(hipMalloc(dst, c));
+            cap = c;
+        }
         if (bytes) BSM_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         return 0;
     };
-    if (up((void**)&b->tjobs_d, b->tjobs.data(), b->tjobs.size() * sizeof(TuftJob))) return 1;
-    if (up((void**)&b->sjobs_d, b->sjobs.data(), b->sjobs.size() * sizeof(SscJob))) return 1;
-    if (up((void**)&b->pjobs_d, b->pjobs.data(), b->pjobs.size() * sizeof(PyrJob))) return 1;
-    if (up((void**)&b->items_d, items.data(), items.size() * sizeof(WorkItem))) return 1;
+    if (up((void**)&b->tjobs_d, b->cap_t, b->tjobs.data(), b->tjobs.size() * sizeof(TuftJob)) ||
+        up((void**)&b->sjobs_d, b->cap_s, b->sjobs.data(), b->sjobs.size() * sizeof(SscJob)) ||
+        up((void**)&b->pjobs_d, b->cap_p, b->pjobs.data(), b->pjobs.size() * sizeof(PyrJob)) ||
+        up((void**)&b->items_d, b->cap_i, items.data(), items.size() * sizeof(WorkItem)))
+        return 1;
     b->dirty = false;
     return 0;
 }
@@ -888,9 +894,9 @@ int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,
                      const uint64_t* epochs, int32_t reach) {
     if (!b || !seeds || !epochs) return fail("null arg");
     if (reach < 0 || reach > 127) return fail("reach must be in [0, 127]");
-    // every enqueued cycle completes first (the side stream joins the batch
-    // stream within each cycle); sync also brings dynamic parities to the host
-    if (bsm_batch_sync(b)) return 1;
+    BSM_HIP(hipSetDevice(b->ctx->device));
+    // enqueued on the batch stream: after every cycle enqueued so far (the side
+    // stream joins the batch stream within each cycle), before the next one
     for (size_t i = 0; i < b->areas.size(); i++) {
         if (which && !which[i]) continue;
         bsm_area* a = b->areas[i];
@@ -902,7 +908,6 @@ int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,
             a->layout_epoch++;
         }
     }
-    BSM_HIP(hipStreamSynchronize(b->stream));
     return 0;
 }
 

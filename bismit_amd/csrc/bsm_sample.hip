@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bsm_internal.h"
@@ -430,10 +431,64 @@ int bsm_sampler_recv(bsm_sampler* sp, int wait_for_frame, void* dst, uint64_t ds
 int bsm_sampler_recv_many(bsm_sampler* const* s, uint32_t n, int wait_for_frame,
                           void* const* dst, const uint64_t* dst_len, uint64_t* cycles, int* got) {
     if ((!s || !dst || !dst_len || !got) && n) return fail("null arg");
-    for (uint32_t i = 0; i < n; i++)
-        if (bsm_sampler_recv(s[i], wait_for_frame, dst[i], dst_len[i], cycles ? cycles + i : nullptr,
-                             got + i))
-            return 1;
+    // take every pending frame (waiting where asked), then copy them out --
+    // split over threads when there is much to copy -- then release them
+    struct Take {
+        uint32_t i;
+        std::shared_ptr<FrameBlock> blk;
+        size_t off;
+        uint64_t fc;
+    };
+    std::vector<Take> takes;
+    size_t bytes = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        got[i] = 0;
+        bsm_sampler* sp = s[i];
+        if (!sp) return fail("null sampler");
+        if (dst_len[i] < sp->len || !dst[i]) return fail("destination shorter than the sampler range");
+        std::unique_lock<std::mutex> lk(sp->m);
+        if (sp->closed) continue;
+        if (sp->state != SMP_PENDING) {
+            if (!wait_for_frame) continue;
+            sp->cv.wait(lk, [&] { return sp->state == SMP_PENDING || sp->closed; });
+            if (sp->closed || sp->state != SMP_PENDING) continue;
+        }
+        takes.push_back(Take{i, sp->blk, sp->blk_off, sp->frame_cycle});
+        bytes += sp->len;
+    }
+    const FrameBlock* last = nullptr;
+    for (const Take& t : takes) {
+        if (t.blk.get() == last) continue;
+        last = t.blk.get();
+        hipError_t e = hipEventSynchronize(t.blk->ev);
+        if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize (sampler frame)");
+    }
+    auto copy = [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; k++)
+            std::memcpy(dst[takes[k].i], takes[k].blk->host + takes[k].off, s[takes[k].i]->len);
+    };
+    const size_t nt = bytes >= ((size_t)4 << 20) ? std::min<size_t>(8, takes.size()) : 1;
+    if (nt > 1) {
+        std::vector<std::thread> pool;
+        for (size_t w = 0; w < nt; w++)
+            pool.emplace_back(copy, takes.size() * w / nt, takes.size() * (w + 1) / nt);
+        for (std::thread& th : pool) th.join();
+    } else {
+        copy(0, takes.size());
+    }
+    for (const Take& t : takes) {
+        bsm_sampler* sp = s[t.i];
+        {
+            std::lock_guard<std::mutex> g(sp->m);
+            if (!sp->closed) {
+                sp->state = SMP_EMPTY;
+                sp->blk.reset();
+            }
+            sp->cv.notify_all();
+        }
+        if (cycles) cycles[t.i] = t.fc;
+        got[t.i] = 1;
+    }
     return 0;
 }
 

@@ -164,10 +164,13 @@ class ReceiverSet:
     arrays built once (a server receiving every resident session's output each
     cycle)."""
 
-    def __init__(self, rxs):
+    def __init__(self, rxs, reuse=None):
+        """reuse: an earlier set whose buffers the receivers it shares keep."""
         self.rxs = list(rxs)
         n = len(self.rxs)
-        self.outs = [np.empty(rx.len, dtype=rx.dtype) for rx in self.rxs]
+        old = {id(rx): o for rx, o in zip(reuse.rxs, reuse.outs)} if reuse is not None else {}
+        self.outs = [old.get(id(rx)) if id(rx) in old else np.empty(rx.len, dtype=rx.dtype)
+                     for rx in self.rxs]
         self._hs = np.array([rx.h.value for rx in self.rxs], dtype=np.uint64)
         self._ds = np.array([o.ctypes.data for o in self.outs], dtype=np.uint64)
         self._ls = np.array([rx.len for rx in self.rxs], dtype=np.uint64)

@@ -162,6 +162,10 @@ class ServeStats:
     membership_s: list = field(default_factory=list)
     setup_s: list = field(default_factory=list)
     config: dict = field(default_factory=dict)
+    host_s: dict = field(default_factory=dict)    # loop phase -> seconds
+
+    def phase(self, name, dt):
+        self.host_s[name] = self.host_s.get(name, 0.0) + dt
 
     def summary(self):
         def pct(x, q):
@@ -196,6 +200,8 @@ class ServeStats:
             "session_setup_ms_mean": float(np.mean(self.setup_s) * 1e3) if self.setup_s else 0.0,
             "wall_s": self.wall_s,
         }
+        out["host_ms_per_cycle"] = {k: 1e3 * v / max(self.cycles, 1)
+                                    for k, v in sorted(self.host_s.items())}
         out.update(self.config)
         return out
 
@@ -206,18 +212,19 @@ class _Cycle:
 
 
 class Server:
-    def __init__(self, ctx, hbm_budget_bytes=None, max_sessions=None, epoch_s=0.02,
+    def __init__(self, ctx, hbm_budget_bytes=None, max_sessions=None, epoch_s=0.05,
                  use_graph=False, on_retire=None, setup_workers=4, regrow_every=201,
-                 regrow_reach=6, sample_outputs=True, on_output=None, max_wait_s=1.0,
-                 capacity=1.0e12, target_util=0.85, max_lag_s=0.1):
+                 regrow_reach=6, sample_outputs=True, on_output=None, max_wait_s=0.25,
+                 capacity=1.0e12, target_util=0.8, max_lag_s=0.1, initial_capacity=None):
         self.ctx = ctx
         # Masked cycles are enqueued directly by default: the batch changes with
         # most admission epochs, and capturing plus instantiating its two graphs
         # costs more latency than the launches per cycle a graph saves.
         self.use_graph = use_graph
         self.sample_outputs = sample_outputs
-        self.on_output = on_output            # on_output(session, [Frame per area]); the data
-                                              # stays valid until the session's next cycle
+        self.on_output = on_output            # on_output(session, [Frame per area]) after the
+                                              # cycle is counted (frame.cycle == session.done);
+                                              # the data stays valid until the session's next cycle
         self.on_retire = on_retire
         self.regrow_every = regrow_every      # 0: regrowth off
         self.regrow_reach = regrow_reach
@@ -227,7 +234,13 @@ class Server:
         self.max_sessions = max_sessions      # None: HBM and compute decide
         self.epoch_s = epoch_s
         self.max_wait_s = max_wait_s          # None: sessions wait as long as it takes
-        self.capacity = capacity              # synapse-updates/s the GPU sustains (hint)
+        self.capacity = capacity              # synapse-updates/s the GPU sustains at most
+        self.initial_capacity = initial_capacity or capacity
+        self.capacity_est = self.initial_capacity   # moved by _observe
+        self.window_s = 0.5
+        self.ramp = 0.1                       # new demand admitted per epoch, of the estimate
+        self._win = (0.0, 0.0)
+        self._idle_s = 0.0
         self.target_util = target_util
         self.max_lag_s = max_lag_s
         self.setup_workers = setup_workers
@@ -255,16 +268,32 @@ class Server:
     def session_demand(self, spec):
         return sum(self._side(s)[1] for s in spec.sides) * spec.rate_hz
 
+    def _observe(self, st, now):
+        """Capacity estimate, every `window_s`: when resident sessions fall
+        behind their clients, what the server delivered in that window is all it
+        sustains, and the estimate drops to it; while they keep up with demand
+        close to the admission limit, the estimate rises 2% (up to `capacity`)."""
+        w0, u0 = self._win
+        if now - w0 < self.window_s:
+            return
+        rate = (st.synapse_updates - u0) / (now - w0)
+        if not self._keeping_up(now):
+            self.capacity_est = min(self.capacity_est, rate)
+        elif self.demand >= 0.95 * self.target_util * self.capacity_est:
+            self.capacity_est = min(self.capacity, 1.02 * self.capacity_est)
+        self._win = (now, st.synapse_updates)
+
     def _keeping_up(self, now):
         """Resident sessions (joined over half a second ago) keep up with their
         clients: 90th percentile of how long their next frame has waited."""
         lags = [now - s.produced(s.next) for s in self.resident
-                if s.next < s.spec.n_frames and s.t_join is not None and now - s.t_join > 0.5]
+                if s.next < s.spec.n_frames and s.t_join is not None and now - s.t_join > 0.25]
         return not lags or float(np.percentile(lags, 90)) <= self.max_lag_s
 
     def _admit(self, waiting, setting_up, pool, st, now):
         if not self._keeping_up(now):
             return
+        budget = self.ramp * self.capacity_est   # demand this epoch may add
         while waiting:
             spec, t_arr = waiting[0]
             if self.max_sessions is not None and \
@@ -274,8 +303,11 @@ class Server:
             d = self.session_demand(spec)
             if self.used + fp > self.budget:
                 break
-            if self.resident and self.demand + d > self.target_util * self.capacity:
+            if self.resident and self.demand + d > self.target_util * self.capacity_est:
                 break
+            if d > budget and budget < self.ramp * self.capacity_est:
+                break
+            budget -= d
             waiting.popleft()
             self.used += fp
             self.demand += d
@@ -315,8 +347,9 @@ class Server:
             for s in retired:
                 self.used -= s.fp
                 self.demand -= s.demand
-                s.close()
                 st.completed += 1
+            # freeing device memory waits for the device: off the loop thread
+            self._pool.submit(lambda ss=retired: [x.close() for x in ss])
         if ready:
             self.batch.add([a for s in ready for a in s.areas])
             for s in ready:
@@ -328,7 +361,7 @@ class Server:
             s.pos = np.array([pos[id(a)] for a in s.areas], np.int64)
             s.rx = np.arange(len(rxs), len(rxs) + len(s.outputs), dtype=np.int64)
             rxs.extend(s.outputs)
-        self.recvset = ReceiverSet(rxs)
+        self.recvset = ReceiverSet(rxs, reuse=self.recvset)
         st.membership_s.append(time.perf_counter() - t0)
         st.membership_changes += 1
         st.max_resident = max(st.max_resident, len(self.resident))
@@ -368,26 +401,30 @@ class Server:
         fo[pos] = c.ofs | np.uint64(L.FRAME_SDR)
         self.batch.cycle_masked(mask, self._buf[:c.total], fo, graph=self.use_graph)
 
-    def _finish(self, c, st, clock):
-        """Receives the cycle's output frames, hands them out, records latency
-        and regrows the sessions whose cycle count is due."""
-        got = self.recvset.recv(wait_for_frame=False)
+    def _receive(self, c, clock):
+        """Waits for the cycle in flight and receives its output frames; counts
+        the cycle for its sessions and enqueues the regrowth it makes due (on the
+        batch stream, ahead of the next cycle)."""
+        rs = self.recvset
+        t0 = clock()
+        while not self.batch.query():      # the cycle's completion, polled
+            time.sleep(0)
+        self._stats.phase("receive_gpu_wait", clock() - t0)
+        got = rs.recv(wait_for_frame=False)
         if not self.sample_outputs:
             self.batch.sync()
         t = clock()
-        for s, prod in zip(c.sessions, c.produced):
-            if self.sample_outputs:
-                if not all(got[i] for i in s.rx):
+        # the frames, looked up before a membership change rebuilds the receiver set
+        frames = None
+        if self.sample_outputs:
+            for s in c.sessions:
+                if not got[s.rx].all():
                     raise RuntimeError(f"session {s.spec.sid}: output frame missing")
-                if self.on_output:
-                    self.on_output(s, [Frame(self.recvset.outs[i], int(self.recvset.cycles[i]))
-                                       for i in s.rx])
+            if self.on_output:
+                frames = [[Frame(rs.outs[i], int(rs.cycles[i])) for i in s.rx]
+                          for s in c.sessions]
+        for s in c.sessions:
             s.done += 1
-            st.latencies.append(t - prod)
-            st.service.append(t - max(prod, s.t_join))
-        st.cycles += 1
-        st.synapse_updates += c.syn
-        st.active_areas += c.n_areas
         if self.regrow_every:
             due = [s for s in c.sessions if s.done % self.regrow_every == 0]
             if due:
@@ -401,17 +438,38 @@ class Server:
                         seeds[p] = s.regrow_seed(k)
                         epochs[p] = s.done // self.regrow_every - 1
                 self.batch.regrow(seeds, epochs, self.regrow_reach, which)
+        return frames, t
+
+    def _account(self, c, frames, t, st):
+        """Hands the received frames out and records the cycle's latencies (the
+        next cycle is already running)."""
+        if frames is not None:
+            for s, f in zip(c.sessions, frames):
+                self.on_output(s, f)
+        prod = np.array(c.produced)
+        st.latencies.extend((t - prod).tolist())
+        join = np.array([s.t_join for s in c.sessions])
+        st.service.extend((t - np.maximum(prod, join)).tolist())
+        st.cycles += 1
+        st.synapse_updates += c.syn
+        st.active_areas += c.n_areas
 
     # ------------------------------------------------------------ loop
     def run(self, specs, wall_limit_s=30.0, time_scale=1.0):
         """Serve `specs` (arrival times in seconds, scaled by time_scale) until
         every session is done or rejected, or wall_limit_s passed."""
         with ThreadPoolExecutor(max_workers=self.setup_workers) as pool:
+            self._pool = pool
             return self._run(specs, wall_limit_s, time_scale, pool)
 
     def _run(self, specs, wall_limit_s, time_scale, pool):
         st = ServeStats()
-        st.config = {"hbm_budget_gb": self.budget / 2**30, "capacity_hint": self.capacity,
+        self._stats = st
+        self.capacity_est = self.initial_capacity
+        self._win = (0.0, 0.0)
+        self._idle_s = 0.0
+        st.config = {"hbm_budget_gb": self.budget / 2**30, "capacity_max": self.capacity,
+                     "capacity_initial": self.initial_capacity,
                      "target_util": self.target_util, "max_wait_s": self.max_wait_s,
                      "max_lag_ms": self.max_lag_s * 1e3, "max_sessions": self.max_sessions}
         self.batch = Batch([], self.ctx, dynamic=True)
@@ -442,15 +500,32 @@ class Server:
             if now - last_epoch >= self.epoch_s:
                 last_epoch = now
                 self._admit(waiting, setting_up, pool, st, now)
-            # the next cycle's frames are made while the GPU runs the current one
+            # the next cycle's frames are made while the GPU runs the current one;
+            # the current one's frames are handed out while the next one runs
+            t0 = clock()
             nxt = self._prepare(now)
+            t1 = clock()
+            st.phase("prepare", t1 - t0)
+            done = None
             if inflight is not None:
-                self._finish(inflight, st, clock)
-                inflight = None
+                got, t_got = self._receive(inflight, clock)
+                done, inflight = (inflight, got, t_got), None
+                t2 = clock()
+                st.phase("receive", t2 - t1)
+                t1 = t2
             self._membership(setting_up, st, clock())
+            t2 = clock()
+            st.phase("membership", t2 - t1)
             if nxt is not None:
                 self._enqueue(nxt)
                 inflight = nxt
+            t3 = clock()
+            st.phase("enqueue", t3 - t2)
+            if done is not None:
+                self._account(*done, st)
+                self._observe(st, clock())
+            st.phase("account", clock() - t3)
+            if inflight is not None:
                 continue
             if not self.resident and not pending and not waiting and not setting_up:
                 with self._ready_lock:
@@ -461,10 +536,15 @@ class Server:
             if pending:
                 wake.append(pending[0].arrival * time_scale)
             nxt_t = min(wake) if wake else now + 1e-3
-            time.sleep(max(0.0, min(nxt_t - clock(), 0.002)))
+            t_sl = clock()
+            time.sleep(max(0.0, min(nxt_t - t_sl, 0.002)))
+            self._idle_s += clock() - t_sl
         if inflight is not None:
-            self._finish(inflight, st, clock)
+            got, t_got = self._receive(inflight, clock)
+            self._account(inflight, got, t_got, st)
         st.wall_s = clock()
+        st.config["capacity_estimate_end"] = self.capacity_est
+        st.config["loop_idle_frac"] = self._idle_s / max(st.wall_s, 1e-9)
         busy = ctypes.c_double()
         L.check(self.ctx.lib.bsm_batch_busy_ms(self.batch.h, ctypes.byref(busy)))
         st.gpu_busy_s = busy.value / 1000.0

@@ -464,8 +464,9 @@ int bsm_batch_busy_ms(bsm_batch* b, double* ms);
 /* Regrowth of batched areas (the batch's form of bsm_area_regrow, same draws):
  * area i regrows with seeds[i] and epochs[i] when which is null or which[i] is
  * nonzero, so each area of a dynamic batch keeps its own 201-cycle schedule
- * (src/cortex/cortical_area.rs:792-806).  Waits for the batch's enqueued cycles
- * and completes on return; a widened source summary re-plans the next cycle. */
+ * (src/cortex/cortical_area.rs:792-806).  Enqueued on the batch stream after
+ * its enqueued cycles and before the next (buffer I/O on the areas waits for
+ * it); a widened source summary re-plans the next cycle. */
 int bsm_batch_regrow(bsm_batch* b, const uint8_t* which, const uint64_t* seeds,
                      const uint64_t* epochs, int32_t reach);
 /* Inter-area tracts (thalamic routing, src/subcortex/thalamus/thalamus.rs:71-285,
