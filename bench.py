@@ -30,9 +30,9 @@ def engine_basis(syn_bs):
     per = " or ".join(f"{b} B" for b in sorted(set(syn_bs)))
     return (f"compulsory bytes of this design's launch: {per} per synapse by tuft (a 2-bit "
             "staged-slice code when the tuft's gather stages several slices, offsets packed into "
-            "1 B when |v|,|u| <= 7, strength read, new state written; the previous-active flag "
-            "is derived from the previous state buffer) + 4 B per dendrite + 12 B per pyramidal "
-            "cell-tuft")
+            "1 B when |v|,|u| <= 7, one strength-sign bit; states are virtual: none written, "
+            "recomputed from a snapshot of the staged slices where read) + 4 B per dendrite + "
+            "12 B per pyramidal cell-tuft + the snapshot copy of the staged slices")
 
 
 def cpu_info():
@@ -63,15 +63,21 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
 def syn_bytes(stage, tuft):
-    """Per-synapse HBM bytes of one tuft in an F1 launch: the source slice (the
-    check-free gather reads a 2-bit staged-slice code when it stages several
-    slices and nothing for one; the checked gather the 1 B id), the offsets
-    (1 B packed, else 2 B), strength in and state out."""
+    """Per-synapse HBM bytes of one tuft in an F1 launch.  Check-free gather
+    (virtual states): a 2-bit staged-slice code when it stages several slices
+    and nothing for one, the offsets (1 B packed, else 2 B) and one strength-sign
+    bit; no state is written.  Checked gather: the 1 B slice id, 2 B offsets,
+    strength in and state out."""
     if stage["fast"]:
-        slc = 0 if len(tuft.src_slcs) <= 1 else 0.25
-    else:
-        slc = 1
-    return slc + (1 if stage["packed"] else 2) + 2
+        slc = 0 if len(set(tuft.src_slcs)) <= 1 else 0.25
+        return slc + (1 if stage["packed"] else 2) + 0.125
+    return 1 + 2 + 2
+
+
+def snap_bytes(scheme, tuft):
+    """Bytes of a check-free tuft's snapshot: its staged source slices."""
+    sl = scheme.slices[min(tuft.src_slcs)]
+    return len(set(tuft.src_slcs)) * sl.v_size * sl.u_size
 
 
 def stage_bytes(scheme, li, kind, stage):
@@ -84,9 +90,11 @@ def stage_bytes(scheme, li, kind, stage):
     if kind == 1:
         b = 0
         for t in lay.tufts:
-            b += C * t.syns_per_tft * syn_bytes(stage, t) + C * t.dens_per_tft * 4
+            b += int(C * t.syns_per_tft * syn_bytes(stage, t)) + C * t.dens_per_tft * 4
             if lay.kind == 2:
                 b += C * 12
+            if stage["fast"]:
+                b += 2 * snap_bytes(scheme, t)
         if lay.kind == 1:
             b += C
         return b
@@ -456,9 +464,13 @@ def main():
                            "GB/s peak: the byte count is not what the launch moves")
     traffic = read_traffic()
     ref8 = REF_BYTES_PER_SYNAPSE * syn_launch
-    # the calibration kernel with the launch's most read streams (syn_bytes - 1) and
-    # its one write stream
-    ceil = stream_ceiling(ctx, n_reads=int(max(syn_bs)) - 1)
+    # the calibration kernel with the launch's access pattern: read-only streams for
+    # the check-free gather (no synapse byte written), else its read streams
+    # (syn_bytes - 1) and one write stream
+    if dstage["fast"]:
+        ceil = stream_ceiling(ctx, n_reads=1, write=False)
+    else:
+        ceil = stream_ceiling(ctx, n_reads=int(max(syn_bs)) - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(frac, 4),
                 "traffic": traffic,
@@ -518,22 +530,26 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stream_ceiling(ctx, n_reads=4, n=128 << 20, reps=5):
+def stream_ceiling(ctx, n_reads=4, write=True, n=128 << 20, reps=5):
     """Best of `reps` launches of the stream calibration kernel (F1's read
-    streams and its one write stream, no gather): a practical HBM ceiling."""
+    streams and, for the checked gather, its one write stream; no gather): a
+    practical HBM ceiling."""
     import ctypes as C
     from bismit_amd import _lib as L
-    src, dst = ctx.alloc(n_reads * n), ctx.alloc(n)
+    src = ctx.alloc(n_reads * n)
+    dst = ctx.alloc(n) if write else None
     best = None
     for _ in range(reps):
         ms = C.c_float()
         L.check(ctx.lib.bsm_calib_stream(ctx.h, dst, src, n, n_reads, C.byref(ms)))
         best = ms.value if best is None else min(best, ms.value)
     ctx.free(src)
-    ctx.free(dst)
-    return {"gbs": round((n_reads + 1) * n / (best / 1000.0) / 1e9, 1),
-            "pattern": f"{n_reads} non-temporal 16-B read streams + 1 16-B write stream, "
-                       f"{n >> 20} MiB each"}
+    if write:
+        ctx.free(dst)
+    return {"gbs": round((n_reads + (1 if write else 0)) * n / (best / 1000.0) / 1e9, 1),
+            "pattern": f"{n_reads} non-temporal 16-B read stream(s)" +
+                       (" + 1 16-B write stream" if write else ", nothing written") +
+                       f", {n >> 20} MiB each"}
 
 
 def C_float_copy(ctx, dst, src, n):

@@ -155,6 +155,10 @@ static void plan_area(bsm_area* a, ArenaPlan& ap) {
         ap.add(&L.syn_vu, ns);
         ap.add(&L.syn_sb, L.n_sb);
         ap.add(&L.syn_str, ns);
+        ap.add(&L.syn_sg, L.n_sg);
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) ap.add(&L.snap[t], 2 * (size_t)L.snap_cap[t]);
+        ap.add(&L.vbits, sizeof(uint32_t) * BSM_MAX_TUFTS);
+        ap.add(&L.vt_d, sizeof(VirtTuft) * BSM_MAX_TUFTS);
         ap.add(&L.den_states, nd);
         ap.add(&L.den_raw, nd);
         ap.add(&L.den_energies, nd);
@@ -182,15 +186,26 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
             T.syn_idz = ns;
             T.den_idz = nd;
             T.sb_idz = nsb;
-            nsb += ((uint64_t)L.cells * T.spt + 3) / 4;
+            nsb += ((uint64_t)L.cells * T.spt + 15) / 16 * 4;
             // zero-filled buffers: slice 0, offsets 0
             T.reach_v = T.reach_u = T.reach_max = 0;
             T.slc_bits[0] = 1;
             T.slc_bits[1] = T.slc_bits[2] = T.slc_bits[3] = 0;
             ns += (uint64_t)L.cells * T.spt;
             nd += (uint64_t)L.cells * T.d.dens_per_tft;
+            // snapshot room: every distinct source slice (a fast tuft's staged
+            // slices are among them)
+            uint64_t cap = 0;
+            for (uint32_t k = 0; k < T.d.n_src_slcs && k < BSM_MAX_TUFT_SRCS; k++) {
+                bool dup = false;
+                for (uint32_t m = 0; m < k; m++) dup |= T.d.src_slc_ids[m] == T.d.src_slc_ids[k];
+                const bsm_slice_desc& sd = d->slices[T.d.src_slc_ids[k]];
+                if (!dup) cap += (uint64_t)sd.v_size * sd.u_size;
+            }
+            L.snap_cap[t] = (uint32_t)cap;
         }
         L.n_syn = ns;
+        L.n_sg = (ns + 31) / 32 * 4;
         L.n_den = nd;
         L.n_sb = nsb;
         L.n_celtft = (uint64_t)L.cells * L.d.n_tufts;
@@ -230,21 +245,21 @@ __global__ __launch_bounds__(256) void k_pack_vu(const int8_t* __restrict__ v,
         vu[i] = (uint8_t)((((v[i] + 8) & 15) << 4) | ((u[i] + 8) & 15));
 }
 
-// Staged-slice codes for the gather: byte b of a tuft holds (id - base) & 3 of
-// its synapses 4b..4b+3 (tuft-relative; bits 2k..2k+1 for synapse 4b+k), exact
-// when every id lies in [base, base + 4) (the planner reads them only then).
-// Bytes [b0, b1) of one tuft of n synapses.
+// Staged-slice codes for the gather: 32-bit word g of a tuft holds
+// (id - base) & 3 of its synapses 16g..16g+15, synapse 16g + 4w + k at bits
+// 8k + 2w (tuft-relative), exact when every id lies in [base, base + 4) (the
+// planner reads them only then).  Words [g0, g1) of one tuft of n synapses.
 __global__ __launch_bounds__(256) void k_pack_sb(const uint8_t* __restrict__ slc,
-                                                 uint8_t* __restrict__ sb, uint64_t n,
-                                                 uint64_t b0, uint64_t b1, uint32_t base) {
-    for (uint64_t b = b0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < b1;
-         b += (uint64_t)gridDim.x * blockDim.x) {
+                                                 uint32_t* __restrict__ sb, uint64_t n,
+                                                 uint64_t g0, uint64_t g1, uint32_t base) {
+    for (uint64_t g = g0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < g1;
+         g += (uint64_t)gridDim.x * blockDim.x) {
         uint32_t code = 0;
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint64_t i = 4 * b + k;
-            if (i < n) code |= ((slc[i] - base) & 3u) << (2 * k);
+        for (uint32_t j = 0; j < 16; j++) {
+            const uint64_t i = 16 * g + j;
+            if (i < n) code |= ((slc[i] - base) & 3u) << (8 * (j & 3u) + 2 * (j >> 2));
         }
-        sb[b] = (uint8_t)code;
+        sb[g] = code;
     }
 }
 
@@ -264,10 +279,131 @@ int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, si
         if (lo >= hi) continue;
         uint32_t staged[4];
         const uint32_t base = stage_slices(a, T, staged) ? staged[0] : 0u;
-        const uint64_t b0 = (lo - T.syn_idz) / 4, b1 = (hi - T.syn_idz + 3) / 4;
-        const uint32_t g = (uint32_t)std::min<uint64_t>((b1 - b0 + 255) / 256, 8192);
-        k_pack_sb<<<g, 256, 0, s>>>(L.syn_slc + T.syn_idz, L.syn_sb + T.sb_idz, n, b0, b1, base);
+        const uint64_t g0 = (lo - T.syn_idz) / 16, g1 = (hi - T.syn_idz + 15) / 16;
+        const uint32_t g = (uint32_t)std::min<uint64_t>((g1 - g0 + 255) / 256, 8192);
+        k_pack_sb<<<g, 256, 0, s>>>(L.syn_slc + T.syn_idz, (uint32_t*)(L.syn_sb + T.sb_idz), n, g0,
+                                    g1, base);
         BSM_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Strength signs: word w of a layer holds (strength < 0) of synapses 32w..32w+31,
+// synapse 32w + 4j + k at bit 8k + j.  Words [w0, w1) of a layer of n synapses.
+__global__ __launch_bounds__(256) void k_pack_sg(const int8_t* __restrict__ str,
+                                                 uint32_t* __restrict__ sg, uint64_t n,
+                                                 uint64_t w0, uint64_t w1) {
+    for (uint64_t w = w0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < w1;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t bits = 0;
+        for (uint32_t k = 0; k < 32; k++) {
+            const uint64_t i = 32 * w + k;
+            if (i < n && str[i] < 0) bits |= 1u << (8 * (k & 3u) + (k >> 2));
+        }
+        sg[w] = bits;
+    }
+}
+
+int launch_pack_sg(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len) {
+    LayerRt& L = a->L[layer];
+    if (offset >= L.n_syn || !len) return 0;
+    len = std::min(len, (size_t)L.n_syn - offset);
+    const uint64_t w0 = offset / 32, w1 = (offset + len + 31) / 32;
+    const uint32_t g = (uint32_t)std::min<uint64_t>((w1 - w0 + 255) / 256, 8192);
+    k_pack_sg<<<g, 256, 0, s>>>(L.syn_str, L.syn_sg, L.n_syn, w0, w1);
+    BSM_HIP(hipGetLastError());
+    return 0;
+}
+
+// Parity q's states of tuft synapses [lo, hi) into st[q], for each parity whose
+// states are virtual (cl/syns.cl:2-4 on the snapshot's source axon).
+__global__ __launch_bounds__(256) void k_materialize(const VirtTuft* __restrict__ vt,
+                                                     uint64_t lo, uint64_t hi) {
+    const VirtTuft T = *vt;
+    const uint32_t vb = *BSM_G(const uint32_t, T.vbits) & 3u;
+    if (!vb) return;
+    for (uint64_t i = lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < hi;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t cel = (uint32_t)(i / T.spt);
+        const uint32_t sid = BSM_G(const uint8_t, T.slc)[i];
+        const int32_t vo = BSM_G(const int8_t, T.vofs)[i], uo = BSM_G(const int8_t, T.uofs)[i];
+        for (uint32_t q = 0; q < 2; q++)
+            if ((vb >> q) & 1u)
+                BSM_G(uint8_t, T.st[q])[i] =
+                    (uint8_t)syn_fire(virt_axon(T, T.snap + (size_t)q * T.snap_len, cel, sid, vo, uo));
+    }
+}
+
+__global__ void k_vbits_clear(uint32_t* vbits, uint32_t n) {
+    if (threadIdx.x < n) BSM_G(uint32_t, vbits)[threadIdx.x] = 0u;
+}
+
+int settle_states(bsm_area* a, uint32_t layer, hipStream_t s, size_t offset, size_t len,
+                  bool clear) {
+    LayerRt& L = a->L[layer];
+    if (clear) {
+        offset = 0;
+        len = L.n_syn;
+    }
+    for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+        const TuftRt& T = L.t[t];
+        const uint64_t n = (uint64_t)L.cells * T.spt;
+        const uint64_t lo = std::max<uint64_t>(offset, T.syn_idz);
+        const uint64_t hi = std::min<uint64_t>(offset + len, T.syn_idz + n);
+        if (lo >= hi) continue;
+        const uint32_t g = (uint32_t)std::min<uint64_t>((hi - lo + 255) / 256, 16384);
+        k_materialize<<<g, 256, 0, s>>>(L.vt_d + t, lo - T.syn_idz, hi - T.syn_idz);
+        BSM_HIP(hipGetLastError());
+    }
+    if (clear) {
+        k_vbits_clear<<<1, 64, 0, s>>>(L.vbits, BSM_MAX_TUFTS);
+        BSM_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+int settle_area(bsm_area* a, hipStream_t s) {
+    for (uint32_t l = 0; l < a->n_layers; l++)
+        if (settle_states(a, l, s, 0, 0, true)) return 1;
+    return 0;
+}
+
+int upload_virt(bsm_area* a, hipStream_t s) {
+    for (uint32_t l = 0; l < a->n_layers; l++) {
+        LayerRt& L = a->L[l];
+        std::vector<VirtTuft> h(BSM_MAX_TUFTS);
+        for (uint32_t t = 0; t < L.d.n_tufts; t++) {
+            const TuftRt& T = L.t[t];
+            VirtTuft& v = h[t];
+            uint32_t staged[4];
+            const uint32_t ns = stage_slices(a, T, staged);
+            v.snap = L.snap[t];
+            v.st[0] = L.syn_state[0] + T.syn_idz;
+            v.st[1] = L.syn_state[1] + T.syn_idz;
+            v.slc = L.syn_slc + T.syn_idz;
+            v.vofs = L.syn_v + T.syn_idz;
+            v.uofs = L.syn_u + T.syn_idz;
+            v.vbits = L.vbits + t;
+            v.n = (uint64_t)L.cells * T.spt;
+            v.spt = T.spt;
+            v.VU = a->V * a->U;
+            v.U = a->U;
+            if (ns) {
+                const SliceRt& s0 = a->slices_h[staged[0]];
+                v.base = staged[0];
+                v.vs = s0.v_size;
+                v.us = s0.u_size;
+                v.vsc = s0.v_scale;
+                v.usc = s0.u_scale;
+                v.vmid = s0.v_mid;
+                v.umid = s0.u_mid;
+                v.slice_len = (uint32_t)s0.v_size * s0.u_size;
+                v.snap_len = std::min(ns * v.slice_len, L.snap_cap[t]);
+            }
+        }
+        BSM_HIP(hipStreamSynchronize(s));
+        BSM_HIP(hipMemcpy(L.vt_d, h.data(), sizeof(VirtTuft) * BSM_MAX_TUFTS,
+                          hipMemcpyHostToDevice));
     }
     return 0;
 }
@@ -486,6 +622,8 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
     *widened = false;
     for (uint32_t l = 0; l < a->n_layers; l++) {
         LayerRt& L = a->L[l];
+        // the regrown synapses' states stay those of their old sources
+        if (settle_states(a, l, s, 0, 0, true)) return 1;
         for (uint32_t t = 0; t < L.d.n_tufts; t++) {
             TuftRt& T = L.t[t];
             const uint32_t n_src = std::min<uint32_t>(T.d.n_src_slcs, BSM_MAX_TUFT_SRCS);
@@ -518,6 +656,7 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
                                               (int8_t*)L.syn_str + T.syn_idz);
                 BSM_HIP(hipGetLastError());
                 if (launch_pack_vu(s, a, l, T.syn_idz, (size_t)L.cells * T.spt)) return 1;
+                if (launch_pack_sg(s, a, l, T.syn_idz, (size_t)L.cells * T.spt)) return 1;
             }
             // the source summary widens to the regrown sources; the fused plan
             // stays exact while the summary is unchanged
@@ -533,6 +672,14 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
         }
     }
     return 0;
+}
+
+// Buffers whose writes change synapse states as the cycle left them: the
+// states themselves, the flags (their previous-active bit is the previous
+// state) and the sources (a virtual state follows its synapse's source).
+static bool changes_states(int buf) {
+    return buf == BSM_SYN_STATES || buf == BSM_SYN_FLAG_SETS || buf == BSM_SYN_SRC_SLC_IDS ||
+           buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS;
 }
 
 // Buffer I/O on an area inside a batch first waits for the batch's enqueued
@@ -682,6 +829,7 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     BSM_HIP(hipMemcpyAsync(a->slices_d, a->slices_h.data(), sizeof(SliceRt) * a->n_slices,
                            hipMemcpyHostToDevice, a->stream));
     BSM_HIP(hipStreamSynchronize(a->stream));
+    if (upload_virt(a, a->stream)) return 1;
     BSM_HIP(hipEventCreate(&a->ev0));
     BSM_HIP(hipEventCreate(&a->ev1));
     for (int i = 0; i < 2; i++) {
@@ -824,6 +972,10 @@ int bsm_area_read(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t le
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("read range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
+    // virtual states of the range into the state buffers (the parity bits stay)
+    if ((buf == BSM_SYN_STATES || buf == BSM_SYN_FLAG_SETS) && len &&
+        settle_states(a, layer, a->stream, offset, len, false))
+        return 1;
     BSM_HIP(hipStreamSynchronize(a->stream));
     if (len == 0) return 0;
     BSM_HIP(hipMemcpy(dst, p + offset, len, hipMemcpyDeviceToHost));
@@ -848,6 +1000,7 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     if (offset > n || len > n - offset) return fail("write range out of bounds");
     BSM_HIP(hipSetDevice(a->ctx->device));
+    if (len && changes_states(buf) && settle_states(a, layer, a->stream, 0, 0, true)) return 1;
     BSM_HIP(hipStreamSynchronize(a->stream));
     if (len == 0) return 0;
     if (buf == BSM_SYN_FLAG_SETS) {
@@ -870,6 +1023,10 @@ int bsm_area_write(bsm_area* a, int buf, uint32_t layer, size_t offset, size_t l
         if (launch_pack_vu(a->stream, a, layer, offset, len)) return 1;
         BSM_HIP(hipStreamSynchronize(a->stream));
     }
+    if (buf == BSM_SYN_STRENGTHS) {
+        if (launch_pack_sg(a->stream, a, layer, offset, len)) return 1;
+        BSM_HIP(hipStreamSynchronize(a->stream));
+    }
     return 0;
 }
 
@@ -880,6 +1037,7 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
     size_t n;
     if (resolve_buf(a, buf, layer, &p, &n)) return 1;
     BSM_HIP(hipSetDevice(a->ctx->device));
+    if (changes_states(buf) && settle_states(a, layer, a->stream, 0, 0, true)) return 1;
     if (buf == BSM_SYN_FLAG_SETS) {
         LayerRt& L = a->L[layer];
         BSM_HIP(hipMemsetAsync(p, val & ~SYN_PREV_ACTIVE, n, a->stream));
@@ -892,6 +1050,7 @@ int bsm_area_fill(bsm_area* a, int buf, uint32_t layer, uint8_t val) {
         if (buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS ||
             buf == BSM_SYN_SRC_SLC_IDS)
             if (launch_pack_vu(a->stream, a, layer, 0, n)) return 1;
+        if (buf == BSM_SYN_STRENGTHS && launch_pack_sg(a->stream, a, layer, 0, n)) return 1;
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;
@@ -1031,6 +1190,7 @@ int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
     BSM_HIP(hipSetDevice(a->ctx->device));
     for (uint32_t l = 0; l < a->n_layers; l++) {
         LayerRt& L = a->L[l];
+        if (settle_states(a, l, a->stream, 0, 0, true)) return 1;
         for (uint32_t t = 0; t < L.d.n_tufts; t++) {
             TuftRt& T = L.t[t];
             if (T.d.n_src_slcs == 0) return fail("tuft has no source slices");
@@ -1042,6 +1202,7 @@ int bsm_area_init_layout(bsm_area* a, uint64_t seed, int32_t reach) {
         }
         if (launch_pack_vu(a->stream, a, l, 0, L.n_syn)) return 1;
         BSM_HIP(hipMemsetAsync(L.syn_str, 0, L.n_syn, a->stream));
+        if (launch_pack_sg(a->stream, a, l, 0, L.n_syn)) return 1;
     }
     BSM_HIP(hipStreamSynchronize(a->stream));
     return 0;

@@ -116,6 +116,18 @@ __device__ inline uint32_t load_codes(Rsrc r, uint32_t off) {
     else return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, BUF_NT);
 }
 
+// Bit 7 of each byte of x gathered into bits 0..3 (byte k -> bit k).
+__device__ __forceinline__ uint32_t sign_bits4(uint32_t x) {
+    return (((x >> 7) & 0x01010101u) * 0x01020408u) >> 24;
+}
+// Bits 0..3 of b spread to the low bit of bytes 0..3.
+__device__ __forceinline__ uint32_t spread4(uint32_t b) { return (b * 0x00204081u) & 0x01010101u; }
+
+// Strength-sign words: word g of a layer covers synapses 32g..32g+31; synapse i
+// (w = (i % 32) / 4, k = i % 4) sits at bit 8k + w, so the four synapses of one
+// 4-byte word of a synapse stream are bit w of each byte.
+__device__ __forceinline__ uint32_t sg_pos(uint32_t i) { return 8u * (i & 3u) + ((i & 31u) >> 2); }
+
 // The epilogue's global inputs of one tile, loaded when the tile starts so that
 // their latency hides behind the synapse loop: the first TAIL_PD dendrites and
 // TAIL_PC cell-tufts of each thread (bytes packed four to a register).  Every
@@ -136,10 +148,53 @@ __device__ inline uint32_t tile_cell(uint32_t ct, uint32_t tvn, uint32_t tun, ui
     return sl * VU + (v0 + r) * U + (u0 + cc);
 }
 
-__device__ inline void tail_prefetch(const TuftJob& J, uint32_t depth, uint32_t tvn,
-                                     uint32_t tun, uint32_t v0, uint32_t u0, uint32_t VU,
-                                     uint32_t U, uint32_t tid, uint32_t nth, TailPre& pre) {
-    const uint32_t dens = J.dens, cells_t = depth * tvn * tun, nd_t = cells_t * dens;
+// n / d as a multiply-high with magic = ceil(2^32 / d) (d > 1), exact whenever
+// n * d < 2^32 (the rounding error n * (magic * d - 2^32) / 2^32 stays below
+// the gap to the next multiple of d); d == 1 passes n through.
+__device__ __forceinline__ uint32_t magic_u32(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
+__device__ __forceinline__ uint32_t quot_u32(uint32_t n, uint32_t d, uint32_t magic) {
+    return d > 1 ? __umulhi(n, magic) : n;
+}
+// The same quotient without a branch: d == 1 takes m = 2^32 - 1 and adds 1 to
+// n first (floor((n + 1)(2^32 - 1) / 2^32) = n for n + 1 < 2^32).
+struct Quot {
+    uint32_t m, e;
+    __device__ static Quot of(uint32_t d) { return d > 1 ? Quot{0xFFFFFFFFu / d + 1u, 0u} : Quot{0xFFFFFFFFu, 1u}; }
+    __device__ uint32_t operator()(uint32_t n) const { return __umulhi(n + e, m); }
+};
+
+// Tile cell ct (slice-major rows of tun columns) -> layer cell, and tile
+// dendrite x -> (cell, dendrite of the cell).  DivMap divides (checked path);
+// RowMap reads the first cell of each tile row from LDS and divides by
+// multiply-high (check-free path; every product stays below 2^32).
+struct DivMap {
+    uint32_t tvn, tun, v0, u0, VU, U, dens;
+    __device__ uint32_t cell(uint32_t ct) const { return tile_cell(ct, tvn, tun, v0, u0, VU, U); }
+    __device__ uint32_t den_cell(uint32_t x, uint32_t& dj) const {
+        const uint32_t ct = x / dens;
+        dj = x - ct * dens;
+        return ct;
+    }
+};
+struct RowMap {
+    const uint32_t* rowc;
+    uint32_t tun, dens;
+    Quot q_tun, q_dens;
+    __device__ uint32_t cell(uint32_t ct) const {
+        const uint32_t r = q_tun(ct);
+        return rowc[r] + (ct - r * tun);
+    }
+    __device__ uint32_t den_cell(uint32_t x, uint32_t& dj) const {
+        const uint32_t ct = q_dens(x);
+        dj = x - ct * dens;
+        return ct;
+    }
+};
+
+template <class Map>
+__device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t nd_t,
+                                     uint32_t cells_t, uint32_t tid, uint32_t nth, TailPre& pre) {
+    const uint32_t dens = J.dens;
     auto den_act = BSM_G(const uint8_t, J.den_act);
     auto energy = BSM_G(const uint8_t, J.ssc_energy);
 #pragma unroll
@@ -151,8 +206,9 @@ __device__ inline void tail_prefetch(const TuftJob& J, uint32_t depth, uint32_t 
     for (uint32_t k = 0; k < TAIL_PD; k++) {
         const uint32_t x = tid + k * nth;
         if (x < nd_t) {
-            const uint32_t ct = x / dens, dj = x - ct * dens;
-            const uint32_t dl = tile_cell(ct, tvn, tun, v0, u0, VU, U) * dens + dj;
+            uint32_t dj;
+            const uint32_t ct = map.den_cell(x, dj);
+            const uint32_t dl = map.cell(ct) * dens + dj;
             const uint32_t dg = J.den_idz + dl;
             pre.act[k / 4] |= (uint32_t)den_act[dl] << (8 * (k % 4));
             if (J.ssc_energy != nullptr && dg < J.C)
@@ -164,7 +220,7 @@ __device__ inline void tail_prefetch(const TuftJob& J, uint32_t depth, uint32_t 
         pre.tft[k] = 0;
         const uint32_t ct = tid + k * nth;
         if (J.tft[0] != nullptr && ct < cells_t) {
-            const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
+            const uint32_t cel = map.cell(ct);
             pre.tft[k] = (uint32_t)BSM_G(const uint8_t, J.tft[0])[cel] |
                          ((uint32_t)BSM_G(const uint8_t, J.tft[1])[cel] << 8) |
                          ((uint32_t)BSM_G(const uint8_t, J.tft[2])[cel] << 16) |
@@ -175,15 +231,13 @@ __device__ inline void tail_prefetch(const TuftJob& J, uint32_t depth, uint32_t 
 
 // Dendrites (den_cycle_tft, ssc_cycle on the SSC soma) and tufts (tft_cycle) of
 // one tile from the per-chunk (sum, raw) partials in LDS; ends with a barrier.
-template <int W>
+template <class Map>
 __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* acc,
-                                 uint32_t stage_cap, uint32_t depth, uint32_t tvn, uint32_t tun,
-                                 uint32_t v0, uint32_t u0, uint32_t VU, uint32_t U, int32_t rnd,
-                                 uint32_t tid, uint32_t nth, const TailPre& pre) {
+                                 uint32_t stage_cap, uint32_t nd_t, uint32_t cells_t,
+                                 const Map& map, int32_t rnd, uint32_t tid, uint32_t nth,
+                                 const TailPre& pre) {
     // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
     const uint32_t dens = J.dens;
-    const uint32_t cells_t = depth * tvn * tun;
-    const uint32_t nd_t = cells_t * dens;
     const bool do_tft = J.tft[0] != nullptr;
     uint8_t* denc = lds + stage_cap + nd_t * 4;
     auto den_act = BSM_G(uint8_t, J.den_act);
@@ -191,11 +245,12 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
     auto den_st = BSM_G(uint8_t, J.den_st);
     auto energy = BSM_G(uint8_t, J.ssc_energy);
     auto den_one = [&](uint32_t x, uint32_t act0, uint32_t e0, bool have) {
-        const uint32_t ct = x / dens, dj = x - ct * dens;
+        uint32_t dj;
+        const uint32_t ct = map.den_cell(x, dj);
         const uint32_t pv = acc[x];
         uint32_t sum = pv & 0xFFFFu;
         const uint32_t raw = pv >> 16;
-        const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
+        const uint32_t cel = map.cell(ct);
         const uint32_t dl = cel * dens + dj;
         const uint32_t dg = J.den_idz + dl;
         const int32_t active = sum > J.thresh;
@@ -242,7 +297,7 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
                 asum += s;
             }
             const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
-            const uint32_t cel = tile_cell(ct, tvn, tun, v0, u0, VU, U);
+            const uint32_t cel = map.cell(ct);
             auto t0 = BSM_G(uint8_t, J.tft[0]), t1 = BSM_G(uint8_t, J.tft[1]);
             auto t2 = BSM_G(uint8_t, J.tft[2]), t3 = BSM_G(uint8_t, J.tft[3]);
             // the current values become prev_* (prefetched for the first TAIL_PC)
@@ -266,14 +321,6 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
     __syncthreads();
 }
 
-// n / d as a multiply-high with magic = ceil(2^32 / d) (d > 1), exact whenever
-// n * d < 2^32 (the rounding error n * (magic * d - 2^32) / 2^32 stays below
-// the gap to the next multiple of d); d == 1 passes n through.
-__device__ __forceinline__ uint32_t magic_u32(uint32_t d) { return d > 1 ? 0xFFFFFFFFu / d + 1u : 0u; }
-__device__ __forceinline__ uint32_t quot_u32(uint32_t n, uint32_t d, uint32_t magic) {
-    return d > 1 ? __umulhi(n, magic) : n;
-}
-
 // Adds one chunk's per-word (sum, raw) partials to the per-dendrite packed
 // accumulators (sum in bits 0-15, raw in bits 16-31; a dendrite's raw sum is at
 // most 256*255 < 2^16, so the halves never carry into each other).  A chunk
@@ -295,7 +342,244 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
     }
 }
 
-template <int W, int NST, bool FAST, bool PACKED>
+// The check-free gather's loads of one chunk, kept as loaded: every value
+// derived from them is computed in work(), so that a chunk's loads stay in
+// flight while the previous chunk is gathered.
+template <int NW>
+struct ChunkMeta {
+    uint32_t v[NW], u[NW];   // packed offsets (or v / u offsets)
+    uint32_t c, csh;         // staged-slice code word of the chunk's group, its shift
+    uint32_t s, ssh;         // strength-sign word, its shift
+    uint32_t so, den0, off0;
+    int32_t lb;
+};
+
+// F1, check-free form: every synapse of each job sources a staged slice with
+// |offsets| <= halo (proved on the host), and the staged slices are
+// consecutive ids of one geometry (contiguous in the axon space).  Per tile:
+// the window of each staged slice (tile + halo, zeros off the slice) is staged
+// in LDS as syn_fire of each axon (cl/syns.cl:2-4), so the gather reads fired
+// states directly; slot j of a several-slice window starts 32 rows after slot
+// j - 1, so that with packed offsets the 2-bit staged-slice code merges into
+// the v nibble (v + 8 + 32 * code, < 128) and one dot product per synapse forms
+// its window address.  Synapse states are virtual: no state is written; the
+// item's share of the snapshot of the staged slices is copied instead.  Per
+// dendrite, (sum over non-negative strengths, raw sum) come from a dot product
+// with the strength-sign bits and a sum of absolute differences.
+template <int W, int NST, bool PACKED>
+__global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
+                                                    const WorkItem* __restrict__ items,
+                                                    uint32_t n_items, uint32_t par,
+                                                    uint32_t rnd_ofs, uint32_t stage_cap) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr uint32_t NTH = 256;   // launched with 256 threads
+    constexpr int NW = Chunk<W>::NW;
+    const uint32_t tid = threadIdx.x;
+    uint32_t* acc = (uint32_t*)(lds + stage_cap);
+
+    for (uint32_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+        const WorkItem wi = items[it];
+        const TuftJob& J = jobs[wi.job];
+        const uint32_t V = J.V, U = J.U, depth = J.depth, spt = J.spt, dens = J.dens;
+        const uint32_t TV = J.tile_v, TU = J.tile_u;
+        const uint32_t v0 = (uint32_t)wi.tv * TV, u0 = (uint32_t)wi.tu * TU;
+        const uint32_t tvn = min(TV, V - v0), tun = min(TU, U - u0);
+        uint32_t p = par ^ J.par_xor;
+        if (J.dyn) {  // dynamic batch: skip areas that do not cycle now (uniform per block)
+            const uint32_t d = *J.dyn;
+            if (!(d & 2u)) continue;
+            p = d & 1u;
+        }
+        const uint32_t pu = __builtin_amdgcn_readfirstlane(p) & 1u;
+        const uint32_t rows = depth * tvn, cells_t = rows * tun, nd_t = cells_t * dens;
+        for (uint32_t x = tid; x < nd_t; x += NTH) acc[x] = 0u;
+        const SliceRt s0 = J.slices[J.staged[0]];
+        // virtual states: this item's share of the snapshot of the staged slices,
+        // and the parity's bit
+        {
+            const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;
+            const uint32_t lo = idx * J.snap_per, hi = min(lo + J.snap_per, J.snap_len);
+            auto src = BSM_G(const uint8_t, J.axn + s0.idz);
+            auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
+            for (uint32_t x = lo + tid; x < hi; x += NTH) dst[x] = src[x];
+            if (idx == 0 && tid == 0) {
+                auto vb = BSM_G(uint32_t, J.vbits);
+                *vb = *vb | (1u << pu);
+            }
+        }
+        // uniform per item: the geometry in scalar registers
+        const uint32_t vsc = __builtin_amdgcn_readfirstlane(s0.v_scale);
+        const uint32_t usc = __builtin_amdgcn_readfirstlane(s0.u_scale);
+        const int32_t vmid = (int32_t)__builtin_amdgcn_readfirstlane(s0.v_mid);
+        const int32_t umid = (int32_t)__builtin_amdgcn_readfirstlane(s0.u_mid);
+        const uint32_t svs = __builtin_amdgcn_readfirstlane(s0.v_size);
+        const uint32_t sus = __builtin_amdgcn_readfirstlane(s0.u_size);
+        const int32_t ih = (int32_t)J.halo;
+        const int32_t a0 = (int32_t)((v0 * vsc) >> 4) + vmid - ih;
+        const int32_t b0 = (int32_t)((u0 * usc) >> 4) + umid - ih;
+        const uint32_t nr = (uint32_t)((int32_t)(((v0 + tvn - 1) * vsc) >> 4) -
+                                       (int32_t)((v0 * vsc) >> 4) + 1 + 2 * ih);
+        const uint32_t nc = (uint32_t)((int32_t)(((u0 + tun - 1) * usc) >> 4) -
+                                       (int32_t)((u0 * usc) >> 4) + 1 + 2 * ih);
+        const uint32_t WS = nr * nc;
+        // slot stride of the staged windows (32 rows with packed offsets)
+        const uint32_t WSS = (PACKED && NST > 1) ? 32u * nc : WS;
+        const uint32_t n_st = J.n_staged, SS = svs * sus;
+        const Quot q_nc = Quot::of(nc);
+        for (uint32_t j = 0; j < n_st; j++) {
+            auto src = BSM_G(const uint8_t, J.axn + s0.idz + j * SS);
+            for (uint32_t x = tid; x < WS; x += NTH) {
+                const uint32_t rr = q_nc(x), cc = x - rr * nc;
+                const int32_t gv = a0 + (int32_t)rr, gu = b0 + (int32_t)cc;
+                uint32_t val = 0;
+                if ((uint32_t)gv < svs && (uint32_t)gu < sus) val = src[(uint32_t)gv * sus + (uint32_t)gu];
+                lds[j * WSS + x] = (uint8_t)syn_fire(val);
+            }
+        }
+        // per tile row (slice, v): synapse offset and window offset of its first
+        // cell, and the cell itself
+        uint32_t* rowt = (uint32_t*)(lds + (((n_st - 1) * WSS + WS + 15) & ~15u));
+        uint32_t* rowc = rowt + 2 * rows;
+        const uint32_t VU = V * U;
+        {
+            const Quot q_tvn = Quot::of(tvn);
+            for (uint32_t rq = tid; rq < rows; rq += NTH) {
+                const uint32_t sl = q_tvn(rq), v = v0 + (rq - sl * tvn);
+                const uint32_t c0 = sl * VU + v * U + u0;
+                rowc[rq] = c0;
+                rowt[2 * rq] = c0 * spt;
+                rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * (int32_t)nc +
+                                              umid - b0);
+            }
+        }
+        __syncthreads();
+        const RowMap map{rowc, tun, dens, Quot::of(tun), Quot::of(dens)};
+        TailPre pre;
+        tail_prefetch(J, map, nd_t, cells_t, tid, NTH, pre);
+        const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
+
+        // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
+        const uint32_t nrec = J.C * spt;
+        const bool multi = NST > 1 && n_st > 1;
+        // staged-slice codes: 32-bit words of 16 synapses (synapse (w, k) of a
+        // group at bit 8k + 2w); a job with one staged slice reads a zero-length
+        // stream, whose loads return 0 without touching memory
+        const Rsrc r_slc = make_rsrc(J.sb, multi ? (nrec + 15) / 16 * 4 : 0u);
+        const Rsrc r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
+        const Rsrc r_u = make_rsrc(J.uofs, nrec);
+        const Rsrc r_sg = make_rsrc(J.sg, J.sg_len);
+        const uint32_t sgb0 = J.sg_bit0;
+        // packed offsets carry v + 8 and u + 8: their bias folds into the base
+        const int32_t pbias = PACKED ? 8 * (int32_t)nc + 8 : 0;
+        const uint32_t coefA = (nc & 0xFFu) | (1u << 8);
+        const uint32_t coefB = ((nc & 0xFFu) << 16) | (1u << 24);
+        const uint32_t CPT = spt / W, row_chunks = tun * CPT, n_chunks = rows * row_chunks;
+        const Quot q_rc = Quot::of(row_chunks), q_cpt = Quot::of(CPT);
+        const bool whole = (J.spd % W) == 0;
+        const uint32_t m_spd = 0xFFFFFFFFu / J.spd + 1u;
+        const uint32_t last = n_chunks - 1;
+
+        auto fetch = [&](uint32_t q, ChunkMeta<NW>& m) {
+            const uint32_t rowq = q_rc(q);
+            const uint32_t rem = q - __umul24(rowq, row_chunks);
+            const uint32_t cc = q_cpt(rem);
+            m.den0 = __umul24(__umul24(rowq, tun) + cc, dens);
+            m.off0 = (rem - __umul24(cc, CPT)) * W;
+            const uint32_t so = rowt[2 * rowq] + rem * W;
+            m.so = so;
+            m.lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
+            if constexpr (NST > 1) {
+                m.c = __builtin_amdgcn_raw_buffer_load_b32(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
+                m.csh = 2u * ((so & 15u) >> 2);
+            }
+            const uint32_t sb = sgb0 + so;
+            m.s = __builtin_amdgcn_raw_buffer_load_b32(r_sg, (int)((sb >> 5) << 2), 0, BUF_NT);
+            m.ssh = (sb & 31u) >> 2;
+            Chunk<W>::load_b(r_v, so, m.v);   // packed: (v + 8) << 4 | (u + 8) per byte
+            if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, m.u);
+        };
+        auto work = [&](const ChunkMeta<NW>& m, bool valid) {
+            const int32_t lbp = m.lb - pbias;
+            uint32_t cw = 0;
+            if constexpr (NST > 1) cw = m.c >> m.csh;    // word w of the chunk: bits 8k + 2w
+            const uint32_t nsg = ~(m.s >> m.ssh);        // word w: bit 8k + w set if strength >= 0
+            uint32_t x[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                uint32_t a[4];
+                if constexpr (PACKED) {
+                    uint32_t vw = (m.v[w] >> 4) & 0x0F0F0F0Fu;
+                    const uint32_t uw = m.v[w] & 0x0F0F0F0Fu;
+                    if constexpr (NST > 1) {
+                        // code k of this word to bits 5-6 of byte k: v + 8 + 32 * code
+                        const uint32_t cs = (2 * w <= 5) ? (cw << (5 - 2 * w)) : (cw >> (2 * w - 5));
+                        vw |= cs & 0x60606060u;
+                    }
+                    // [v_b, u_b, v_b+1, u_b+1] byte pairs -> dot4 gives base + v*cols + u
+                    const uint32_t p01 = __builtin_amdgcn_perm(uw, vw, 0x05010400u);
+                    const uint32_t p23 = __builtin_amdgcn_perm(uw, vw, 0x07030602u);
+                    a[0] = __builtin_amdgcn_udot4(p01, coefA, (uint32_t)lbp, false);
+                    a[1] = __builtin_amdgcn_udot4(p01, coefB, (uint32_t)lbp, false);
+                    a[2] = __builtin_amdgcn_udot4(p23, coefA, (uint32_t)lbp, false);
+                    a[3] = __builtin_amdgcn_udot4(p23, coefB, (uint32_t)lbp, false);
+                } else {
+                    const uint32_t p01 = __builtin_amdgcn_perm(m.u[w], m.v[w], 0x05010400u);
+                    const uint32_t p23 = __builtin_amdgcn_perm(m.u[w], m.v[w], 0x07030602u);
+                    int32_t l[4] = {lbp, lbp, lbp, lbp};
+                    if constexpr (NST > 1) {
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            l[k] = (int32_t)__umul24((cw >> (8 * k + 2 * w)) & 3u, WSS) + lbp;
+                    }
+                    a[0] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p01, (int32_t)coefA, l[0], false);
+                    a[1] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p01, (int32_t)coefB, l[1], false);
+                    a[2] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefA, l[2], false);
+                    a[3] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefB, l[3], false);
+                }
+                x[w] = (uint32_t)lds[a[0]] | ((uint32_t)lds[a[1]] << 8) |
+                       ((uint32_t)lds[a[2]] << 16) | ((uint32_t)lds[a[3]] << 24);
+            }
+            if (whole) {
+                uint32_t ps = 0, rs = 0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    ps = __builtin_amdgcn_udot4(x[w], (nsg >> w) & 0x01010101u, ps, false);
+                    rs = __builtin_amdgcn_sad_u8(x[w], 0u, rs);
+                }
+                if (valid) atomicAdd(&acc[m.den0 + __umulhi(m.off0, m_spd)], ps | (rs << 16));
+            } else if (valid) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const uint32_t ps = __builtin_amdgcn_udot4(x[w], (nsg >> w) & 0x01010101u, 0u, false);
+                    const uint32_t rs = __builtin_amdgcn_sad_u8(x[w], 0u, 0u);
+                    atomicAdd(&acc[m.den0 + __umulhi(m.off0 + (uint32_t)(4 * w), m_spd)],
+                              ps | (rs << 16));
+                }
+            }
+        };
+        // two chunks per iteration, both loaded before the first is gathered (the
+        // second's loads stay in flight meanwhile); the loads are unconditional,
+        // their indices clamped to the last chunk, so that each gather waits for
+        // its own loads only
+        ChunkMeta<NW> A, B;
+        for (uint32_t qa = tid; qa < n_chunks; qa += 2 * NTH) {
+            const uint32_t qb = qa + NTH;
+            // A's loads, then B's, both ahead of the first gather
+            fetch(qa, A);
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(min(qb, last), B);
+            __builtin_amdgcn_sched_barrier(0);
+            work(A, true);
+            work(B, qb < n_chunks);   // past the end: B re-gathers the last chunk, unused
+        }
+        __syncthreads();
+        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, NTH, pre);
+    }
+}
+
+// F1, checked form: any offsets, non-staged slices and the scalar char-wrapped
+// mid quirk (Q10); staged windows with bounds checks and a global fallback.
+template <int W, int NST>
 __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
                                                const WorkItem* __restrict__ items, uint32_t n_items,
                                                uint32_t par, uint32_t rnd_ofs,
@@ -320,155 +604,13 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
             p = d & 1u;
         }
         const bool quirk = J.scalar_quirk != 0;
-        const uint32_t nd_t = depth * tvn * tun * J.dens;
+        const uint32_t cells_t = depth * tvn * tun, nd_t = cells_t * J.dens;
+        const DivMap map{tvn, tun, v0, u0, V * U, U, J.dens};
         TailPre pre;
-        tail_prefetch(J, depth, tvn, tun, v0, u0, V * U, U, tid, nth, pre);
+        tail_prefetch(J, map, nd_t, cells_t, tid, nth, pre);
         for (uint32_t x = tid; x < nd_t; x += nth) acc[x] = 0u;
         const bool whole = (J.spd % W) == 0;
         const uint32_t magic = 0xFFFFFFFFu / J.spd + 1u;
-
-        if constexpr (FAST) {
-            // Every synapse of the job sources a staged slice with |offsets| <= halo
-            // (proved on the host), and the staged slices share one geometry: stage
-            // the full tile + halo window of each (zeros off the slice) and gather
-            // without any per-synapse test.
-            const SliceRt s0 = J.slices[J.staged[0]];
-            // uniform per item: keep the geometry (and the parity) in scalar registers
-            const uint32_t vsc = __builtin_amdgcn_readfirstlane(s0.v_scale);
-            const uint32_t usc = __builtin_amdgcn_readfirstlane(s0.u_scale);
-            const int32_t vmid = (int32_t)__builtin_amdgcn_readfirstlane(s0.v_mid);
-            const int32_t umid = (int32_t)__builtin_amdgcn_readfirstlane(s0.u_mid);
-            const uint32_t pu = __builtin_amdgcn_readfirstlane(p);
-            const int32_t ih = (int32_t)halo;
-            const int32_t a0 = (int32_t)((v0 * vsc) >> 4) + vmid - ih;
-            const int32_t b0 = (int32_t)((u0 * usc) >> 4) + umid - ih;
-            const int32_t nr = (int32_t)(((v0 + tvn - 1) * vsc) >> 4) -
-                               (int32_t)((v0 * vsc) >> 4) + 1 + 2 * ih;
-            const int32_t nc = (int32_t)(((u0 + tun - 1) * usc) >> 4) -
-                               (int32_t)((u0 * usc) >> 4) + 1 + 2 * ih;
-            const uint32_t WS = (uint32_t)(nr * nc);
-            const uint32_t CPT = spt / W;
-            const uint32_t rows = depth * tvn, row_chunks = tun * CPT;
-            const uint32_t n_chunks = rows * row_chunks;
-            const uint32_t VU = V * U;
-            for (uint32_t j = 0; j < J.n_staged; j++) {
-                const SliceRt s = J.slices[J.staged[j]];
-                auto src = BSM_G(const uint8_t, J.axn + s.idz);
-                for (uint32_t x = tid; x < WS; x += nth) {
-                    const int32_t rr = (int32_t)(x / (uint32_t)nc), cc = (int32_t)x - rr * nc;
-                    const int32_t gv = a0 + rr, gu = b0 + cc;
-                    uint8_t val = 0;
-                    if (gv >= 0 && gv < (int32_t)s.v_size && gu >= 0 && gu < (int32_t)s.u_size)
-                        val = src[(uint32_t)gv * s.u_size + (uint32_t)gu];
-                    lds[j * WS + x] = val;
-                }
-            }
-            // per tile row (slice, v): byte offset of its first cell's synapses, and
-            // the window offset of that row's centre minus the first column's
-            uint32_t* rowt = (uint32_t*)(lds + ((J.n_staged * WS + 15) & ~15u));
-            // a job with one staged slice in a several-slice launch reads no
-            // slice codes (its stream below is empty, so every code reads as 0)
-            const bool multi = NST > 1 && J.n_staged > 1;
-            for (uint32_t rq = tid; rq < rows; rq += nth) {
-                const uint32_t sl = rq / tvn, v = v0 + (rq - sl * tvn);
-                rowt[2 * rq] = (sl * VU + v * U + u0) * spt;
-                rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * nc +
-                                              umid - b0);
-            }
-            __syncthreads();
-            const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
-            // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
-            const uint32_t nrec = J.C * spt;
-            // a zero-length stream: its loads return 0 without touching memory
-            // staged-slice codes: 2 bits per synapse (spt is a multiple of W >= 4)
-            const Rsrc r_slc = make_rsrc(J.sb, multi ? nrec / 4 : 0u);
-            const Rsrc r_v = make_rsrc(PACKED ? J.vu : (const uint8_t*)J.vofs, nrec);
-            const Rsrc r_u = make_rsrc(J.uofs, nrec), r_str = make_rsrc(J.str, nrec);
-            // packed offsets carry v + 8 and u + 8: their bias folds into the base
-            const int32_t pbias = PACKED ? 8 * nc + 8 : 0;
-            const Rsrc r_out = make_rsrc(J.st[pu & 1], nrec);
-            const int32_t coefA = (nc & 0xFF) | (1 << 8);
-            const int32_t coefB = ((nc & 0xFF) << 16) | (1 << 24);
-            // chunk index -> (row, chunk in row) -> (column, chunk in cell) by
-            // multiply-high quotients (exact: the planner keeps n_chunks < 2^16)
-            const uint32_t m_rc = magic_u32(row_chunks), m_cpt = magic_u32(CPT);
-            auto fetch = [&](uint32_t q, uint32_t* ms, uint32_t* mv, uint32_t* mu, uint32_t* mt,
-                             uint32_t& so, int32_t& lb, uint32_t& den0, uint32_t& off0) {
-                const uint32_t rowq = quot_u32(q, row_chunks, m_rc);
-                const uint32_t rem = q - __umul24(rowq, row_chunks);
-                const uint32_t cc = quot_u32(rem, CPT, m_cpt);
-                den0 = __umul24(__umul24(rowq, tun) + cc, J.dens);
-                off0 = (rem - __umul24(cc, CPT)) * W;
-                so = rowt[2 * rowq] + rem * W;
-                lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
-                if constexpr (NST > 1) ms[0] = load_codes<W>(r_slc, so >> 2);
-                Chunk<W>::load_b(r_v, so, mv);   // packed: (v + 8) << 4 | (u + 8) per byte
-                if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, mu);
-                Chunk<W>::load_b(r_str, so, mt);
-            };
-            auto work = [&](const uint32_t* ms, const uint32_t* mv, const uint32_t* mu,
-                            const uint32_t* mt, uint32_t so, int32_t lb, uint32_t den0,
-                            uint32_t off0) {
-                uint32_t out[NW], pw[NW];
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    uint32_t vw, uw;
-                    if constexpr (PACKED) {
-                        vw = (mv[w] >> 4) & 0x0F0F0F0Fu;
-                        uw = mv[w] & 0x0F0F0F0Fu;
-                    } else {
-                        vw = mv[w];
-                        uw = mu[w];
-                    }
-                    const int32_t lbp = lb - pbias;
-                    // [vo_b, uo_b, vo_b+1, uo_b+1] byte pairs -> dot4 gives lb + vo*cols + uo
-                    const uint32_t p01 = __builtin_amdgcn_perm(uw, vw, 0x05010400u);
-                    const uint32_t p23 = __builtin_amdgcn_perm(uw, vw, 0x07030602u);
-                    int32_t ad0 = __builtin_amdgcn_sdot4((int32_t)p01, coefA, lbp, false);
-                    int32_t ad1 = __builtin_amdgcn_sdot4((int32_t)p01, coefB, lbp, false);
-                    int32_t ad2 = __builtin_amdgcn_sdot4((int32_t)p23, coefA, lbp, false);
-                    int32_t ad3 = __builtin_amdgcn_sdot4((int32_t)p23, coefB, lbp, false);
-                    if (NST > 1) {  // window of the staged slice: code x WS
-                        const uint32_t cw = ms[0] >> (8 * w);
-                        ad0 += (int32_t)__umul24(cw & 3u, WS);
-                        ad1 += (int32_t)__umul24((cw >> 2) & 3u, WS);
-                        ad2 += (int32_t)__umul24((cw >> 4) & 3u, WS);
-                        ad3 += (int32_t)__umul24((cw >> 6) & 3u, WS);
-                    }
-                    const uint32_t x = (uint32_t)lds[ad0] | ((uint32_t)lds[ad1] << 8) |
-                                       ((uint32_t)lds[ad2] << 16) | ((uint32_t)lds[ad3] << 24);
-                    // syn_fire on four bytes: bit 7 = (axon != 0), low 7 bits = axon >> 1
-                    const uint32_t t = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
-                    const uint32_t stw = ((t | x) & 0x80808080u) | ((x >> 1) & 0x7F7F7F7Fu);
-                    out[w] = stw;
-                    const uint32_t neg = ((mt[w] & 0x80808080u) >> 7) * 0xFFu;
-                    pw[w] = __builtin_amdgcn_sad_u8(stw & ~neg, 0u, 0u) |
-                            (__builtin_amdgcn_sad_u8(stw, 0u, 0u) << 16);
-                }
-                Chunk<W>::store_b(r_out, so, out);
-                acc_chunk<NW, 4>(acc, den0, off0, pw, whole, magic);
-            };
-            // two register sets in turn: the next chunk's metadata is in flight
-            // while the current one is gathered
-            uint32_t aS[NW], aV[NW], aU[NW], aT[NW], bS[NW], bV[NW], bU[NW], bT[NW];
-            uint32_t soA = 0, denA = 0, offA = 0, soB = 0, denB = 0, offB = 0;
-            int32_t lbA = 0, lbB = 0;
-            uint32_t q = tid;
-            if (q < n_chunks) fetch(q, aS, aV, aU, aT, soA, lbA, denA, offA);
-            while (q < n_chunks) {
-                const uint32_t qb = q + nth;
-                if (qb < n_chunks) fetch(qb, bS, bV, bU, bT, soB, lbB, denB, offB);
-                work(aS, aV, aU, aT, soA, lbA, denA, offA);
-                if (qb >= n_chunks) break;
-                const uint32_t qa = qb + nth;
-                if (qa < n_chunks) fetch(qa, aS, aV, aU, aT, soA, lbA, denA, offA);
-                work(bS, bV, bU, bT, soB, lbB, denB, offB);
-                q = qa;
-            }
-            __syncthreads();
-            tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth, pre);
-            continue;
-        }
 
         // ---- stage the tile's hex neighbourhood of each staged source slice
         int32_t vlo[NST], ulo[NST], rows[NST], cols[NST], lbase[NST];
@@ -509,6 +651,10 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         const uint32_t n_chunks = depth * tvn * row_chunks;
         const uint32_t VU = V * U;
         uint8_t* __restrict__ st_out = J.st[p & 1];
+        if (wi.tv == 0 && wi.tu == 0 && tid == 0) {  // parity p's states are written
+            auto vb = BSM_G(uint32_t, J.vbits);
+            *vb = *vb & ~(1u << (p & 1));
+        }
         for (uint32_t q = tid; q < n_chunks; q += nth) {
             const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
             const uint32_t sl = rowq / tvn, r = rowq - sl * tvn;
@@ -562,7 +708,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         __syncthreads();
 
-        tuft_tail<W>(J, lds, acc, stage_cap, depth, tvn, tun, v0, u0, VU, U, rnd, tid, nth, pre);
+        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, nth, pre);
     }
 }
 
@@ -583,8 +729,32 @@ struct LtpChunk {
         Chunk<W>::load(st + i0, ms);
         Chunk<W>::load(str + i0, mt);
     }
+    // Virtual states: the chunk's source axons in a snapshot stand in for the
+    // state bytes (a state is nonzero exactly when its axon is).  i0 is the
+    // layer index of the chunk, t0 the tuft's first synapse, cel its cell.
+    __device__ inline void load_virt(const VirtTuft& T, const uint8_t* snap,
+                                     const int8_t* __restrict__ str, uint32_t i0, uint32_t t0,
+                                     uint32_t cel) {
+        uint32_t sl[NW], vv[NW], uu[NW];
+        Chunk<W>::load(T.slc + (i0 - t0), sl);
+        Chunk<W>::load(T.vofs + (i0 - t0), vv);
+        Chunk<W>::load(T.uofs + (i0 - t0), uu);
+        Chunk<W>::load(str + i0, mt);
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            uint32_t o = 0;
+#pragma unroll
+            for (int b = 0; b < NB; b++)
+                o |= virt_axon(T, snap, cel, byte_of(sl[w], b), sbyte_of(vv[w], b),
+                               sbyte_of(uu[w], b)) << (8 * b);
+            ms[w] = o;
+        }
+    }
+    // New strengths of the chunk; the sign bits of those whose sign changed are
+    // flipped in the layer's sign words (a chunk of W <= 16 synapses lies in one
+    // word's 32 synapses).
     __device__ inline void apply(int8_t* __restrict__ str, uint32_t z, uint32_t i0, int32_t rnd,
-                                 uint32_t dec_mask) const {
+                                 uint32_t dec_mask, uint32_t* __restrict__ sg) const {
         uint32_t out[NW];
 #pragma unroll
         for (int w = 0; w < NW; w++) {
@@ -601,6 +771,15 @@ struct LtpChunk {
             out[w] = ow;
         }
         Chunk<W>::store(str + i0, out);
+        uint32_t flip = 0;
+        if constexpr (W == 1) {
+            if ((mt[0] ^ out[0]) & 0x80u) flip = 1u << sg_pos(i0);
+        } else {
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                flip |= spread4(sign_bits4(mt[w] ^ out[w])) << (((i0 & 31u) >> 2) + (uint32_t)w);
+        }
+        if (flip) atomicXor(sg + (i0 >> 5), flip);
     }
 };
 // LTP chunks each thread loads before it applies the first: four of the
@@ -735,6 +914,9 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             // chunk past the list loads chunk 0 again and is not applied)
             const uint32_t na = n_act, spt = J.spt, CPC = spt / W, total = na * CPC;
             const uint8_t* __restrict__ st = J.st[p & 1];
+            // this cycle's states: virtual when the gather kept a snapshot
+            const bool virt = (*BSM_G(const uint32_t, J.vt->vbits) >> (p & 1)) & 1u;
+            const uint8_t* snap = J.vt->snap + (size_t)(p & 1) * J.vt->snap_len;
             constexpr int LU = ltp_u<W>();
             for (uint32_t y0 = tid; y0 < total; y0 += LU * nth) {
                 LtpChunk<W> ch[LU];
@@ -745,11 +927,12 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     const uint32_t ci = y / CPC, k = y - ci * CPC;
                     zz[j] = list[ci] * spt;
                     ii[j] = zz[j] + k * W;
-                    ch[j].load(st, J.str, ii[j]);
+                    if (virt) ch[j].load_virt(*J.vt, snap, J.str, ii[j], 0u, list[ci]);
+                    else ch[j].load(st, J.str, ii[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < LU; j++)
-                    if (y0 + j * nth < total) ch[j].apply(J.str, zz[j], ii[j], rnd_m, 0x1FF);
+                    if (y0 + j * nth < total) ch[j].apply(J.str, zz[j], ii[j], rnd_m, 0x1FF, J.sg);
             }
         }
         __syncthreads();
@@ -884,7 +1067,8 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
             __syncthreads();
             if (learn) {
                 const uint32_t nl = n_l;
-                const uint8_t* __restrict__ prev_st = J.st[(p & 1) ^ 1];
+                const uint32_t pq = (p & 1) ^ 1;   // the previous cycle's states
+                const uint8_t* __restrict__ prev_st = J.st[pq];
                 const uint32_t maxch = J.max_chunks;
                 const int32_t r0 = J.rnd[rnd_ofs + J.slot_mtp0];
                 const int32_t r1 = n_tufts > 1 ? J.rnd[rnd_ofs + J.slot_mtp0 + 1] : 0;
@@ -908,13 +1092,20 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                         ok[j] = y < total && k * W < J.t_spd[tt[j]];
                         zz[j] = lz[li];
                         ii[j] = zz[j] + (ok[j] ? k * W : 0u);
-                        ch[j].load(prev_st, J.str, ii[j]);
+                        const VirtTuft& T = J.vt[tt[j]];
+                        if ((*BSM_G(const uint32_t, T.vbits) >> pq) & 1u) {
+                            const uint32_t t0 = J.t_syn_idz[tt[j]];
+                            ch[j].load_virt(T, T.snap + (size_t)pq * T.snap_len, J.str, ii[j], t0,
+                                            (zz[j] - t0) / J.t_spt[tt[j]]);
+                        } else {
+                            ch[j].load(prev_st, J.str, ii[j]);
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < LU; j++)
                         if (ok[j])
                             ch[j].apply(J.str, zz[j], ii[j],
-                                        tt[j] == 0 ? r0 : (tt[j] == 1 ? r1 : r2), 0xFFF);
+                                        tt[j] == 0 ? r0 : (tt[j] == 1 ? r1 : r2), 0xFFF, J.sg);
                 }
             }
             __syncthreads();
@@ -1036,6 +1227,18 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     }
     j.fast = fast ? 1u : 0u;
     if (fast) j.halo = T.reach_max;
+    j.vbits = L.vbits + t;
+    j.sg = (const uint8_t*)L.syn_sg;
+    j.sg_bit0 = (uint32_t)T.syn_idz;
+    j.sg_len = (uint32_t)L.n_sg;
+    // a fast tuft's states are virtual: the staged slices (consecutive ids of one
+    // geometry, so contiguous in the axon space) are its snapshot
+    j.snap = L.snap[t];
+    j.snap_len = fast ? n * (uint32_t)j.st_vsize[0] * j.st_usize[0] : 0u;
+    if (j.snap_len > L.snap_cap[t]) {
+        j.fast = 0;
+        j.snap_len = 0;
+    }
     // the packed offsets are exact when every |offset| fits in 3 bits + sign
     j.packed = (fast && T.reach_v <= PACK_REACH && T.reach_u <= PACK_REACH) ? 1u : 0u;
     return j;
@@ -1065,6 +1268,8 @@ SscJob make_ssc_job(bsm_area* a, uint32_t l, const int32_t* rnd) {
                                                    : 2u;
     j.learn = !(a->settings & BSM_SET_DISABLE_LEARNING);
     j.par_xor = 0;
+    j.vt = L.vt_d;
+    j.sg = L.syn_sg;
     return j;
 }
 
@@ -1119,16 +1324,20 @@ PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3) {
         if ((j.distal_mask >> t) & 1u) mc = std::max(mc, j.t_spd[t] / w);
     j.max_chunks = mc;
     j.par_xor = 0;
+    j.vt = L.vt_d;
+    j.sg = L.syn_sg;
     *w3 = w;
     return j;
 }
 
 static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fast) {
     if (fast) {
-        // windows, then 8 bytes per tile row (slice, v)
+        // windows (several staged slices with packed offsets: slots of 32 rows),
+        // then 12 bytes per tile row (slice, v)
         const uint32_t r = (((TV - 1) * j.st_vsc[0]) >> 4) + 2 + 2 * j.halo;
         const uint32_t c = (((TU - 1) * j.st_usc[0]) >> 4) + 2 + 2 * j.halo;
-        return ((j.n_staged * r * c + 15) & ~15u) + 8 * j.depth * TV;
+        const uint32_t slot = (j.packed && j.n_staged > 1) ? 32 * c : r * c;
+        return (((j.n_staged - 1) * slot + r * c + 15) & ~15u) + 12 * j.depth * TV;
     }
     uint32_t tot = 0;
     for (uint32_t k = 0; k < j.n_staged; k++) {
@@ -1153,6 +1362,23 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
     fast &= W >= 4;
     packed &= fast;
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
+    // several staged slices with packed offsets: window slots of 32 rows, so a
+    // window (tile rows + reach) must fit in 32 rows
+    const bool slots32 = fast && packed && nst > 1;
+    auto rows_fit = [&](uint32_t tv) {
+        if (!slots32) return true;
+        for (uint32_t id : job_ids) {
+            const TuftJob& j = jobs[id];
+            if ((((tv - 1) * j.st_vsc[0]) >> 4) + 2 + 2 * j.halo > 32) return false;
+        }
+        return true;
+    };
+    while (TV > 1 && !rows_fit(TV)) TV = (TV + 1) / 2;
+    if (!rows_fit(TV)) {
+        // reach too large for 32-row slots: every job takes the unpacked reads
+        for (uint32_t id : job_ids) jobs[id].packed = 0;
+        packed = false;
+    }
     uint32_t cap = 0, lds = 0;
     for (;;) {
         cap = 0;
@@ -1199,7 +1425,7 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
                 if (per_col * j.tile_v * j.tile_u >= 2 * 256) break;
                 uint32_t ntv = j.tile_v, ntu = j.tile_u;
                 if (ntu < 32 && 2 * ntu <= j.U) ntu *= 2;
-                else if (ntv < 64 && 2 * ntv <= j.V) ntv *= 2;
+                else if (ntv < 64 && 2 * ntv <= j.V && rows_fit(2 * ntv)) ntv *= 2;
                 else break;
                 if (ntv * ntu > 1024 || per_col * ntv * ntu >= 65536u ||
                     job_lds(j, ntv, ntu) > LDS_LIMIT)
@@ -1227,21 +1453,28 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
     st.kind = 1;
     st.item0 = (uint32_t)items.size();
     st.alg_bytes = 0;
-    // per synapse, in quarter bytes: the slice (fast: a 2-bit code when several
-    // slices are staged, nothing for one; else the 1 B id), the offsets (1 B
-    // packed, 2 B otherwise), the strength in and the state out
+    // per synapse, in eighth bytes: fast (virtual states): the 2-bit staged-slice
+    // code when several slices are staged (nothing for one), the offsets (1 B
+    // packed, 2 B otherwise) and the strength's sign bit, no state written;
+    // checked: the 1 B slice id, 2 B offsets, the strength in and the state out
     for (uint32_t id : job_ids) {
         // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
-        // out (pyramidal); per SSC cell its energy
+        // out (pyramidal); per SSC cell its energy; fast: the snapshot copy
         const TuftJob& j = jobs[id];
-        const uint32_t syn_q = (fast ? (j.n_staged <= 1 ? 0u : 1u) : 4u) + (packed ? 4u : 8u) + 8u;
-        st.alg_bytes += (uint64_t)j.C * j.spt * syn_q / 4 + (uint64_t)j.C * j.dens * 4 +
-                        (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0);
+        const uint32_t syn_e = fast ? (j.n_staged <= 1 ? 0u : 2u) + (packed ? 8u : 16u) + 1u
+                                    : 8u + 16u + 8u + 8u;
+        st.alg_bytes += (uint64_t)j.C * j.spt * syn_e / 8 + (uint64_t)j.C * j.dens * 4 +
+                        (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0) +
+                        (fast ? 2 * (uint64_t)j.snap_len : 0);
     }
     for (uint32_t id : job_ids) {
-        const TuftJob& j = jobs[id];
+        TuftJob& j = jobs[id];
         uint32_t ntv = (j.V + j.tile_v - 1) / j.tile_v, ntu = (j.U + j.tile_u - 1) / j.tile_u;
         if (ntv > 65535 || ntu > 65535 || id > 65535) return fail("too many tiles");
+        if (fast && !j.snap_len) return fail("check-free tuft without a snapshot");
+        j.ntu = ntu;
+        j.n_tiles = ntv * ntu;
+        j.snap_per = (j.snap_len + j.n_tiles - 1) / j.n_tiles;
         for (uint32_t a = 0; a < ntv; a++)
             for (uint32_t b = 0; b < ntu; b++)
                 items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)a, (uint16_t)b});
@@ -1430,15 +1663,14 @@ int build_plan(bsm_area* a) {
 
 #define F1_CASE(w, n)                                                                        \
     case (w * 8 + n):                                                                        \
-        kf_tuft<w, n, false, false><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
-                                                                       par, rnd_ofs,         \
-                                                                       st.stage_cap);        \
+        kf_tuft<w, n><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, par, rnd_ofs, \
+                                                         st.stage_cap);                      \
         break;
 #define F1_FAST_CASE(w, n, pk)                                                               \
     case ((w * 8 + n) * 2 + pk):                                                             \
-        kf_tuft<w, n, true, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items, \
-                                                                        par, rnd_ofs,        \
-                                                                        st.stage_cap);       \
+        kf_tuft_fast<w, n, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items,  \
+                                                                       par, rnd_ofs,         \
+                                                                       st.stage_cap);        \
         break;
 
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
@@ -1704,6 +1936,24 @@ __global__ __launch_bounds__(256) void k_calib_stream(const u32x4* __restrict__ 
         *BSM_G(u32x4, dst + i) = x;
     }
 }
+
+// The read streams alone (the check-free gather writes no synapse bytes): the
+// loads feed an xor that is stored only if it takes one fixed value, so that
+// they are not removed and almost nothing is written.
+template <int NR>
+__global__ __launch_bounds__(256) void k_calib_read(const u32x4* __restrict__ src,
+                                                    u32x4* __restrict__ sink, size_t n) {
+    u32x4 acc = {0u, 0u, 0u, 0u};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < NR; k++)
+            acc ^= __builtin_nontemporal_load(BSM_G(const u32x4, src + k * n + i));
+    }
+    if (acc.x == 0x9E3779B9u && acc.y == 0x7F4A7C15u && acc.z == 0xF39CC060u &&
+        acc.w == 0x5CEDC834u)
+        *BSM_G(u32x4, sink) = acc;
+}
 }  // namespace bsm
 
 extern "C" {
@@ -1760,7 +2010,7 @@ int bsm_area_stage_plan(bsm_area* a, uint32_t max_stages, bsm_stage_info* out,
 
 int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, uint32_t n_reads,
                      float* ms) {
-    if (!ctx || !dst || !src) return fail("null arg");
+    if (!ctx || !src) return fail("null arg");
     if (bytes % 16) return fail("bytes must be a multiple of 16");
     if (n_reads < 1 || n_reads > 4) return fail("n_reads must be in [1, 4]");
     BSM_HIP(hipSetDevice(ctx->device));
@@ -1770,7 +2020,17 @@ int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, uin
     const size_t n = bytes / 16;
     const uint32_t grid = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
     BSM_HIP(hipEventRecord(e0, 0));
-    switch (n_reads) {
+    if (!dst) {
+        // read-only: the sink is the first vector of src (written only if the xor
+        // of all loads takes one fixed value)
+        u32x4* sink = (u32x4*)const_cast<void*>(src);
+        switch (n_reads) {
+        case 1: k_calib_read<1><<<grid, 256, 0, 0>>>((const u32x4*)src, sink, n); break;
+        case 2: k_calib_read<2><<<grid, 256, 0, 0>>>((const u32x4*)src, sink, n); break;
+        case 3: k_calib_read<3><<<grid, 256, 0, 0>>>((const u32x4*)src, sink, n); break;
+        default: k_calib_read<4><<<grid, 256, 0, 0>>>((const u32x4*)src, sink, n); break;
+        }
+    } else switch (n_reads) {
     case 1: k_calib_stream<1><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
     case 2: k_calib_stream<2><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;
     case 3: k_calib_stream<3><<<grid, 256, 0, 0>>>((const u32x4*)src, (u32x4*)dst, n); break;

@@ -23,6 +23,45 @@ int hip_fail(hipError_t e, const char* what);
         if (e_ != hipSuccess) return ::bsm::hip_fail(e_, #call); \
     } while (0)
 
+// ---------------------------------------------------------------- virtual states
+// Synapse states of a check-free tuft are virtual: a state is syn_fire of the
+// source axon the gather read, so the gather keeps a snapshot of the staged
+// slices (contiguous in the axon space: consecutive ids of one geometry)
+// instead of writing one state byte per synapse.  Parity q's snapshot sits at
+// snap + q * snap_len; bit q of *vbits says that st[q] is stale and parity q's
+// states come from that snapshot.  Every reader recomputes them (the learning
+// stages, buffer reads, samplers); every writer of states or sources first
+// materialises both parities and clears the bits (settle_states).
+struct VirtTuft {
+    const uint8_t* snap;
+    uint8_t* st[2];           // tuft base in the two state buffers
+    const uint8_t* slc;       // tuft base of the source arrays (authoritative)
+    const int8_t* vofs;
+    const int8_t* uofs;
+    uint32_t* vbits;
+    uint64_t n;               // synapses of the tuft
+    uint32_t snap_len;        // bytes per parity: n_staged * slice_len
+    uint32_t slice_len;       // vs * us
+    uint32_t base;            // first staged slice id
+    uint32_t spt, VU, U;      // synapses per cell; layer columns
+    uint16_t vs, us;          // staged slices' geometry
+    uint8_t vsc, usc, vmid, umid;
+};
+
+// The source axon of one synapse of cell `cel` (layer index) in a snapshot:
+// the staged window of the gather, zero off the slice (coord_is_safe).
+__device__ __forceinline__ uint32_t virt_axon(const VirtTuft& T, const uint8_t* snap,
+                                              uint32_t cel, uint32_t sid, int32_t vo,
+                                              int32_t uo) {
+    const uint32_t vu = cel % T.VU, v = vu / T.U, u = vu - v * T.U;
+    const int32_t gv = (int32_t)((v * T.vsc) >> 4) + (int32_t)T.vmid + vo;
+    const int32_t gu = (int32_t)((u * T.usc) >> 4) + (int32_t)T.umid + uo;
+    const uint32_t code = sid - T.base;
+    if ((uint32_t)gv >= T.vs || (uint32_t)gu >= T.us || code * T.slice_len >= T.snap_len)
+        return 0u;
+    return BSM_G(const uint8_t, snap)[code * T.slice_len + (uint32_t)gv * T.us + (uint32_t)gu];
+}
+
 // ---------------------------------------------------------------- jobs
 // F1 work unit: one tuft of one layer of one area (gather + dendrites [+ tufts]).
 struct TuftJob {
@@ -33,7 +72,8 @@ struct TuftJob {
     const int8_t* uofs;
     const uint8_t* vu;     // packed offsets: (v + 8) << 4 | (u + 8), valid when |v|, |u| <= 7
     const uint8_t* sb;     // staged-slice codes: (slice id - staged[0]) & 3, 2 bits per
-                           // synapse, four to a byte (low bits first); valid when fast
+                           // synapse in 32-bit words of 16 synapses (synapse 4w + k of a
+                           // word's group at bit 8k + 2w); valid when fast
     const int8_t* str;
     uint8_t* st[2];        // tuft base in the two state buffers; st[par] is written
     uint8_t* den_raw;      // tuft dendrite base
@@ -57,6 +97,16 @@ struct TuftJob {
     uint32_t fast;         // every synapse sources a staged slice within `halo`
     uint32_t packed;       // fast, and every |offset| <= 7: the gather reads `vu`
     const uint32_t* dyn;   // dynamic batches: per (area, layer) word, bit0 parity, bit1 active
+    // virtual states (fast jobs): snapshot of the staged slices, parity bits
+    uint8_t* snap;         // parity q at snap + q * snap_len
+    uint32_t snap_len;
+    uint32_t* vbits;
+    // strength signs, one bit per synapse of the layer (bit sg_bit0 + i for
+    // synapse i of the tuft), read by the fast gather instead of the strengths
+    const uint8_t* sg;
+    uint32_t sg_bit0, sg_len;
+    uint32_t ntu, n_tiles; // the job's tiles per row and in all (set by the F1 planner)
+    uint32_t snap_per;     // snapshot bytes each tile copies (the last tile: the rest)
     // host-side planning copies of the staged slices' geometry
     uint16_t st_vsize[4], st_usize[4];
     uint8_t st_vsc[4], st_usc[4];
@@ -79,6 +129,8 @@ struct SscJob {
     uint32_t learn;
     uint32_t par_xor;
     const uint32_t* dyn;
+    const VirtTuft* vt;    // the layer's tuft (device)
+    uint32_t* sg;          // the layer's strength signs (32 per word)
 };
 
 // F3 work unit: pyramidal layer post-processing (soma + column control + distal LTP).
@@ -105,6 +157,8 @@ struct PyrJob {
     uint32_t par_xor;
     uint32_t max_chunks;   // max over distal tufts of syns_per_den / chunk
     const uint32_t* dyn;
+    const VirtTuft* vt;    // the layer's tufts (device array)
+    uint32_t* sg;          // the layer's strength signs (32 per word)
     uint32_t t_spd[BSM_MAX_TUFTS];
     uint32_t t_spt[BSM_MAX_TUFTS];
     uint32_t t_syn_idz[BSM_MAX_TUFTS];
@@ -135,7 +189,7 @@ struct LayerRt {
     bsm_layer_desc d;
     uint32_t cells;
     uint64_t n_syn, n_den, n_celtft;
-    uint64_t n_sb;           // bytes of syn_sb: ceil(synapses / 4) per tuft
+    uint64_t n_sb;           // bytes of syn_sb: 4 * ceil(synapses / 16) per tuft
     TuftRt t[BSM_MAX_TUFTS];
     uint8_t* syn_state[2];   // double-buffered synapse states
     int cur;                 // index of the current state buffer
@@ -146,6 +200,14 @@ struct LayerRt {
     uint8_t* syn_vu;         // packed copy of the offsets for the gather (see TuftJob::vu)
     uint8_t* syn_sb;         // staged-slice codes of each tuft for the gather (see TuftJob::sb)
     int8_t* syn_str;
+    uint32_t* syn_sg;        // strength signs: bit i of the layer = (strength i < 0)
+    uint64_t n_sg;           // bytes of syn_sg (whole 32-bit words)
+    // virtual states (see VirtTuft): per tuft the two snapshots, the parity
+    // bits (one word per tuft) and the device descriptors
+    uint8_t* snap[BSM_MAX_TUFTS];
+    uint32_t snap_cap[BSM_MAX_TUFTS];  // bytes per parity
+    uint32_t* vbits;
+    VirtTuft* vt_d;
     uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;
     uint8_t* tft[8];
     uint8_t *soma, *cel_flags, *cel_energies, *cel_act;
@@ -337,6 +399,20 @@ int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len)
 // layer after any write to their offsets or slice ids: the packed offsets, and
 // the staged-slice codes of every four-synapse group the range touches.
 int launch_pack_vu(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len);
+// Refreshes the strength-sign bits of synapses [offset, offset + len) of a
+// layer (whole 32-bit words) after any write to their strengths.
+int launch_pack_sg(hipStream_t s, bsm_area* a, uint32_t layer, size_t offset, size_t len);
+// Writes parity q's states of the layer's synapses [offset, offset + len) into
+// st[q] wherever they are virtual (both parities).  clear: the whole layer,
+// and its parity bits are cleared afterwards, so every state lives in the
+// state buffers again -- required before states, flags or synapse sources
+// change by any route other than the cycle.
+int settle_states(bsm_area* a, uint32_t layer, hipStream_t s, size_t offset, size_t len,
+                  bool clear);
+// settle_states(clear) of every layer of the area.
+int settle_area(bsm_area* a, hipStream_t s);
+// The device descriptors of the area's virtual-state tufts (at creation).
+int upload_virt(bsm_area* a, hipStream_t s);
 // The source slices the fused gather stages for a tuft (at most 4, sorted; with
 // the scalar quirk only slices without mids).  Depends on the area's scheme only.
 uint32_t stage_slices(const bsm_area* a, const TuftRt& T, uint32_t* staged);

@@ -229,6 +229,11 @@ using namespace bsm;
     LayerRt& L = (a)->L[(l)];                              \
     BSM_HIP(hipSetDevice((a)->ctx->device))
 
+// Unit kernels read and write the state buffers directly: virtual states of
+// the layer are settled into them first.
+#define SETTLE(a, l) \
+    if (settle_states((a), (l), (a)->stream, 0, 0, true)) return 1
+
 #define CHECK_TUFT(t) \
     if ((t) >= L.d.n_tufts) return fail("tuft out of range")
 
@@ -238,6 +243,7 @@ extern "C" {
 
 int bsm_k_syn_flags(bsm_area* a, uint32_t layer) {
     GET_LAYER(a, layer);
+    SETTLE(a, layer);
     BSM_HIP(hipMemcpyAsync(L.syn_state[L.cur ^ 1], L.syn_state[L.cur], L.n_syn,
                            hipMemcpyDeviceToDevice, a->stream));
     return 0;
@@ -246,6 +252,7 @@ int bsm_k_syn_flags(bsm_area* a, uint32_t layer) {
 int bsm_k_syn_cycle(bsm_area* a, uint32_t layer, uint32_t tuft) {
     GET_LAYER(a, layer);
     CHECK_TUFT(tuft);
+    SETTLE(a, layer);
     TuftRt& T = L.t[tuft];
     uint64_t n = (uint64_t)L.cells * T.spt;
     if (!n) return 0;
@@ -260,6 +267,7 @@ int bsm_k_syn_cycle(bsm_area* a, uint32_t layer, uint32_t tuft) {
 int bsm_k_den_cycle(bsm_area* a, uint32_t layer, uint32_t tuft, int32_t rnd) {
     GET_LAYER(a, layer);
     CHECK_TUFT(tuft);
+    SETTLE(a, layer);
     TuftRt& T = L.t[tuft];
     uint32_t nd = L.cells * T.d.dens_per_tft;
     kr_den_cycle<<<nblk(nd, 256), 256, 0, a->stream>>>(
@@ -313,6 +321,7 @@ int bsm_k_inhib_passthrough(bsm_area* a, uint32_t layer, int32_t rnd) {
 int bsm_k_ssc_mtp(bsm_area* a, uint32_t layer, uint32_t rnd) {
     GET_LAYER(a, layer);
     if (L.d.kind != BSM_LAYER_SSC) return fail("ssc_mtp needs a spiny stellate layer");
+    SETTLE(a, layer);
     TuftRt& T = L.t[0];
     uint64_t n = (uint64_t)L.cells * T.spt;
     if (!n) return 0;
@@ -320,7 +329,7 @@ int bsm_k_ssc_mtp(bsm_area* a, uint32_t layer, uint32_t rnd) {
                                                      L.syn_state[L.cur], L.syn_str, n, T.spt,
                                                      (int32_t)rnd);
     LAUNCH_CHECK();
-    return 0;
+    return launch_pack_sg(a->stream, a, layer, 0, L.n_syn);
 }
 
 static void pyr_tuft_ids(const LayerRt& L, uint32_t* en, uint32_t* prx, uint32_t* bdst,
@@ -375,6 +384,7 @@ int bsm_k_tft_dst_mtp(bsm_area* a, uint32_t layer, uint32_t tuft, int32_t rnd) {
     GET_LAYER(a, layer);
     CHECK_TUFT(tuft);
     if (L.d.kind != BSM_LAYER_PYR) return fail("tft_dst_mtp needs a pyramidal layer");
+    SETTLE(a, layer);
     TuftRt& T = L.t[tuft];
     uint32_t o = tuft * L.cells;
     kr_tft_dst_mtp<<<nblk(L.cells, 256), 256, 0, a->stream>>>(
@@ -382,7 +392,7 @@ int bsm_k_tft_dst_mtp(bsm_area* a, uint32_t layer, uint32_t tuft, int32_t rnd) {
         L.syn_state[L.cur ^ 1], L.syn_str, L.cel_flags, L.cells, (uint32_t)T.syn_idz, T.spt,
         T.d.syns_per_den, rnd);
     LAUNCH_CHECK();
-    return 0;
+    return launch_pack_sg(a->stream, a, layer, T.syn_idz, (size_t)L.cells * T.spt);
 }
 
 }  // extern "C"

@@ -282,6 +282,16 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
             sp->cv.notify_all();
         }
     };
+    // virtual synapse states of sampled ranges into the state buffers (both
+    // parities: states read the current one, flags the previous one)
+    for (uint32_t k : inc) {
+        const bsm_sampler* sp = P->smp[k];
+        if ((sp->buf == BSM_SYN_STATES || sp->buf == BSM_SYN_FLAG_SETS) &&
+            settle_states(areas[P->area_of[k]], sp->layer, s, sp->offset, sp->len, false)) {
+            reset();
+            return 1;
+        }
+    }
     for (size_t g0 = 0; g0 < inc.size(); g0 += SAMPLE_MAX) {
         SampleArgs args{};
         args.n = (uint32_t)std::min<size_t>(SAMPLE_MAX, inc.size() - g0);

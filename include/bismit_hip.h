@@ -372,7 +372,8 @@ int bsm_calib_copy(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, float
 /* The fused synapse stage's HBM pattern without its gather: src holds n_reads
  * (1-4) streams of `bytes` (multiple of 16) back to back, read with
  * non-temporal 16-byte loads; dst receives one stream of `bytes`.  The practical
- * bandwidth ceiling for that stage's read/write mix. */
+ * bandwidth ceiling for that stage's read/write mix.  dst null: the read
+ * streams alone, nothing written (the check-free gather's pattern). */
 int bsm_calib_stream(bsm_ctx* ctx, void* dst, const void* src, size_t bytes, uint32_t n_reads,
                      float* ms);
 

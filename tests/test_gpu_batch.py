@@ -162,12 +162,18 @@ def test_config3_plan_uses_fast_wide_variant(ctx):
         assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
 
     def launch_bytes(p, pick):
-        # per synapse: a 2-bit staged-slice code unless the tuft stages one slice,
-        # packed offsets, strength, state
-        def syn_b(t):
-            return (0 if len(t.src_slcs) <= 1 else 0.25) + 1 + 2
-        return sum(s.cells(li) * (t.syns_per_tft * syn_b(t) + t.dens_per_tft * 4 +
-                                  (12 if lay.kind == 2 else 0)) + (s.cells(li) if lay.kind == 1 else 0)
+        # per synapse, in eighth bytes: a 2-bit staged-slice code unless the tuft
+        # stages one slice, packed offsets, the strength's sign bit (no state
+        # written); the snapshot of the staged slices read and written
+        def syn_e(t):
+            return (0 if len(set(t.src_slcs)) <= 1 else 2) + 8 + 1
+
+        def snap(s, t):
+            sl = s.slices[min(t.src_slcs)]
+            return len(set(t.src_slcs)) * sl.v_size * sl.u_size
+        return sum(s.cells(li) * t.syns_per_tft * syn_e(t) // 8 +
+                   s.cells(li) * (t.dens_per_tft * 4 + (12 if lay.kind == 2 else 0)) +
+                   (s.cells(li) if lay.kind == 1 else 0) + 2 * snap(s, t)
                    for s in schemes for li, lay in enumerate(s.layers)
                    for ti, t in enumerate(lay.tufts) if pick(lay, ti))
 

@@ -240,11 +240,13 @@ def test_cycle_config2_full_size(ctx):
         if p["kind"] == 1:
             assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
     # bytes of the pyramidal tuft launches, one tuft each: per synapse the packed
-    # offsets, strength and state, plus a 2-bit staged-slice code for the distal
-    # tuft (two staged slices); 4 B/dendrite, 12 B/cell-tuft
+    # offsets (1 B) and the strength's sign bit, plus a 2-bit staged-slice code
+    # for the distal tuft (two staged slices), no state written; 4 B/dendrite,
+    # 12 B/cell-tuft; the snapshot of the staged slices read and written
     cells = 2 * 256 * 256
     assert [p["alg_bytes"] for p in plan if p["kind"] == 1][1:] == \
-        [cells * 512 * b + cells * 8 * 4 + cells * 12 for b in (3.25, 3)]
+        [cells * 512 * e // 8 + cells * 8 * 4 + cells * 12 + 2 * n * 256 * 256
+         for e, n in ((11, 2), (9, 1))]
     g.close()
     o.close()
 
@@ -316,6 +318,46 @@ def test_slice_codes_follow_writes(ctx):
     g.fill("SYN_SRC_SLC_IDS", 1, val)
     o.buf("SYN_SRC_SLC_IDS", 1)[:] = val
     run_cycles(g, o, s, frames[16:24], 8, tag="fill")
+    g.close()
+    o.close()
+
+
+def test_virtual_states_across_writes(ctx):
+    """Check-free tufts write no states (they are recomputed from a snapshot of
+    the staged slices) and read strength-sign bits instead of strengths.  Both
+    follow every route that changes them outside the cycle: partial writes of
+    strengths (odd offset and length), of states and of flags (the previous
+    state), a strength fill, and cycles run with no buffer read in between (so
+    that the learning stages take previous states from the snapshots).  Every
+    buffer is compared with the oracle."""
+    s = recipes.cortical_area("vs", 32, pyr_depth=2, ssc_dens=4, ssc_spd=16, pyr_dens=4,
+                              pyr_spd=16)
+    g, o = make_pair(ctx, s)
+    rng = np.random.default_rng(33)
+    frames = inputs.SdrStream(32 * 32, 0.08, 0.5, seed=9).frames(64)
+    k = [0]
+
+    def cycles(tag, n=8, every=1):
+        run_cycles(g, o, s, frames[k[0]:k[0] + n], n, every=every, tag=tag)
+        k[0] += n
+
+    def put(name, li, arr, offset=0):
+        g.write(name, li, arr, offset)
+        o.write(name, li, arr, offset)
+
+    assert all(p["fast"] == 1 for p in g.stage_plan() if p["kind"] == 1)
+    cycles("initial", 10, every=5)
+    for li in range(2):
+        n = g.buf_len("SYN_STRENGTHS", li)
+        put("SYN_STRENGTHS", li, rng.integers(-60, 60, 999).astype(np.int8), n // 3 + 5)
+    cycles("strength write", 8, every=4)
+    n = g.buf_len("SYN_STATES", 1)
+    put("SYN_STATES", 1, rng.choice(np.array([0, 128, 200], dtype=np.uint8), 777), n // 2 + 3)
+    put("SYN_FLAG_SETS", 1, rng.integers(0, 256, 555).astype(np.uint8), n // 4 + 1)
+    cycles("state and flag writes", 8)
+    g.fill("SYN_STRENGTHS", 1, 0xF0)
+    o.buf("SYN_STRENGTHS", 1)[:] = -16
+    cycles("strength fill", 12, every=6)
     g.close()
     o.close()
 
