@@ -62,20 +62,21 @@ def baseline_threads():
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
-def syn_bytes(stage, tuft):
-    """Per-synapse HBM bytes of one tuft in an F1 launch.  Check-free gather
-    (virtual states): a 2-bit staged-slice code when it stages several slices
-    and nothing for one, the offsets (1 B packed, else 2 B) and one strength-sign
-    bit; no state is written.  Checked gather: the 1 B slice id, 2 B offsets,
+def syn_bytes(stage, tuft, kind=2):
+    """Per-synapse HBM bytes of one tuft in an F1 launch.  Check-free gather: a
+    2-bit staged-slice code when it stages several slices and nothing for one,
+    the offsets (1 B packed, else 2 B) and one strength-sign bit; pyramidal
+    tufts (kind 2) keep their states virtual (none written), spiny stellate
+    ones write them (1 B).  Checked gather: the 1 B slice id, 2 B offsets,
     strength in and state out."""
     if stage["fast"]:
         slc = 0 if len(set(tuft.src_slcs)) <= 1 else 0.25
-        return slc + (1 if stage["packed"] else 2) + 0.125
+        return slc + (1 if stage["packed"] else 2) + 0.125 + (0 if kind == 2 else 1)
     return 1 + 2 + 2
 
 
 def snap_bytes(scheme, tuft):
-    """Bytes of a check-free tuft's snapshot: its staged source slices."""
+    """Bytes of a check-free pyramidal tuft's snapshot: its staged source slices."""
     sl = scheme.slices[min(tuft.src_slcs)]
     return len(set(tuft.src_slcs)) * sl.v_size * sl.u_size
 
@@ -90,10 +91,10 @@ def stage_bytes(scheme, li, kind, stage):
     if kind == 1:
         b = 0
         for t in lay.tufts:
-            b += int(C * t.syns_per_tft * syn_bytes(stage, t)) + C * t.dens_per_tft * 4
+            b += int(C * t.syns_per_tft * syn_bytes(stage, t, lay.kind)) + C * t.dens_per_tft * 4
             if lay.kind == 2:
                 b += C * 12
-            if stage["fast"]:
+            if stage["fast"] and lay.kind == 2:
                 b += 2 * snap_bytes(scheme, t)
         if lay.kind == 1:
             b += C
@@ -448,7 +449,7 @@ def main():
             "how": "host frame per cycle through bsm_area_intake (synchronous copy), then "
                    "bsm_area_cycle with host-drawn rnd; Python host loop"}
     (dk, dl), (dt, alg_tot, n_launch, dstage) = max(groups.items(), key=lambda kv: kv[1][0])
-    syn_bs = [syn_bytes(dstage, t) for t in scheme.layers[dl].tufts]
+    syn_bs = [syn_bytes(dstage, t, scheme.layers[dl].kind) for t in scheme.layers[dl].tufts]
     # the engine's count equals the recipe's
     assert alg_tot == stage_bytes(scheme, dl, dk, dstage), (alg_tot, syn_bs)
     avg_s = dt / n_prof / n_launch / 1000.0

@@ -157,6 +157,8 @@ static void plan_area(bsm_area* a, ArenaPlan& ap) {
         ap.add(&L.syn_str, ns);
         ap.add(&L.syn_sg, L.n_sg);
         for (uint32_t t = 0; t < L.d.n_tufts; t++) ap.add(&L.snap[t], 2 * (size_t)L.snap_cap[t]);
+        for (uint32_t t = 0; t < L.d.n_tufts; t++)
+            ap.add(&L.bmask[t], 2 * sizeof(uint32_t) * (size_t)L.cells * L.bm_wpd[t]);
         ap.add(&L.vbits, sizeof(uint32_t) * BSM_MAX_TUFTS);
         ap.add(&L.vt_d, sizeof(VirtTuft) * BSM_MAX_TUFTS);
         ap.add(&L.den_states, nd);
@@ -203,6 +205,8 @@ static void fill_layers(bsm_area* a, const bsm_area_desc* d) {
                 if (!dup) cap += (uint64_t)sd.v_size * sd.u_size;
             }
             L.snap_cap[t] = (uint32_t)cap;
+            L.bm_wpd[t] = (L.d.kind == BSM_LAYER_PYR && T.d.den_kind == BSM_DEN_DISTAL)
+                              ? (T.d.syns_per_den + 31) / 32 : 0u;
         }
         L.n_syn = ns;
         L.n_sg = (ns + 31) / 32 * 4;
@@ -384,6 +388,9 @@ int upload_virt(bsm_area* a, hipStream_t s) {
             v.vofs = L.syn_v + T.syn_idz;
             v.uofs = L.syn_u + T.syn_idz;
             v.vbits = L.vbits + t;
+            v.bmask = L.bm_wpd[t] ? L.bmask[t] : nullptr;
+            v.wpd = L.bm_wpd[t];
+            v.bm_len = L.cells * L.bm_wpd[t];
             v.n = (uint64_t)L.cells * T.spt;
             v.spt = T.spt;
             v.VU = a->V * a->U;
@@ -676,10 +683,13 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
 
 // Buffers whose writes change synapse states as the cycle left them: the
 // states themselves, the flags (their previous-active bit is the previous
-// state) and the sources (a virtual state follows its synapse's source).
+// state) and the sources (a virtual state follows its synapse's source); and
+// the best dendrite ids, which the distal tufts' kept fired bits follow.
 static bool changes_states(int buf) {
     return buf == BSM_SYN_STATES || buf == BSM_SYN_FLAG_SETS || buf == BSM_SYN_SRC_SLC_IDS ||
-           buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS;
+           buf == BSM_SYN_SRC_COL_V_OFFS || buf == BSM_SYN_SRC_COL_U_OFFS ||
+           // the best-dendrite fired bits follow the best dendrite ids the cycle set
+           buf == BSM_TFT_BEST_DEN_IDS;
 }
 
 // Buffer I/O on an area inside a batch first waits for the batch's enqueued

@@ -229,13 +229,25 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
     }
 }
 
+// The fired bits of each tile chunk of a distal tuft (LDS, W bits per chunk in
+// a 16-bit slot), copied for each cell's best dendrite to the tuft's kept bits
+// of this parity (dst; cells' words of wpd, bit i = synapse i of the dendrite).
+// With 16-synapse chunks the slots form one bit stream (a cell's synapses are
+// contiguous), so a dendrite of any length is a bit range of it; narrower
+// chunks are copied slot by slot, whole chunks only.
+struct MaskOut {
+    const uint16_t* lm;   // null: none kept (4-byte aligned, one spare word)
+    uint32_t* dst;
+    uint32_t cpt, spd, wpd, w;
+};
+
 // Dendrites (den_cycle_tft, ssc_cycle on the SSC soma) and tufts (tft_cycle) of
 // one tile from the per-chunk (sum, raw) partials in LDS; ends with a barrier.
 template <class Map>
 __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t* acc,
                                  uint32_t stage_cap, uint32_t nd_t, uint32_t cells_t,
                                  const Map& map, int32_t rnd, uint32_t tid, uint32_t nth,
-                                 const TailPre& pre) {
+                                 const TailPre& pre, const MaskOut& mo) {
     // ---- dendrites (den_cycle_tft), optional ssc_cycle on the SSC soma
     const uint32_t dens = J.dens;
     const bool do_tft = J.tft[0] != nullptr;
@@ -316,6 +328,34 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             t1[cel] = (uint8_t)best_raw;
             t2[cel] = (uint8_t)best;
             t3[cel] = (uint8_t)tstate;
+            if (mo.lm) {
+                auto dst = BSM_G(uint32_t, mo.dst + (size_t)cel * mo.wpd);
+                if (mo.w == 16) {
+                    const uint32_t* l32 = (const uint32_t*)mo.lm;
+                    const uint32_t start = ct * mo.cpt * 16 + best_id * mo.spd;
+                    for (uint32_t i = 0; i < mo.wpd; i++) {
+                        const uint32_t bp = start + 32 * i, k = bp >> 5;
+                        uint32_t v = __builtin_amdgcn_alignbit(l32[k + 1], l32[k], bp & 31u);
+                        const uint32_t rem = mo.spd - 32 * i;
+                        if (rem < 32) v &= (1u << rem) - 1u;
+                        dst[i] = v;
+                    }
+                } else {
+                    const uint32_t cpd = mo.spd / mo.w;
+                    const uint16_t* src = mo.lm + ct * mo.cpt + best_id * cpd;
+                    uint32_t word = 0, bit = 0, wi = 0;
+                    for (uint32_t c = 0; c < cpd; c++) {
+                        word |= (uint32_t)src[c] << bit;
+                        bit += mo.w;
+                        if (bit == 32) {
+                            dst[wi++] = word;
+                            word = 0;
+                            bit = 0;
+                        }
+                    }
+                    if (bit) dst[wi] = word;
+                }
+            }
         }
     }
     __syncthreads();
@@ -342,6 +382,9 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
     }
 }
 
+// A byte of LDS at byte address a.
+#define LDS8(a) ((const __attribute__((address_space(3))) uint8_t*)(uintptr_t)(a))
+
 // The check-free gather's loads of one chunk, kept as loaded: every value
 // derived from them is computed in work(), so that a chunk's loads stay in
 // flight while the previous chunk is gathered.
@@ -350,7 +393,7 @@ struct ChunkMeta {
     uint32_t v[NW], u[NW];   // packed offsets (or v / u offsets)
     uint32_t c, csh;         // staged-slice code word of the chunk's group, its shift
     uint32_t s, ssh;         // strength-sign word, its shift
-    uint32_t so, den0, off0;
+    uint32_t so, den0, off0, q;
     int32_t lb;
 };
 
@@ -373,6 +416,8 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
                                                     uint32_t rnd_ofs, uint32_t stage_cap) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr uint32_t NTH = 256;   // launched with 256 threads
+    // the LDS byte address of the dynamic LDS array
+    const uint32_t lds0 = (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint8_t*)lds;
     constexpr int NW = Chunk<W>::NW;
     const uint32_t tid = threadIdx.x;
     uint32_t* acc = (uint32_t*)(lds + stage_cap);
@@ -395,16 +440,20 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
         for (uint32_t x = tid; x < nd_t; x += NTH) acc[x] = 0u;
         const SliceRt s0 = J.slices[J.staged[0]];
         // virtual states: this item's share of the snapshot of the staged slices,
-        // and the parity's bit
+        // and the parity's bit (set; cleared when this job writes its states)
         {
             const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;
-            const uint32_t lo = idx * J.snap_per, hi = min(lo + J.snap_per, J.snap_len);
-            auto src = BSM_G(const uint8_t, J.axn + s0.idz);
-            auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
-            for (uint32_t x = lo + tid; x < hi; x += NTH) dst[x] = src[x];
+            if (J.virt) {
+                const uint32_t lo = idx * J.snap_per, hi = min(lo + J.snap_per, J.snap_len);
+                auto src = BSM_G(const uint8_t, J.axn + s0.idz);
+                auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
+                for (uint32_t x = lo + tid; x < hi; x += NTH) dst[x] = src[x];
+            }
             if (idx == 0 && tid == 0) {
+                // bit pu: states virtual; bit 2 + pu: best-dendrite fired bits kept
+                const uint32_t keep = J.bmask && (W == 16 || (J.spd % W) == 0);
                 auto vb = BSM_G(uint32_t, J.vbits);
-                *vb = *vb | (1u << pu);
+                *vb = (*vb & ~(5u << pu)) | (J.virt ? (1u << pu) : 0u) | (keep ? (4u << pu) : 0u);
             }
         }
         // uniform per item: the geometry in scalar registers
@@ -469,6 +518,9 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
         const Rsrc r_u = make_rsrc(J.uofs, nrec);
         const Rsrc r_sg = make_rsrc(J.sg, J.sg_len);
         const uint32_t sgb0 = J.sg_bit0;
+        // a job whose states are not virtual writes them (its fired window bytes)
+        const Rsrc r_out = make_rsrc(J.st[pu], J.virt ? 0u : nrec);
+        const bool wr = !J.virt;
         // packed offsets carry v + 8 and u + 8: their bias folds into the base
         const int32_t pbias = PACKED ? 8 * (int32_t)nc + 8 : 0;
         const uint32_t coefA = (nc & 0xFFu) | (1u << 8);
@@ -478,8 +530,12 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
         const bool whole = (J.spd % W) == 0;
         const uint32_t m_spd = 0xFFFFFFFFu / J.spd + 1u;
         const uint32_t last = n_chunks - 1;
+        // distal tufts: each chunk's fired bits, for the best dendrites' copy
+        uint16_t* lmask = (J.bmask && (W == 16 || whole))
+                              ? (uint16_t*)(lds + ((stage_cap + nd_t * 6 + 3) & ~3u)) : nullptr;
 
         auto fetch = [&](uint32_t q, ChunkMeta<NW>& m) {
+            m.q = q;
             const uint32_t rowq = q_rc(q);
             const uint32_t rem = q - __umul24(rowq, row_chunks);
             const uint32_t cc = q_cpt(rem);
@@ -499,7 +555,8 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
             if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, m.u);
         };
         auto work = [&](const ChunkMeta<NW>& m, bool valid) {
-            const int32_t lbp = m.lb - pbias;
+            // window addresses as LDS byte addresses (the dynamic LDS base folded in)
+            const int32_t lbp = m.lb - pbias + (int32_t)lds0;
             uint32_t cw = 0;
             if constexpr (NST > 1) cw = m.c >> m.csh;    // word w of the chunk: bits 8k + 2w
             const uint32_t nsg = ~(m.s >> m.ssh);        // word w: bit 8k + w set if strength >= 0
@@ -536,8 +593,21 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
                     a[2] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefA, l[2], false);
                     a[3] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefB, l[3], false);
                 }
-                x[w] = (uint32_t)lds[a[0]] | ((uint32_t)lds[a[1]] << 8) |
-                       ((uint32_t)lds[a[2]] << 16) | ((uint32_t)lds[a[3]] << 24);
+                x[w] = (uint32_t)*LDS8(a[0]) | ((uint32_t)*LDS8(a[1]) << 8) |
+                       ((uint32_t)*LDS8(a[2]) << 16) | ((uint32_t)*LDS8(a[3]) << 24);
+            }
+            if (wr && valid) Chunk<W>::store_b(r_out, m.so, x);
+            if (lmask && valid) {
+                // fired = bit 7 of each state byte; bit 4w + k for byte k of word w
+                uint32_t lo = 0, hi = 0;
+                lo = __builtin_amdgcn_udot4((x[0] >> 7) & 0x01010101u, 0x08040201u, lo, false);
+                if constexpr (NW > 1)
+                    lo = __builtin_amdgcn_udot4((x[1] >> 7) & 0x01010101u, 0x80402010u, lo, false);
+                if constexpr (NW > 2) {
+                    hi = __builtin_amdgcn_udot4((x[2] >> 7) & 0x01010101u, 0x08040201u, hi, false);
+                    hi = __builtin_amdgcn_udot4((x[3] >> 7) & 0x01010101u, 0x80402010u, hi, false);
+                }
+                lmask[m.q] = (uint16_t)(lo | (hi << 8));
             }
             if (whole) {
                 uint32_t ps = 0, rs = 0;
@@ -573,7 +643,8 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
             work(B, qb < n_chunks);   // past the end: B re-gathers the last chunk, unused
         }
         __syncthreads();
-        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, NTH, pre);
+        const MaskOut mo{lmask, J.bmask + (size_t)pu * J.bm_len, CPT, J.spd, J.wpd, (uint32_t)W};
+        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, NTH, pre, mo);
     }
 }
 
@@ -653,7 +724,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         uint8_t* __restrict__ st_out = J.st[p & 1];
         if (wi.tv == 0 && wi.tu == 0 && tid == 0) {  // parity p's states are written
             auto vb = BSM_G(uint32_t, J.vbits);
-            *vb = *vb & ~(1u << (p & 1));
+            *vb = *vb & ~(5u << (p & 1));
         }
         for (uint32_t q = tid; q < n_chunks; q += nth) {
             const uint32_t rowq = q / row_chunks, rem = q - rowq * row_chunks;
@@ -708,7 +779,7 @@ __global__ __launch_bounds__(256) void kf_tuft(const TuftJob* __restrict__ jobs,
         }
         __syncthreads();
 
-        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, nth, pre);
+        tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, nth, pre, MaskOut{});
     }
 }
 
@@ -748,6 +819,20 @@ struct LtpChunk {
                 o |= virt_axon(T, snap, cel, byte_of(sl[w], b), sbyte_of(vv[w], b),
                                sbyte_of(uu[w], b)) << (8 * b);
             ms[w] = o;
+        }
+    }
+    // Previous states from a distal tuft's kept best-dendrite fired bits (the
+    // dendrite's words at bits; bytes 0 or 1): the chunk starts at synapse k0 of
+    // the dendrite.
+    __device__ inline void load_bits(const uint32_t* bits_d, uint32_t k0,
+                                     const int8_t* __restrict__ str, uint32_t i0) {
+        const uint32_t word = BSM_G(const uint32_t, bits_d)[k0 >> 5];
+        const uint32_t bits = word >> (k0 & 31u);
+        Chunk<W>::load(str + i0, mt);
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            if constexpr (W == 1) ms[w] = bits & 1u;
+            else ms[w] = spread4((bits >> (4 * w)) & 0xFu);
         }
     }
     // New strengths of the chunk; the sign bits of those whose sign changed are
@@ -913,10 +998,8 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             // else decrement (0x1FF); ltp_u chunks per thread in flight at once (a
             // chunk past the list loads chunk 0 again and is not applied)
             const uint32_t na = n_act, spt = J.spt, CPC = spt / W, total = na * CPC;
+            // this cycle's states (spiny stellate tufts always write theirs)
             const uint8_t* __restrict__ st = J.st[p & 1];
-            // this cycle's states: virtual when the gather kept a snapshot
-            const bool virt = (*BSM_G(const uint32_t, J.vt->vbits) >> (p & 1)) & 1u;
-            const uint8_t* snap = J.vt->snap + (size_t)(p & 1) * J.vt->snap_len;
             constexpr int LU = ltp_u<W>();
             for (uint32_t y0 = tid; y0 < total; y0 += LU * nth) {
                 LtpChunk<W> ch[LU];
@@ -927,8 +1010,7 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     const uint32_t ci = y / CPC, k = y - ci * CPC;
                     zz[j] = list[ci] * spt;
                     ii[j] = zz[j] + k * W;
-                    if (virt) ch[j].load_virt(*J.vt, snap, J.str, ii[j], 0u, list[ci]);
-                    else ch[j].load(st, J.str, ii[j]);
+                    ch[j].load(st, J.str, ii[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < LU; j++)
@@ -952,7 +1034,7 @@ template <int W>
 __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
                                                    uint32_t n_items, uint32_t par, uint32_t rnd_ofs) {
-    __shared__ uint32_t lz[F3_COLS * F3_G * BSM_MAX_TUFTS];  // queued dendrite: first synapse
+    __shared__ uint32_t lz[F3_COLS * F3_G * BSM_MAX_TUFTS];  // queued dendrite: cell | best << 24
     __shared__ uint8_t lt[F3_COLS * F3_G * BSM_MAX_TUFTS];   // queued dendrite: its tuft
     __shared__ uint32_t n_l;
     const uint32_t tid = threadIdx.x;
@@ -1055,7 +1137,7 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                             const uint32_t spd = J.t_spd[t];
                             if (!prev && act && pbr[g][t] != 0 && pbi[g][t] * spd < J.t_spt[t]) {
                                 const uint32_t q = atomicAdd(&n_l, 1u);
-                                lz[q] = c * J.t_spt[t] + J.t_syn_idz[t] + pbi[g][t] * spd;
+                                lz[q] = c | (pbi[g][t] << 24);   // cell < 2^24 (planner)
                                 lt[q] = (uint8_t)t;
                             }
                             f = (f & ~(uint32_t)CEL_PREV_ACTIVE) | (act ? CEL_PREV_ACTIVE : 0u);
@@ -1078,6 +1160,15 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                 // at once (a chunk past a dendrite or the queue loads the queue's
                 // first chunk and is not applied)
                 const uint32_t total = nl * maxch;
+                // where each distal tuft's previous states are, uniform per item:
+                // bit 0 virtual (else the state buffer), bit 2 its best dendrites'
+                // fired bits kept (else the snapshot)
+                uint32_t pv[BSM_MAX_TUFTS];
+#pragma unroll
+                for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++)
+                    pv[t] = (t < n_tufts && ((dmask >> t) & 1u))
+                                ? *BSM_G(const uint32_t, J.t_vbits[t]) >> pq : 0u;
+                const Quot q_mc = Quot::of(maxch);
                 constexpr int LU = ltp_u<W>();
                 for (uint32_t y0 = tid; y0 < total; y0 += LU * blockDim.x) {
                     LtpChunk<W> ch[LU];
@@ -1086,17 +1177,25 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
 #pragma unroll
                     for (int j = 0; j < LU; j++) {
                         const uint32_t y = y0 + j * blockDim.x;
-                        const uint32_t li = y < total ? y / maxch : 0u;
+                        const uint32_t li = y < total ? q_mc(y) : 0u;
                         const uint32_t k = y < total ? y - li * maxch : 0u;
-                        tt[j] = lt[li];
-                        ok[j] = y < total && k * W < J.t_spd[tt[j]];
-                        zz[j] = lz[li];
+                        const uint32_t t = lt[li], e = lz[li];
+                        const uint32_t cel = e & 0xFFFFFFu, best = e >> 24;
+                        tt[j] = t;
+                        ok[j] = y < total && k * W < J.t_spd[t];
+                        zz[j] = cel * J.t_spt[t] + J.t_syn_idz[t] + best * J.t_spd[t];
                         ii[j] = zz[j] + (ok[j] ? k * W : 0u);
-                        const VirtTuft& T = J.vt[tt[j]];
-                        if ((*BSM_G(const uint32_t, T.vbits) >> pq) & 1u) {
-                            const uint32_t t0 = J.t_syn_idz[tt[j]];
-                            ch[j].load_virt(T, T.snap + (size_t)pq * T.snap_len, J.str, ii[j], t0,
-                                            (zz[j] - t0) / J.t_spt[tt[j]]);
+                        const uint32_t vb = t == 0 ? pv[0] : (t == 1 ? pv[1] : pv[2]);
+                        if (vb & 1u) {
+                            // virtual previous states: the kept fired bits of the
+                            // best dendrite, else the snapshot
+                            if (vb & 4u)
+                                ch[j].load_bits(J.t_bmask[t] + (size_t)pq * J.t_bmlen[t] +
+                                                    (size_t)cel * J.t_wpd[t],
+                                                ii[j] - zz[j], J.str, ii[j]);
+                            else
+                                ch[j].load_virt(J.vt[t], J.vt[t].snap + (size_t)pq * J.vt[t].snap_len,
+                                                J.str, ii[j], J.t_syn_idz[t], cel);
                         } else {
                             ch[j].load(prev_st, J.str, ii[j]);
                         }
@@ -1235,6 +1334,13 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     // geometry, so contiguous in the axon space) are its snapshot
     j.snap = L.snap[t];
     j.snap_len = fast ? n * (uint32_t)j.st_vsize[0] * j.st_usize[0] : 0u;
+    // Pyramidal tufts' states are virtual.  A spiny stellate tuft writes its
+    // states: its launch is bound by latency, not bytes, and its layer's
+    // learning reads the states of every active cell's synapses.
+    j.virt = L.d.kind == BSM_LAYER_PYR ? 1u : 0u;
+    j.bmask = L.bm_wpd[t] ? L.bmask[t] : nullptr;
+    j.wpd = L.bm_wpd[t];
+    j.bm_len = L.cells * L.bm_wpd[t];
     if (j.snap_len > L.snap_cap[t]) {
         j.fast = 0;
         j.snap_len = 0;
@@ -1268,7 +1374,6 @@ SscJob make_ssc_job(bsm_area* a, uint32_t l, const int32_t* rnd) {
                                                    : 2u;
     j.learn = !(a->settings & BSM_SET_DISABLE_LEARNING);
     j.par_xor = 0;
-    j.vt = L.vt_d;
     j.sg = L.syn_sg;
     return j;
 }
@@ -1318,6 +1423,10 @@ PyrJob make_pyr_job(bsm_area* a, uint32_t l, const int32_t* rnd, uint32_t* w3) {
         j.t_spd[t] = d.syns_per_den;
         j.t_spt[t] = T.spt;
         j.t_syn_idz[t] = (uint32_t)T.syn_idz;
+        j.t_vbits[t] = L.vbits + t;
+        j.t_bmask[t] = L.bmask[t];
+        j.t_bmlen[t] = L.cells * L.bm_wpd[t];
+        j.t_wpd[t] = L.bm_wpd[t];
     }
     uint32_t mc = 1;
     for (uint32_t t = 0; t < L.d.n_tufts; t++)
@@ -1360,6 +1469,12 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
         packed &= jobs[job_ids[k]].packed != 0;
     }
     fast &= W >= 4;
+    // a distal tuft's per-chunk fired bits (check-free path, whole chunks): 2 B
+    // per chunk of the tile
+    auto mask_bytes = [&](const TuftJob& j, uint32_t tiles) -> uint32_t {
+        return (fast && j.bmask && (W == 16 || j.spd % W == 0))
+                   ? 2 * j.depth * tiles * (j.spt / W) + 8 : 0u;   // + alignment, spare word
+    };
     packed &= fast;
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
     // several staged slices with packed offsets: window slots of 32 rows, so a
@@ -1387,7 +1502,7 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const TuftJob& j = jobs[id];
             cap = std::max(cap, stage_bytes(j, TV, TU, fast));
             accb = std::max(accb, 4 * j.depth * TV * TU * j.dens);
-            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * TV * TU * j.dens);
+            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * TV * TU * j.dens + mask_bytes(j, TV * TU));
         }
         cap = (cap + 15) & ~15u;
         lds = cap + accb + denc;
@@ -1415,7 +1530,7 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
     // tile: 15.2 us per launch; at 2x32: 12.6 us.)
     auto job_lds = [&](const TuftJob& j, uint32_t tv, uint32_t tu) {
         return ((stage_bytes(j, tv, tu, fast) + 15) & ~15u) + 4 * j.depth * tv * tu * j.dens +
-               (j.tft[0] ? 2 * j.depth * tv * tu * j.dens : 0u);
+               (j.tft[0] ? 2 * j.depth * tv * tu * j.dens + mask_bytes(j, tv * tu) : 0u);
     };
     if (fast) {
         for (uint32_t id : job_ids) {
@@ -1444,7 +1559,7 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const uint32_t tiles = j.tile_v * j.tile_u;
             cap = std::max(cap, stage_bytes(j, j.tile_v, j.tile_u, fast));
             accb = std::max(accb, 4 * j.depth * tiles * j.dens);
-            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * tiles * j.dens);
+            if (j.tft[0]) denc = std::max(denc, 2 * j.depth * tiles * j.dens + mask_bytes(j, tiles));
         }
         cap = (cap + 15) & ~15u;
         lds = cap + accb + denc;
@@ -1461,17 +1576,18 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
         // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
         // out (pyramidal); per SSC cell its energy; fast: the snapshot copy
         const TuftJob& j = jobs[id];
-        const uint32_t syn_e = fast ? (j.n_staged <= 1 ? 0u : 2u) + (packed ? 8u : 16u) + 1u
+        const uint32_t syn_e = fast ? (j.n_staged <= 1 ? 0u : 2u) + (packed ? 8u : 16u) + 1u +
+                                          (j.virt ? 0u : 8u)
                                     : 8u + 16u + 8u + 8u;
         st.alg_bytes += (uint64_t)j.C * j.spt * syn_e / 8 + (uint64_t)j.C * j.dens * 4 +
                         (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0) +
-                        (fast ? 2 * (uint64_t)j.snap_len : 0);
+                        (fast && j.virt ? 2 * (uint64_t)j.snap_len : 0);
     }
     for (uint32_t id : job_ids) {
         TuftJob& j = jobs[id];
         uint32_t ntv = (j.V + j.tile_v - 1) / j.tile_v, ntu = (j.U + j.tile_u - 1) / j.tile_u;
         if (ntv > 65535 || ntu > 65535 || id > 65535) return fail("too many tiles");
-        if (fast && !j.snap_len) return fail("check-free tuft without a snapshot");
+        if (fast && j.virt && !j.snap_len) return fail("check-free tuft without a snapshot");
         j.ntu = ntu;
         j.n_tiles = ntv * ntu;
         j.snap_per = (j.snap_len + j.n_tiles - 1) / j.n_tiles;
@@ -1540,6 +1656,7 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
     st.item0 = (uint32_t)items.size();
     for (uint32_t id : job_ids) {
         const PyrJob& j = jobs[id];
+        if (j.C >= (1u << 24)) return fail("pyramidal layer of 2^24 cells or more");
         uint32_t nb = (j.V * j.U + F3_COLS - 1) / F3_COLS;
         if (nb > 65535) return fail("too many column blocks");
         for (uint32_t b = 0; b < nb; b++) items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)b, 0});

@@ -46,6 +46,11 @@ struct VirtTuft {
     uint32_t spt, VU, U;      // synapses per cell; layer columns
     uint16_t vs, us;          // staged slices' geometry
     uint8_t vsc, usc, vmid, umid;
+    // distal tufts: per parity and cell, the fired bits of the cell's best
+    // dendrite as the gather found them (bit i = synapse i of the dendrite),
+    // valid for parity q while bit 2 + q of *vbits is set
+    const uint32_t* bmask;    // parity q at bmask + q * bm_len
+    uint32_t bm_len, wpd;     // words per parity, words per dendrite
 };
 
 // The source axon of one synapse of cell `cel` (layer index) in a snapshot:
@@ -97,7 +102,11 @@ struct TuftJob {
     uint32_t fast;         // every synapse sources a staged slice within `halo`
     uint32_t packed;       // fast, and every |offset| <= 7: the gather reads `vu`
     const uint32_t* dyn;   // dynamic batches: per (area, layer) word, bit0 parity, bit1 active
-    // virtual states (fast jobs): snapshot of the staged slices, parity bits
+    // virtual states (fast jobs with virt set): snapshot of the staged slices,
+    // parity bits; a fast job without virt writes its states
+    uint32_t virt;
+    uint32_t* bmask;       // distal tufts: best-dendrite fired bits (see VirtTuft), or null
+    uint32_t bm_len, wpd;
     uint8_t* snap;         // parity q at snap + q * snap_len
     uint32_t snap_len;
     uint32_t* vbits;
@@ -129,7 +138,6 @@ struct SscJob {
     uint32_t learn;
     uint32_t par_xor;
     const uint32_t* dyn;
-    const VirtTuft* vt;    // the layer's tuft (device)
     uint32_t* sg;          // the layer's strength signs (32 per word)
 };
 
@@ -162,6 +170,9 @@ struct PyrJob {
     uint32_t t_spd[BSM_MAX_TUFTS];
     uint32_t t_spt[BSM_MAX_TUFTS];
     uint32_t t_syn_idz[BSM_MAX_TUFTS];
+    const uint32_t* t_vbits[BSM_MAX_TUFTS];   // each tuft's parity bits (see VirtTuft)
+    const uint32_t* t_bmask[BSM_MAX_TUFTS];   // its kept best-dendrite fired bits
+    uint32_t t_bmlen[BSM_MAX_TUFTS], t_wpd[BSM_MAX_TUFTS];
 };
 
 struct WorkItem {
@@ -206,6 +217,8 @@ struct LayerRt {
     // bits (one word per tuft) and the device descriptors
     uint8_t* snap[BSM_MAX_TUFTS];
     uint32_t snap_cap[BSM_MAX_TUFTS];  // bytes per parity
+    uint32_t* bmask[BSM_MAX_TUFTS];    // distal pyramidal tufts: best-dendrite fired bits
+    uint32_t bm_wpd[BSM_MAX_TUFTS];    // words per dendrite (0: none kept)
     uint32_t* vbits;
     VirtTuft* vt_d;
     uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;

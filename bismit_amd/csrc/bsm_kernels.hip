@@ -282,6 +282,7 @@ int bsm_k_tft_cycle(bsm_area* a, uint32_t layer, uint32_t tuft) {
     GET_LAYER(a, layer);
     CHECK_TUFT(tuft);
     if (L.d.kind != BSM_LAYER_PYR) return fail("tft_cycle needs a pyramidal layer");
+    SETTLE(a, layer);   // it sets the best dendrite ids
     TuftRt& T = L.t[tuft];
     uint32_t o = tuft * L.cells;
     kr_tft_cycle<<<nblk(L.cells, 256), 256, 0, a->stream>>>(
