@@ -21,6 +21,7 @@
 // code (only when the tuft's gather stages several slices) and the strength
 // in, and the new state out: 3-3.25 B.
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 #include "bsm_internal.h"
@@ -133,8 +134,8 @@ __device__ __forceinline__ uint32_t sg_pos(uint32_t i) { return 8u * (i & 3u) + 
 // TAIL_PC cell-tufts of each thread (bytes packed four to a register).  Every
 // byte belongs to this tile alone (its dendrites, its cells), so nothing else
 // writes it meanwhile.
-constexpr uint32_t TAIL_PD = 8;
-constexpr uint32_t TAIL_PC = 2;
+constexpr uint32_t TAIL_PD = 4;
+constexpr uint32_t TAIL_PC = 1;
 struct TailPre {
     uint32_t act[TAIL_PD / 4];   // dendrite activity ratings
     uint32_t en[TAIL_PD / 4];    // SSC soma energies (dendrite index < cells)
@@ -194,9 +195,36 @@ struct RowMap {
 template <class Map>
 __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t nd_t,
                                      uint32_t cells_t, uint32_t tid, uint32_t nth, TailPre& pre) {
+    // every load first (indices clamped to valid bytes, results masked after),
+    // so that they share one round trip
     const uint32_t dens = J.dens;
     auto den_act = BSM_G(const uint8_t, J.den_act);
-    auto energy = BSM_G(const uint8_t, J.ssc_energy);
+    // no branch around a load: a layer without energies or tuft arrays reads
+    // its dendrite activities in their place (masked out below)
+    const bool has_en = J.ssc_energy != nullptr, has_tft = J.tft[0] != nullptr;
+    auto energy = BSM_G(const uint8_t, has_en ? (const uint8_t*)J.ssc_energy : J.den_act);
+    const uint32_t en_lim = has_en ? J.C : 1u;
+    uint32_t av[TAIL_PD], ev[TAIL_PD], ok[TAIL_PD];
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PD; k++) {
+        const uint32_t x = tid + k * nth;
+        uint32_t dj;
+        const uint32_t ct = map.den_cell(x < nd_t ? x : 0u, dj);
+        const uint32_t dl = map.cell(ct) * dens + dj;
+        const uint32_t dg = J.den_idz + dl;
+        av[k] = den_act[dl];
+        ev[k] = energy[dg < en_lim ? dg : 0u];
+        ok[k] = (uint32_t)(x < nd_t) | ((uint32_t)(has_en && dg < J.C) << 1);
+    }
+    uint32_t tv[TAIL_PC][4];
+#pragma unroll
+    for (uint32_t k = 0; k < TAIL_PC; k++) {
+        const uint32_t ct = tid + k * nth;
+        const uint32_t cel = map.cell(ct < cells_t ? ct : 0u);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++)
+            tv[k][b] = BSM_G(const uint8_t, has_tft ? (const uint8_t*)J.tft[b] : J.den_act)[has_tft ? cel : 0u];
+    }
 #pragma unroll
     for (uint32_t k = 0; k < TAIL_PD / 4; k++) {
         pre.act[k] = 0;
@@ -204,29 +232,12 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
     }
 #pragma unroll
     for (uint32_t k = 0; k < TAIL_PD; k++) {
-        const uint32_t x = tid + k * nth;
-        if (x < nd_t) {
-            uint32_t dj;
-            const uint32_t ct = map.den_cell(x, dj);
-            const uint32_t dl = map.cell(ct) * dens + dj;
-            const uint32_t dg = J.den_idz + dl;
-            pre.act[k / 4] |= (uint32_t)den_act[dl] << (8 * (k % 4));
-            if (J.ssc_energy != nullptr && dg < J.C)
-                pre.en[k / 4] |= (uint32_t)energy[dg] << (8 * (k % 4));
-        }
+        if (ok[k] & 1u) pre.act[k / 4] |= av[k] << (8 * (k % 4));
+        if (ok[k] == 3u) pre.en[k / 4] |= ev[k] << (8 * (k % 4));
     }
 #pragma unroll
-    for (uint32_t k = 0; k < TAIL_PC; k++) {
-        pre.tft[k] = 0;
-        const uint32_t ct = tid + k * nth;
-        if (J.tft[0] != nullptr && ct < cells_t) {
-            const uint32_t cel = map.cell(ct);
-            pre.tft[k] = (uint32_t)BSM_G(const uint8_t, J.tft[0])[cel] |
-                         ((uint32_t)BSM_G(const uint8_t, J.tft[1])[cel] << 8) |
-                         ((uint32_t)BSM_G(const uint8_t, J.tft[2])[cel] << 16) |
-                         ((uint32_t)BSM_G(const uint8_t, J.tft[3])[cel] << 24);
-        }
-    }
+    for (uint32_t k = 0; k < TAIL_PC; k++)
+        pre.tft[k] = has_tft ? tv[k][0] | (tv[k][1] << 8) | (tv[k][2] << 16) | (tv[k][3] << 24) : 0u;
 }
 
 // The fired bits of each tile chunk of a distal tuft (LDS, W bits per chunk in
@@ -391,10 +402,11 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
 template <int NW>
 struct ChunkMeta {
     uint32_t v[NW], u[NW];   // packed offsets (or v / u offsets)
-    uint32_t c, csh;         // staged-slice code word of the chunk's group, its shift
-    uint32_t s, ssh;         // strength-sign word, its shift
-    uint32_t so, den0, off0, q;
-    int32_t lb;
+    uint32_t c;              // staged-slice code word of the chunk's group
+    uint32_t s;              // strength-sign word of the chunk's 32-synapse group
+    uint32_t so, q;          // first synapse (tuft-relative), chunk index in the tile
+    uint32_t dd;             // tile dendrite of the chunk's cell | synapse in cell << 16
+    int32_t lb;              // window offset of the chunk's cell
 };
 
 // F1, check-free form: every synapse of each job sources a staged slice with
@@ -410,7 +422,7 @@ struct ChunkMeta {
 // dendrite, (sum over non-negative strengths, raw sum) come from a dot product
 // with the strength-sign bits and a sum of absolute differences.
 template <int W, int NST, bool PACKED>
-__global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
                                                     const WorkItem* __restrict__ items,
                                                     uint32_t n_items, uint32_t par,
                                                     uint32_t rnd_ofs, uint32_t stage_cap) {
@@ -439,22 +451,12 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
         const uint32_t rows = depth * tvn, cells_t = rows * tun, nd_t = cells_t * dens;
         for (uint32_t x = tid; x < nd_t; x += NTH) acc[x] = 0u;
         const SliceRt s0 = J.slices[J.staged[0]];
-        // virtual states: this item's share of the snapshot of the staged slices,
-        // and the parity's bit (set; cleared when this job writes its states)
-        {
-            const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;
-            if (J.virt) {
-                const uint32_t lo = idx * J.snap_per, hi = min(lo + J.snap_per, J.snap_len);
-                auto src = BSM_G(const uint8_t, J.axn + s0.idz);
-                auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
-                for (uint32_t x = lo + tid; x < hi; x += NTH) dst[x] = src[x];
-            }
-            if (idx == 0 && tid == 0) {
-                // bit pu: states virtual; bit 2 + pu: best-dendrite fired bits kept
-                const uint32_t keep = J.bmask && (W == 16 || (J.spd % W) == 0);
-                auto vb = BSM_G(uint32_t, J.vbits);
-                *vb = (*vb & ~(5u << pu)) | (J.virt ? (1u << pu) : 0u) | (keep ? (4u << pu) : 0u);
-            }
+        const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;   // the item among the job's
+        if (idx == 0 && tid == 0) {
+            // bit pu: states virtual; bit 2 + pu: best-dendrite fired bits kept
+            const uint32_t keep = J.bmask && (W == 16 || (J.spd % W) == 0);
+            auto vb = BSM_G(uint32_t, J.vbits);
+            *vb = (*vb & ~(5u << pu)) | (J.virt ? (1u << pu) : 0u) | (keep ? (4u << pu) : 0u);
         }
         // uniform per item: the geometry in scalar registers
         const uint32_t vsc = __builtin_amdgcn_readfirstlane(s0.v_scale);
@@ -474,37 +476,86 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
         // slot stride of the staged windows (32 rows with packed offsets)
         const uint32_t WSS = (PACKED && NST > 1) ? 32u * nc : WS;
         const uint32_t n_st = J.n_staged, SS = svs * sus;
-        const Quot q_nc = Quot::of(nc);
-        for (uint32_t j = 0; j < n_st; j++) {
-            auto src = BSM_G(const uint8_t, J.axn + s0.idz + j * SS);
-            for (uint32_t x = tid; x < WS; x += NTH) {
-                const uint32_t rr = q_nc(x), cc = x - rr * nc;
-                const int32_t gv = a0 + (int32_t)rr, gu = b0 + (int32_t)cc;
-                uint32_t val = 0;
-                if ((uint32_t)gv < svs && (uint32_t)gu < sus) val = src[(uint32_t)gv * sus + (uint32_t)gu];
-                lds[j * WSS + x] = (uint8_t)syn_fire(val);
+        const Quot q_nc = Quot::of(nc), q_ws = Quot::of(WS);
+        const uint32_t VU = V * U;
+
+        // ---- every global load of the prologue is issued before any is used:
+        // the window bytes (syn_fire of each axon of the tile + halo of each
+        // staged slice, zeros off the slice), the item's share of the snapshot
+        // of the staged slices (virtual states), the epilogue's inputs
+        auto axs = BSM_G(const uint8_t, J.axn + s0.idz);
+        const uint32_t wtot = n_st * WS;
+        constexpr int WK = 6;   // window bytes per thread in the first pass
+        uint32_t wv[WK], wok = 0;
+#pragma unroll
+        for (int k = 0; k < WK; k++) {
+            // a byte off the slice or past the window reads byte 0 and is zeroed below
+            const uint32_t x = tid + (uint32_t)k * NTH;
+            const uint32_t j = q_ws(x), xx = x - j * WS;
+            const uint32_t rr = q_nc(xx), cc = xx - rr * nc;
+            const int32_t gv = a0 + (int32_t)rr, gu = b0 + (int32_t)cc;
+            const bool in = x < wtot && (uint32_t)gv < svs && (uint32_t)gu < sus;
+            wv[k] = axs[in ? j * SS + (uint32_t)gv * sus + (uint32_t)gu : 0u];
+            wok |= (uint32_t)in << k;
+        }
+        const uint32_t snap_lo = idx * J.snap_per;
+        const uint32_t snap_hi = J.virt ? min(snap_lo + J.snap_per, J.snap_len) : 0u;
+        uint32_t sv = 0;
+        if (snap_lo + tid < snap_hi) sv = axs[snap_lo + tid];
+        // the epilogue's inputs, by tile cell index (no row table yet)
+        const Quot q_tun = Quot::of(tun), q_tvn = Quot::of(tvn);
+        struct DirectMap {
+            uint32_t tun, tvn, v0, u0, VU, U, dens;
+            Quot q_tun, q_tvn, q_dens;
+            __device__ uint32_t cell(uint32_t ct) const {
+                const uint32_t r = q_tun(ct), sl = q_tvn(r);
+                return sl * VU + (v0 + r - sl * tvn) * U + u0 + (ct - r * tun);
             }
+            __device__ uint32_t den_cell(uint32_t x, uint32_t& dj) const {
+                const uint32_t ct = q_dens(x);
+                dj = x - ct * dens;
+                return ct;
+            }
+        };
+        const DirectMap dmap{tun, tvn, v0, u0, VU, U, dens, q_tun, q_tvn, Quot::of(dens)};
+        TailPre pre;
+        tail_prefetch(J, dmap, nd_t, cells_t, tid, NTH, pre);
+        // ---- then the stores
+#pragma unroll
+        for (int k = 0; k < WK; k++) {
+            const uint32_t x = tid + (uint32_t)k * NTH;
+            if (x < wtot) {
+                const uint32_t j = q_ws(x);
+                lds[j * WSS + (x - j * WS)] = (uint8_t)syn_fire(((wok >> k) & 1u) ? wv[k] : 0u);
+            }
+        }
+        for (uint32_t x = tid + WK * NTH; x < wtot; x += NTH) {   // larger windows
+            const uint32_t j = q_ws(x), xx = x - j * WS;
+            const uint32_t rr = q_nc(xx), cc = xx - rr * nc;
+            const int32_t gv = a0 + (int32_t)rr, gu = b0 + (int32_t)cc;
+            uint32_t val = 0;
+            if ((uint32_t)gv < svs && (uint32_t)gu < sus) val = axs[j * SS + (uint32_t)gv * sus + (uint32_t)gu];
+            lds[j * WSS + xx] = (uint8_t)syn_fire(val);
+        }
+        {
+            auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
+            if (snap_lo + tid < snap_hi) dst[snap_lo + tid] = (uint8_t)sv;
+            for (uint32_t x = snap_lo + tid + NTH; x < snap_hi; x += NTH) dst[x] = axs[x];
         }
         // per tile row (slice, v): synapse offset and window offset of its first
         // cell, and the cell itself
         uint32_t* rowt = (uint32_t*)(lds + (((n_st - 1) * WSS + WS + 15) & ~15u));
         uint32_t* rowc = rowt + 2 * rows;
-        const uint32_t VU = V * U;
-        {
-            const Quot q_tvn = Quot::of(tvn);
-            for (uint32_t rq = tid; rq < rows; rq += NTH) {
-                const uint32_t sl = q_tvn(rq), v = v0 + (rq - sl * tvn);
-                const uint32_t c0 = sl * VU + v * U + u0;
-                rowc[rq] = c0;
-                rowt[2 * rq] = c0 * spt;
-                rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * (int32_t)nc +
-                                              umid - b0);
-            }
+        for (uint32_t rq = tid; rq < rows; rq += NTH) {
+            const uint32_t sl = q_tvn(rq), v = v0 + (rq - sl * tvn);
+            const uint32_t c0 = sl * VU + v * U + u0;
+            rowc[rq] = c0;
+            rowt[2 * rq] = c0 * spt;
+            rowt[2 * rq + 1] = (uint32_t)(((int32_t)((v * vsc) >> 4) + vmid - a0) * (int32_t)nc +
+                                          umid - b0);
         }
         __syncthreads();
-        const RowMap map{rowc, tun, dens, Quot::of(tun), Quot::of(dens)};
-        TailPre pre;
-        tail_prefetch(J, map, nd_t, cells_t, tid, NTH, pre);
+        const RowMap map{rowc, tun, dens, q_tun, Quot::of(dens)};
         const int32_t rnd = J.rnd[rnd_ofs + J.rnd_slot];
 
         // one buffer resource per synapse stream of the tuft (< 2^32 bytes)
@@ -539,18 +590,14 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
             const uint32_t rowq = q_rc(q);
             const uint32_t rem = q - __umul24(rowq, row_chunks);
             const uint32_t cc = q_cpt(rem);
-            m.den0 = __umul24(__umul24(rowq, tun) + cc, dens);
-            m.off0 = (rem - __umul24(cc, CPT)) * W;
+            m.dd = __umul24(__umul24(rowq, tun) + cc, dens) |
+                   (((rem - __umul24(cc, CPT)) * W) << 16);   // < 2^16 each (planner)
             const uint32_t so = rowt[2 * rowq] + rem * W;
             m.so = so;
             m.lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
-            if constexpr (NST > 1) {
+            if constexpr (NST > 1)
                 m.c = __builtin_amdgcn_raw_buffer_load_b32(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
-                m.csh = 2u * ((so & 15u) >> 2);
-            }
-            const uint32_t sb = sgb0 + so;
-            m.s = __builtin_amdgcn_raw_buffer_load_b32(r_sg, (int)((sb >> 5) << 2), 0, BUF_NT);
-            m.ssh = (sb & 31u) >> 2;
+            m.s = __builtin_amdgcn_raw_buffer_load_b32(r_sg, (int)(((sgb0 + so) >> 5) << 2), 0, BUF_NT);
             Chunk<W>::load_b(r_v, so, m.v);   // packed: (v + 8) << 4 | (u + 8) per byte
             if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, m.u);
         };
@@ -558,8 +605,10 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
             // window addresses as LDS byte addresses (the dynamic LDS base folded in)
             const int32_t lbp = m.lb - pbias + (int32_t)lds0;
             uint32_t cw = 0;
-            if constexpr (NST > 1) cw = m.c >> m.csh;    // word w of the chunk: bits 8k + 2w
-            const uint32_t nsg = ~(m.s >> m.ssh);        // word w: bit 8k + w set if strength >= 0
+            if constexpr (NST > 1) cw = m.c >> ((m.so & 12u) >> 1);   // word w: bits 8k + 2w
+            const uint32_t nsg = ~(m.s >> (((sgb0 + m.so) & 31u) >> 2));  // word w: bit 8k + w
+                                                                          // set if strength >= 0
+            const uint32_t den0 = m.dd & 0xFFFFu, off0 = m.dd >> 16;
             uint32_t x[NW];
 #pragma unroll
             for (int w = 0; w < NW; w++) {
@@ -616,32 +665,38 @@ __global__ __launch_bounds__(256) void kf_tuft_fast(const TuftJob* __restrict__ 
                     ps = __builtin_amdgcn_udot4(x[w], (nsg >> w) & 0x01010101u, ps, false);
                     rs = __builtin_amdgcn_sad_u8(x[w], 0u, rs);
                 }
-                if (valid) atomicAdd(&acc[m.den0 + __umulhi(m.off0, m_spd)], ps | (rs << 16));
+                if (valid) atomicAdd(&acc[den0 + __umulhi(off0, m_spd)], ps | (rs << 16));
             } else if (valid) {
 #pragma unroll
                 for (int w = 0; w < NW; w++) {
                     const uint32_t ps = __builtin_amdgcn_udot4(x[w], (nsg >> w) & 0x01010101u, 0u, false);
                     const uint32_t rs = __builtin_amdgcn_sad_u8(x[w], 0u, 0u);
-                    atomicAdd(&acc[m.den0 + __umulhi(m.off0 + (uint32_t)(4 * w), m_spd)],
+                    atomicAdd(&acc[den0 + __umulhi(off0 + (uint32_t)(4 * w), m_spd)],
                               ps | (rs << 16));
                 }
             }
         };
-        // two chunks per iteration, both loaded before the first is gathered (the
-        // second's loads stay in flight meanwhile); the loads are unconditional,
-        // their indices clamped to the last chunk, so that each gather waits for
-        // its own loads only
-        ChunkMeta<NW> A, B;
-        for (uint32_t qa = tid; qa < n_chunks; qa += 2 * NTH) {
-            const uint32_t qb = qa + NTH;
-            // A's loads, then B's, both ahead of the first gather
-            fetch(qa, A);
-            __builtin_amdgcn_sched_barrier(0);
-            fetch(min(qb, last), B);
-            __builtin_amdgcn_sched_barrier(0);
-            work(A, true);
-            work(B, qb < n_chunks);   // past the end: B re-gathers the last chunk, unused
-        }
+        // S chunks per iteration (4 when the tile has more than two chunks per
+        // lane, else 2), all loaded before the first is gathered, in order, so
+        // that each gather waits for its own loads only while the later chunks'
+        // loads stay in flight; the loads are unconditional, their indices
+        // clamped to the last chunk (a chunk past the end re-gathers the last
+        // one and is not used)
+        auto run = [&](auto S_) {
+            constexpr int S = decltype(S_)::value;
+            ChunkMeta<NW> M[S];
+            for (uint32_t q0 = tid; q0 < n_chunks; q0 += S * NTH) {
+#pragma unroll
+                for (int k = 0; k < S; k++) {
+                    fetch(min(q0 + (uint32_t)k * NTH, last), M[k]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int k = 0; k < S; k++) work(M[k], q0 + (uint32_t)k * NTH < n_chunks);
+            }
+        };
+        if (n_chunks > 2 * NTH) run(std::integral_constant<int, 4>{});
+        else run(std::integral_constant<int, 2>{});
         __syncthreads();
         const MaskOut mo{lmask, J.bmask + (size_t)pu * J.bm_len, CPT, J.spd, J.wpd, (uint32_t)W};
         tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, NTH, pre, mo);
