@@ -341,10 +341,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--autotune", action="store_true",
                     help="plain form: tile chosen by the autotuner instead of the fixed default")
-    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[1, 32],
-                    help="column tile of the plain form (default 1 32: with the one-stream "
-                         "schedule 112.8 us per cycle against 114.5 us for 2x16 on the side "
-                         "stream, 115.0 us for 2x16 on one stream)")
+    ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[4, 32],
+                    help="column tile of the plain form (default 4 32: with the one-stream "
+                         "schedule 80.5 us per cycle against 81.5 us at 2x32 or 4x16 and "
+                         "85.9 us at 1x32)")
+    ap.add_argument("--inhib-tile", type=int, default=0,
+                    help="side of the inhibition tile (8, 16, 32; 0: the planner's choice)")
     ap.add_argument("--schedule", choices=("one", "side"), default="one",
                     help="plain form: one stream (default), or the independent pyramidal "
                          "tuft on a side stream")
@@ -380,6 +382,8 @@ def main():
 
     if not args.autotune:
         tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
+        if args.inhib_tile:
+            tuning["inhib_tile"] = args.inhib_tile
         area.set_tuning(**tuning)
     plain_overlap = 2 if (args.serial or args.schedule == "one") else 1
     area.set_tuning(overlap=plain_overlap)
