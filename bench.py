@@ -343,8 +343,11 @@ def main():
                     help="plain form: tile chosen by the autotuner instead of the fixed default")
     ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[4, 32],
                     help="column tile of the plain form (default 4 32: with the one-stream "
-                         "schedule 80.5 us per cycle against 81.5 us at 2x32 or 4x16 and "
-                         "85.9 us at 1x32)")
+                         "schedule and 32-synapse chunks 76.4 us per cycle against 78.4 us at "
+                         "2x32, 79.5 us at 4x16 and 84.1 us at 1x32)")
+    ap.add_argument("--chunk", type=int, default=32,
+                    help="synapses per lane per chunk of the plain form's synapse stage "
+                         "(default 32: 76.4 us per cycle at 4x32 tiles against 79.9 us at 16)")
     ap.add_argument("--inhib-tile", type=int, default=0,
                     help="side of the inhibition tile (8, 16, 32; 0: the planner's choice)")
     ap.add_argument("--schedule", choices=("one", "side"), default="one",
@@ -381,7 +384,7 @@ def main():
     ctx.upload(dev, frames)
 
     if not args.autotune:
-        tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=16, halo=6)
+        tuning = dict(tile_v=args.tile[0], tile_u=args.tile[1], chunk=args.chunk, halo=6)
         if args.inhib_tile:
             tuning["inhib_tile"] = args.inhib_tile
         area.set_tuning(**tuning)
@@ -402,7 +405,7 @@ def main():
         from bismit_amd.autotune import tune_area
         tiles = [(8, 8), (4, 8), (2, 8), (2, 16), (1, 32), (4, 16), (1, 16)]
         tuning, _ = tune_area(scheme, ctx, cycles=10,
-                              cands=[dict(tile_v=tv, tile_u=tu, chunk=16, halo=6)
+                              cands=[dict(tile_v=tv, tile_u=tu, chunk=32, halo=6)
                                      for tv, tu in tiles])
         area.set_tuning(**tuning)
     area.run(dev, in_len, in_len, args.warmup)

@@ -1,7 +1,7 @@
 """Project autotuner of the fused kernels' launch geometry.
 
 Picks columns per workgroup tile (tile_v x tile_u), the synapse chunk width a
-lane streams (16/8/4 bytes of each metadata array) and the LDS halo staged
+lane streams (32/16/8 synapses of each metadata stream) and the LDS halo staged
 around each tile, by timing a few cycles of each candidate on scratch copies of
 the areas (the real areas' state is never advanced).  Every candidate computes
 identical results (tests/test_gpu_parity.py::test_cycle_tunings).
@@ -11,7 +11,7 @@ import itertools
 from .cortex import Batch, CorticalArea
 
 TILES = [(8, 8), (8, 16), (16, 8), (16, 16), (4, 16), (8, 32)]
-CHUNKS = [16, 8]
+CHUNKS = [32, 16, 8]
 HALOS = [6, 8]
 
 

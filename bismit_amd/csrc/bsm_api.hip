@@ -615,8 +615,8 @@ int merge_tuning(const Tuning& cur, const bsm_tuning* t, Tuning* out) {
     if (nt.tile_v > 64 || nt.tile_u > 64 || nt.tile_v * nt.tile_u > 1024)
         return fail("tile too large");
     if (nt.halo > 64) return fail("halo too large");
-    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16)
-        return fail("chunk must be 1, 4, 8 or 16");
+    if (nt.chunk != 1 && nt.chunk != 4 && nt.chunk != 8 && nt.chunk != 16 && nt.chunk != 32)
+        return fail("chunk must be 1, 4, 8, 16 or 32");
     if (nt.inhib_tile != 0 && nt.inhib_tile != 8 && nt.inhib_tile != 16 && nt.inhib_tile != 32)
         return fail("inhib_tile must be 1 (planner's choice), 8, 16 or 32");
     if (t->overlap > 2) return fail("overlap must be 0 (keep), 1 (on) or 2 (off)");

@@ -46,6 +46,22 @@ constexpr int BUF_NT = 2;
 
 template <int W>
 struct Chunk;
+// 32 synapses per lane (check-free gather only): two 16-byte vectors
+template <>
+struct Chunk<32> {
+    static constexpr int NW = 8;
+    __device__ static inline void load_b(Rsrc r, uint32_t off, uint32_t* o) {
+        u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, BUF_NT);
+        u32x4 y = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off + 16, 0, BUF_NT);
+        o[0] = x.x; o[1] = x.y; o[2] = x.z; o[3] = x.w;
+        o[4] = y.x; o[5] = y.y; o[6] = y.z; o[7] = y.w;
+    }
+    __device__ static inline void store_b(Rsrc r, uint32_t off, const uint32_t* o) {
+        u32x4 x = {o[0], o[1], o[2], o[3]}, y = {o[4], o[5], o[6], o[7]};
+        __builtin_amdgcn_raw_buffer_store_b128(x, r, (int)off, 0, BUF_NT);
+        __builtin_amdgcn_raw_buffer_store_b128(y, r, (int)off + 16, 0, BUF_NT);
+    }
+};
 template <>
 struct Chunk<16> {
     static constexpr int NW = 4;
@@ -241,11 +257,12 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
 }
 
 // The fired bits of each tile chunk of a distal tuft (LDS, W bits per chunk in
-// a 16-bit slot), copied for each cell's best dendrite to the tuft's kept bits
-// of this parity (dst; cells' words of wpd, bit i = synapse i of the dendrite).
-// With 16-synapse chunks the slots form one bit stream (a cell's synapses are
-// contiguous), so a dendrite of any length is a bit range of it; narrower
-// chunks are copied slot by slot, whole chunks only.
+// a 16-bit slot, 32-bit for 32-synapse chunks), copied for each cell's best
+// dendrite to the tuft's kept bits of this parity (dst; cells' words of wpd,
+// bit i = synapse i of the dendrite).  With 16- or 32-synapse chunks the slots
+// form one bit stream (a cell's synapses are contiguous), so a dendrite of any
+// length is a bit range of it; narrower chunks are copied slot by slot, whole
+// chunks only.
 struct MaskOut {
     const uint16_t* lm;   // null: none kept (4-byte aligned, one spare word)
     uint32_t* dst;
@@ -341,9 +358,9 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             t3[cel] = (uint8_t)tstate;
             if (mo.lm) {
                 auto dst = BSM_G(uint32_t, mo.dst + (size_t)cel * mo.wpd);
-                if (mo.w == 16) {
+                if (mo.w >= 16) {
                     const uint32_t* l32 = (const uint32_t*)mo.lm;
-                    const uint32_t start = ct * mo.cpt * 16 + best_id * mo.spd;
+                    const uint32_t start = ct * mo.cpt * mo.w + best_id * mo.spd;
                     for (uint32_t i = 0; i < mo.wpd; i++) {
                         const uint32_t bp = start + 32 * i, k = bp >> 5;
                         uint32_t v = __builtin_amdgcn_alignbit(l32[k + 1], l32[k], bp & 31u);
@@ -402,7 +419,7 @@ __device__ inline void acc_chunk(uint32_t* acc, uint32_t den0, uint32_t off0, co
 template <int NW>
 struct ChunkMeta {
     uint32_t v[NW], u[NW];   // packed offsets (or v / u offsets)
-    uint32_t c;              // staged-slice code word of the chunk's group
+    uint32_t c[(NW + 3) / 4];  // staged-slice code words of the chunk's 16-synapse groups
     uint32_t s;              // strength-sign word of the chunk's 32-synapse group
     uint32_t so, q;          // first synapse (tuft-relative), chunk index in the tile
     uint32_t dd;             // tile dendrite of the chunk's cell | synapse in cell << 16
@@ -454,7 +471,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;   // the item among the job's
         if (idx == 0 && tid == 0) {
             // bit pu: states virtual; bit 2 + pu: best-dendrite fired bits kept
-            const uint32_t keep = J.bmask && (W == 16 || (J.spd % W) == 0);
+            const uint32_t keep = J.bmask && (W >= 16 || (J.spd % W) == 0);
             auto vb = BSM_G(uint32_t, J.vbits);
             *vb = (*vb & ~(5u << pu)) | (J.virt ? (1u << pu) : 0u) | (keep ? (4u << pu) : 0u);
         }
@@ -582,7 +599,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         const uint32_t m_spd = 0xFFFFFFFFu / J.spd + 1u;
         const uint32_t last = n_chunks - 1;
         // distal tufts: each chunk's fired bits, for the best dendrites' copy
-        uint16_t* lmask = (J.bmask && (W == 16 || whole))
+        uint16_t* lmask = (J.bmask && (W >= 16 || whole))
                               ? (uint16_t*)(lds + ((stage_cap + nd_t * 6 + 3) & ~3u)) : nullptr;
 
         auto fetch = [&](uint32_t q, ChunkMeta<NW>& m) {
@@ -595,8 +612,13 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
             const uint32_t so = rowt[2 * rowq] + rem * W;
             m.so = so;
             m.lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
-            if constexpr (NST > 1)
-                m.c = __builtin_amdgcn_raw_buffer_load_b32(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
+            if constexpr (NST > 1 && W == 32) {
+                const u32x2 cc2 = __builtin_amdgcn_raw_buffer_load_b64(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
+                m.c[0] = cc2.x;
+                m.c[1] = cc2.y;
+            } else if constexpr (NST > 1) {
+                m.c[0] = __builtin_amdgcn_raw_buffer_load_b32(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
+            }
             m.s = __builtin_amdgcn_raw_buffer_load_b32(r_sg, (int)(((sgb0 + so) >> 5) << 2), 0, BUF_NT);
             Chunk<W>::load_b(r_v, so, m.v);   // packed: (v + 8) << 4 | (u + 8) per byte
             if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, m.u);
@@ -604,21 +626,27 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         auto work = [&](const ChunkMeta<NW>& m, bool valid) {
             // window addresses as LDS byte addresses (the dynamic LDS base folded in)
             const int32_t lbp = m.lb - pbias + (int32_t)lds0;
-            uint32_t cw = 0;
-            if constexpr (NST > 1) cw = m.c >> ((m.so & 12u) >> 1);   // word w: bits 8k + 2w
+            // codes of word w (within its 16-synapse group: bits 8k + 2(w % 4))
+            uint32_t cwv[(NW + 3) / 4] = {};
+            if constexpr (NST > 1) {
+                if constexpr (W == 32) { cwv[0] = m.c[0]; cwv[1] = m.c[1]; }
+                else cwv[0] = m.c[0] >> ((m.so & 12u) >> 1);
+            }
             const uint32_t nsg = ~(m.s >> (((sgb0 + m.so) & 31u) >> 2));  // word w: bit 8k + w
                                                                           // set if strength >= 0
             const uint32_t den0 = m.dd & 0xFFFFu, off0 = m.dd >> 16;
-            uint32_t x[NW];
+            uint32_t x[NW], aa[NW][4];
 #pragma unroll
             for (int w = 0; w < NW; w++) {
-                uint32_t a[4];
+                uint32_t* a = aa[w];
                 if constexpr (PACKED) {
                     uint32_t vw = (m.v[w] >> 4) & 0x0F0F0F0Fu;
                     const uint32_t uw = m.v[w] & 0x0F0F0F0Fu;
                     if constexpr (NST > 1) {
                         // code k of this word to bits 5-6 of byte k: v + 8 + 32 * code
-                        const uint32_t cs = (2 * w <= 5) ? (cw << (5 - 2 * w)) : (cw >> (2 * w - 5));
+                        const uint32_t cw = cwv[w >> 2];
+                        const int w4 = w & 3;
+                        const uint32_t cs = (2 * w4 <= 5) ? (cw << (5 - 2 * w4)) : (cw >> (2 * w4 - 5));
                         vw |= cs & 0x60606060u;
                     }
                     // [v_b, u_b, v_b+1, u_b+1] byte pairs -> dot4 gives base + v*cols + u
@@ -635,28 +663,43 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                     if constexpr (NST > 1) {
 #pragma unroll
                         for (int k = 0; k < 4; k++)
-                            l[k] = (int32_t)__umul24((cw >> (8 * k + 2 * w)) & 3u, WSS) + lbp;
+                            l[k] = (int32_t)__umul24((cwv[w >> 2] >> (8 * k + 2 * (w & 3))) & 3u, WSS) + lbp;
                     }
                     a[0] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p01, (int32_t)coefA, l[0], false);
                     a[1] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p01, (int32_t)coefB, l[1], false);
                     a[2] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefA, l[2], false);
                     a[3] = (uint32_t)__builtin_amdgcn_sdot4((int32_t)p23, (int32_t)coefB, l[3], false);
                 }
-                x[w] = (uint32_t)*LDS8(a[0]) | ((uint32_t)*LDS8(a[1]) << 8) |
-                       ((uint32_t)*LDS8(a[2]) << 16) | ((uint32_t)*LDS8(a[3]) << 24);
             }
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t bb[NW][4];
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) bb[w][k] = *LDS8(aa[w][k]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int w = 0; w < NW; w++)
+                x[w] = __builtin_amdgcn_perm(__builtin_amdgcn_perm(bb[w][3], bb[w][2], 0x0c0c0400u),
+                                             __builtin_amdgcn_perm(bb[w][1], bb[w][0], 0x0c0c0400u),
+                                             0x05040100u);
             if (wr && valid) Chunk<W>::store_b(r_out, m.so, x);
             if (lmask && valid) {
                 // fired = bit 7 of each state byte; bit 4w + k for byte k of word w
-                uint32_t lo = 0, hi = 0;
-                lo = __builtin_amdgcn_udot4((x[0] >> 7) & 0x01010101u, 0x08040201u, lo, false);
-                if constexpr (NW > 1)
-                    lo = __builtin_amdgcn_udot4((x[1] >> 7) & 0x01010101u, 0x80402010u, lo, false);
-                if constexpr (NW > 2) {
-                    hi = __builtin_amdgcn_udot4((x[2] >> 7) & 0x01010101u, 0x08040201u, hi, false);
-                    hi = __builtin_amdgcn_udot4((x[3] >> 7) & 0x01010101u, 0x80402010u, hi, false);
+                uint32_t bt[(NW + 1) / 2];
+#pragma unroll
+                for (int h = 0; h < (NW + 1) / 2; h++) {
+                    bt[h] = __builtin_amdgcn_udot4((x[2 * h] >> 7) & 0x01010101u, 0x08040201u, 0u, false);
+                    if (2 * h + 1 < NW)
+                        bt[h] = __builtin_amdgcn_udot4((x[2 * h + 1] >> 7) & 0x01010101u, 0x80402010u,
+                                                       bt[h], false);
                 }
-                lmask[m.q] = (uint16_t)(lo | (hi << 8));
+                if constexpr (W == 32)
+                    ((uint32_t*)lmask)[m.q] = bt[0] | (bt[1] << 8) | (bt[2] << 16) | (bt[3] << 24);
+                else if constexpr (NW > 2)
+                    lmask[m.q] = (uint16_t)(bt[0] | (bt[1] << 8));
+                else
+                    lmask[m.q] = (uint16_t)bt[0];
             }
             if (whole) {
                 uint32_t ps = 0, rs = 0;
@@ -695,7 +738,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                 for (int k = 0; k < S; k++) work(M[k], q0 + (uint32_t)k * NTH < n_chunks);
             }
         };
-        if (n_chunks > 2 * NTH) run(std::integral_constant<int, 4>{});
+        if (W < 32 && n_chunks > 2 * NTH) run(std::integral_constant<int, 4>{});
         else run(std::integral_constant<int, 2>{});
         __syncthreads();
         const MaskOut mo{lmask, J.bmask + (size_t)pu * J.bm_len, CPT, J.spd, J.wpd, (uint32_t)W};
@@ -1275,7 +1318,7 @@ namespace bsm {
 static constexpr uint32_t LDS_LIMIT = 64 * 1024;
 
 static uint32_t pick_w(uint64_t a, uint64_t b, uint64_t c, uint32_t max_w) {
-    const uint32_t ws[4] = {16, 8, 4, 1};
+    const uint32_t ws[5] = {32, 16, 8, 4, 1};
     for (uint32_t w : ws)
         if (w <= max_w && a % w == 0 && b % w == 0 && c % w == 0) return w;
     return 1;
@@ -1515,7 +1558,7 @@ static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fas
 int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st) {
-    uint32_t W = 16, nst = 0;
+    uint32_t W = 32, nst = 0;
     bool fast = true, packed = true;
     for (uint32_t k = 0; k < job_ids.size(); k++) {
         W = std::min(W, job_w[k]);
@@ -1524,11 +1567,13 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
         packed &= jobs[job_ids[k]].packed != 0;
     }
     fast &= W >= 4;
-    // a distal tuft's per-chunk fired bits (check-free path, whole chunks): 2 B
-    // per chunk of the tile
+    if (!fast) W = std::min(W, 16u);   // 32-synapse chunks: check-free kernel only
+    // a distal tuft's per-chunk fired bits (check-free path; whole chunks below
+    // 16 synapses): 2 B per chunk of the tile, 4 B for 32-synapse chunks, plus
+    // alignment and a spare word
     auto mask_bytes = [&](const TuftJob& j, uint32_t tiles) -> uint32_t {
-        return (fast && j.bmask && (W == 16 || j.spd % W == 0))
-                   ? 2 * j.depth * tiles * (j.spt / W) + 8 : 0u;   // + alignment, spare word
+        return (fast && j.bmask && (W >= 16 || j.spd % W == 0))
+                   ? (W == 32 ? 4u : 2u) * j.depth * tiles * (j.spt / W) + 8 : 0u;
     };
     packed &= fast;
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
@@ -1875,6 +1920,8 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
     const WorkItem* it = items + st.item0;
     if (st.kind == 1 && st.fast) {
         switch ((st.W * 8 + st.nst) * 2 + (st.packed ? 1 : 0)) {
+            F1_FAST_CASE(32, 1, 0) F1_FAST_CASE(32, 2, 0)
+            F1_FAST_CASE(32, 1, 1) F1_FAST_CASE(32, 2, 1)
             F1_FAST_CASE(16, 1, 0) F1_FAST_CASE(16, 2, 0)
             F1_FAST_CASE(8, 1, 0) F1_FAST_CASE(8, 2, 0)
             F1_FAST_CASE(4, 1, 0) F1_FAST_CASE(4, 2, 0)

@@ -235,7 +235,7 @@ struct LayerRt {
 };
 
 struct Tuning {
-    uint32_t tile_v = 8, tile_u = 8, halo = 8, chunk = 16;
+    uint32_t tile_v = 8, tile_u = 8, halo = 8, chunk = 32;
     uint32_t inhib_tile = 0;  // side of the square inhibition tile; 0 = chosen by the planner
     uint32_t overlap = 1;     // run independent pyramidal tufts on the side stream
 };

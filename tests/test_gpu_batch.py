@@ -150,7 +150,8 @@ def test_cortex_cycle_matches_per_area_calls(ctx):
 def test_config3_plan_uses_fast_wide_variant(ctx):
     """Every C3 tuft has synapses per dendrite a multiple of 4 and offsets within
     6 columns: both persistent tuft launches must take the check-free gather at
-    16 synapses per lane."""
+    16 synapses per lane (a launch takes the narrowest chunk its tufts allow, and
+    some C3 tufts' synapses per tuft are not multiples of 32)."""
     schemes = recipes.config3_areas(64, seed=0)
     areas = make_areas(ctx, schemes, 1000)
     b = Batch(areas, ctx)

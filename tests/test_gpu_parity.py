@@ -156,6 +156,8 @@ def test_cycle_config1_multi_cycle_runs(ctx):
     dict(tile_v=16, tile_u=8, halo=8, chunk=8, inhib_tile=32),
     dict(tile_v=8, tile_u=16, halo=3, chunk=16),
     dict(tile_v=3, tile_u=5, halo=2, chunk=1, inhib_tile=8),
+    dict(tile_v=2, tile_u=16, halo=7, chunk=32),
+    dict(tile_v=4, tile_u=32, halo=7, chunk=32, overlap=2),
 ])
 def test_cycle_tunings(ctx, tuning):
     """Any tile/halo/chunk/inhibition-tile choice gives identical results
@@ -233,12 +235,12 @@ def test_cycle_config2_full_size(ctx):
     g, o = make_pair(ctx, scheme)
     frames = inputs.SdrStream(256 * 256, 0.02, 0.5, seed=0).frames(3)
     run_cycles(g, o, scheme, frames, 3, tag="c2")
-    # the path measured at this size: both tuft launches check-free, 16 synapses per lane
+    # the path measured at this size: both tuft launches check-free, 32 synapses per lane
     plan = g.stage_plan()
     assert [(p["kind"], p["stream"]) for p in plan] == [(1, 0), (2, 0), (1, 1), (1, 0), (3, 0)]
     for p in plan:
         if p["kind"] == 1:
-            assert p["chunk"] == 16 and p["fast"] == 1 and p["packed"] == 1, p
+            assert p["chunk"] == 32 and p["fast"] == 1 and p["packed"] == 1, p
     # bytes of the pyramidal tuft launches, one tuft each: per synapse the packed
     # offsets (1 B) and the strength's sign bit, plus a 2-bit staged-slice code
     # for the distal tuft (two staged slices), no state written; 4 B/dendrite,
