@@ -315,6 +315,25 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             denc[nd_t + x] = (uint8_t)st_o;
         }
     };
+    // the inputs of the next TAIL_PE dendrites of each thread are loaded
+    // together before any dendrite is finished (indices clamped, unused loads
+    // discarded), so that they share one round trip
+    constexpr uint32_t TAIL_PE = 2;
+    const bool more = nd_t > TAIL_PD * nth;
+    uint32_t a2[TAIL_PE], e2[TAIL_PE];
+    if (more) {
+        const bool has_en = J.ssc_energy != nullptr;
+#pragma unroll
+        for (uint32_t k = 0; k < TAIL_PE; k++) {
+            const uint32_t x = tid + (TAIL_PD + k) * nth;
+            uint32_t dj;
+            const uint32_t ct = map.den_cell(x < nd_t ? x : 0u, dj);
+            const uint32_t dl = map.cell(ct) * dens + dj;
+            const uint32_t dg = J.den_idz + dl;
+            a2[k] = den_act[dl];
+            e2[k] = has_en ? (uint32_t)energy[dg < J.C ? dg : 0u] : 0u;
+        }
+    }
 #pragma unroll
     for (uint32_t k = 0; k < TAIL_PD; k++) {
         const uint32_t x = tid + k * nth;
@@ -322,7 +341,14 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             den_one(x, (pre.act[k / 4] >> (8 * (k % 4))) & 0xFFu,
                     (pre.en[k / 4] >> (8 * (k % 4))) & 0xFFu, true);
     }
-    for (uint32_t x = tid + TAIL_PD * nth; x < nd_t; x += nth) den_one(x, 0u, 0u, false);
+    if (more) {
+#pragma unroll
+        for (uint32_t k = 0; k < TAIL_PE; k++) {
+            const uint32_t x = tid + (TAIL_PD + k) * nth;
+            if (x < nd_t) den_one(x, a2[k], e2[k], true);
+        }
+    }
+    for (uint32_t x = tid + (TAIL_PD + TAIL_PE) * nth; x < nd_t; x += nth) den_one(x, 0u, 0u, false);
 
     // ---- tufts (tft_cycle)
     if (do_tft) {
@@ -602,7 +628,10 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         uint16_t* lmask = (J.bmask && (W >= 16 || whole))
                               ? (uint16_t*)(lds + ((stage_cap + nd_t * 6 + 3) & ~3u)) : nullptr;
 
-        auto fetch = [&](uint32_t q, ChunkMeta<NW>& m) {
+        // MS: the job stages several slices (reads staged-slice codes); a job of
+        // one staged slice in a several-slice launch runs the loop without them
+        auto fetch = [&](auto MS_, uint32_t q, ChunkMeta<NW>& m) {
+            constexpr bool MS = decltype(MS_)::value;
             m.q = q;
             const uint32_t rowq = q_rc(q);
             const uint32_t rem = q - __umul24(rowq, row_chunks);
@@ -612,23 +641,24 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
             const uint32_t so = rowt[2 * rowq] + rem * W;
             m.so = so;
             m.lb = (int32_t)rowt[2 * rowq + 1] + (int32_t)(__umul24(u0 + cc, usc) >> 4);
-            if constexpr (NST > 1 && W == 32) {
+            if constexpr (MS && W == 32) {
                 const u32x2 cc2 = __builtin_amdgcn_raw_buffer_load_b64(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
                 m.c[0] = cc2.x;
                 m.c[1] = cc2.y;
-            } else if constexpr (NST > 1) {
+            } else if constexpr (MS) {
                 m.c[0] = __builtin_amdgcn_raw_buffer_load_b32(r_slc, (int)((so >> 4) << 2), 0, BUF_NT);
             }
             m.s = __builtin_amdgcn_raw_buffer_load_b32(r_sg, (int)(((sgb0 + so) >> 5) << 2), 0, BUF_NT);
             Chunk<W>::load_b(r_v, so, m.v);   // packed: (v + 8) << 4 | (u + 8) per byte
             if constexpr (!PACKED) Chunk<W>::load_b(r_u, so, m.u);
         };
-        auto work = [&](const ChunkMeta<NW>& m, bool valid) {
+        auto work = [&](auto MS_, const ChunkMeta<NW>& m, bool valid) {
+            constexpr bool MS = decltype(MS_)::value;
             // window addresses as LDS byte addresses (the dynamic LDS base folded in)
             const int32_t lbp = m.lb - pbias + (int32_t)lds0;
             // codes of word w (within its 16-synapse group: bits 8k + 2(w % 4))
             uint32_t cwv[(NW + 3) / 4] = {};
-            if constexpr (NST > 1) {
+            if constexpr (MS) {
                 if constexpr (W == 32) { cwv[0] = m.c[0]; cwv[1] = m.c[1]; }
                 else cwv[0] = m.c[0] >> ((m.so & 12u) >> 1);
             }
@@ -642,7 +672,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                 if constexpr (PACKED) {
                     uint32_t vw = (m.v[w] >> 4) & 0x0F0F0F0Fu;
                     const uint32_t uw = m.v[w] & 0x0F0F0F0Fu;
-                    if constexpr (NST > 1) {
+                    if constexpr (MS) {
                         // code k of this word to bits 5-6 of byte k: v + 8 + 32 * code
                         const uint32_t cw = cwv[w >> 2];
                         const int w4 = w & 3;
@@ -660,7 +690,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                     const uint32_t p01 = __builtin_amdgcn_perm(m.u[w], m.v[w], 0x05010400u);
                     const uint32_t p23 = __builtin_amdgcn_perm(m.u[w], m.v[w], 0x07030602u);
                     int32_t l[4] = {lbp, lbp, lbp, lbp};
-                    if constexpr (NST > 1) {
+                    if constexpr (MS) {
 #pragma unroll
                         for (int k = 0; k < 4; k++)
                             l[k] = (int32_t)__umul24((cwv[w >> 2] >> (8 * k + 2 * (w & 3))) & 3u, WSS) + lbp;
@@ -725,21 +755,31 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         // loads stay in flight; the loads are unconditional, their indices
         // clamped to the last chunk (a chunk past the end re-gathers the last
         // one and is not used)
-        auto run = [&](auto S_) {
+        auto run = [&](auto S_, auto MS_) {
             constexpr int S = decltype(S_)::value;
             ChunkMeta<NW> M[S];
             for (uint32_t q0 = tid; q0 < n_chunks; q0 += S * NTH) {
 #pragma unroll
                 for (int k = 0; k < S; k++) {
-                    fetch(min(q0 + (uint32_t)k * NTH, last), M[k]);
+                    fetch(MS_, min(q0 + (uint32_t)k * NTH, last), M[k]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
-                for (int k = 0; k < S; k++) work(M[k], q0 + (uint32_t)k * NTH < n_chunks);
+                for (int k = 0; k < S; k++) work(MS_, M[k], q0 + (uint32_t)k * NTH < n_chunks);
             }
         };
-        if (W < 32 && n_chunks > 2 * NTH) run(std::integral_constant<int, 4>{});
-        else run(std::integral_constant<int, 2>{});
+        auto run_s = [&](auto MS_) {
+            if (W < 32 && n_chunks > 2 * NTH) run(std::integral_constant<int, 4>{}, MS_);
+            else run(std::integral_constant<int, 2>{}, MS_);
+        };
+        if constexpr (NST > 1 && W == 32) {
+            if (__builtin_amdgcn_readfirstlane((uint32_t)multi)) run_s(std::true_type{});
+            else run_s(std::false_type{});
+        } else {
+            // narrower chunks keep four chunks in flight: one loop body (a second
+            // would raise the register count past five waves per SIMD)
+            run_s(std::integral_constant<bool, (NST > 1)>{});
+        }
         __syncthreads();
         const MaskOut mo{lmask, J.bmask + (size_t)pu * J.bm_len, CPT, J.spd, J.wpd, (uint32_t)W};
         tuft_tail(J, lds, acc, stage_cap, nd_t, cells_t, map, rnd, tid, NTH, pre, mo);
