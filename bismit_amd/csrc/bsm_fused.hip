@@ -1012,6 +1012,39 @@ struct LtpChunk {
 template <int W>
 constexpr int ltp_u() { return W <= 4 ? 4 : 1; }
 
+// inhib_simple's test for radius R fully unrolled (cl/control.cl:28-135): a
+// neighbour at hex distance d inhibits when n * (M - (d - c)^2) > cs * M, so
+// only the largest neighbour state of each ring matters (the factor depends on
+// d alone, and a non-positive factor never inhibits; the cell itself, d = 0,
+// never does).  ctr is the cell's window byte, rows are WU bytes apart.
+template <int R>
+__device__ inline uint32_t hex_uninhibited(const uint8_t* win, uint32_t ctr, uint32_t WU) {
+    constexpr int M = (R + INHIB_HORIZ_OFS) * (R + INHIB_HORIZ_OFS);
+    uint32_t rmax[R + 1];
+#pragma unroll
+    for (int d = 0; d <= R; d++) rmax[d] = 0;
+#pragma unroll
+    for (int vo = -R; vo <= R; vo++) {
+        // the row's first byte (u offset -R); the bytes read are u offsets within the hex
+        const uint8_t* row = win + (ctr - (uint32_t)R + (uint32_t)(vo * (int)WU));
+#pragma unroll
+        for (int uo = -R; uo <= R; uo++) {
+            const int w = -vo - uo;
+            if (w < -R || w > R || (vo == 0 && uo == 0)) continue;
+            const int d = ((vo < 0 ? -vo : vo) + (uo < 0 ? -uo : uo) + (w < 0 ? -w : w)) >> 1;
+            rmax[d] = max(rmax[d], (uint32_t)row[uo + R]);
+        }
+    }
+    const int32_t ci = (int32_t)win[ctr] * M;
+    uint32_t unin = 1;
+#pragma unroll
+    for (int d = 1; d <= R; d++) {
+        const int f = M - (d - INHIB_CENTER_OFS) * (d - INHIB_CENTER_OFS);
+        if (f > 0) unin &= (uint32_t)(ci >= (int32_t)rmax[d] * f);
+    }
+    return unin;
+}
+
 constexpr uint32_t F2_MAXC = 4;  // tile cells per thread (tiles up to 4 x 256 cells)
 
 template <int W>
@@ -1083,6 +1116,10 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             for (uint32_t y = tid; y < nh; y += nth) {
                 const uint32_t x = hexl[y];
                 const uint32_t tr = x / tun, tc = x - tr * tun;
+                if (r == 4) {   // the reference radius (cl/control.cl:9)
+                    hexo[x] = (uint8_t)hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU);
+                    continue;
+                }
                 const int32_t ci = (int32_t)lds[(tr + r) * WU + tc + r] * imax;
                 uint32_t unin = 1;
                 for (int32_t vo = -r; vo <= r && unin; vo++) {
