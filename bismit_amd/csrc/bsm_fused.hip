@@ -1102,9 +1102,11 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         __syncthreads();
 
         const int32_t imax = (r + INHIB_HORIZ_OFS) * (r + INHIB_HORIZ_OFS);
-        if (J.mode == 0) {
-            // The hex test decides only cells with a non-zero soma: pack those
-            // densely so that lanes of silent cells do not run the loop.
+        if (J.mode == 0 && r != 4) {
+            // Radii other than the reference's: the generic hex test decides only
+            // cells with a non-zero soma, packed densely so that lanes of silent
+            // cells do not run its loop.  (At radius 4 each cell's thread runs the
+            // unrolled test itself below.)
             for (uint32_t x = tid; x < tvn * tun; x += nth) {
                 const uint32_t tr = x / tun, tc = x - tr * tun;
                 int safe;
@@ -1116,10 +1118,6 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             for (uint32_t y = tid; y < nh; y += nth) {
                 const uint32_t x = hexl[y];
                 const uint32_t tr = x / tun, tc = x - tr * tun;
-                if (r == 4) {   // the reference radius (cl/control.cl:9)
-                    hexo[x] = (uint8_t)hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU);
-                    continue;
-                }
                 const int32_t ci = (int32_t)lds[(tr + r) * WU + tc + r] * imax;
                 uint32_t unin = 1;
                 for (int32_t vo = -r; vo <= r && unin; vo++) {
@@ -1153,7 +1151,11 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                 uint32_t is_act;
                 if (J.mode == 0) {
                     cs = safe ? cs : 0u;
-                    const uint32_t unin = cs ? (uint32_t)hexo[x] : 1u;
+                    // radius 4, the reference's (cl/control.cl:9): the unrolled test
+                    const uint32_t unin =
+                        cs ? (r == 4 ? hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU)
+                                     : (uint32_t)hexo[x])
+                           : 1u;
                     out = unin ? cs : 0u;
                     is_act = unin & (cs != 0);
                 } else {
@@ -1205,7 +1207,10 @@ constexpr uint32_t F3_G = 4;  // slices whose per-column inputs are loaded ahead
 // slices at a time: column output (inhibition / passthrough), the distal
 // learning decisions (queued in LDS), then the queued dendrites' LTP by the
 // whole block.  Every load of a group is issued before its stores.
-template <int W>
+// PRE: the first group's pass-2 inputs are loaded with pass 1's (one memory
+// round trip fewer per item, more registers: for launches of about one
+// workgroup per CU, where occupancy does not matter).
+template <int W, bool PRE>
 __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
                                                    uint32_t n_items, uint32_t par, uint32_t rnd_ofs) {
@@ -1240,6 +1245,36 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
         const uint32_t dmask = J.distal_mask, n_tufts = J.n_tufts;
         const uint32_t v = col / U, u = col - v * U;
 
+        const uint32_t pq = (p & 1) ^ 1;   // the previous cycle's states
+        // pass 2's per-cell inputs of a group of F3_G slices
+        auto load2 = [&](uint32_t s0, uint32_t* fl, uint32_t* ain,
+                         uint32_t (*pbr)[BSM_MAX_TUFTS], uint32_t (*pbi)[BSM_MAX_TUFTS]) {
+#pragma unroll
+            for (uint32_t g = 0; g < F3_G; g++) {
+                const uint32_t c = (s0 + g) * VU + col;
+                const bool in = valid && s0 + g < depth;
+                fl[g] = (in && learn) ? flags[c] : 0u;
+                ain[g] = (in && !out_ctl) ? axn[J.slices[J.base_slc].idz + c] : 0u;
+#pragma unroll
+                for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
+                    const bool on = in && learn && t < n_tufts && ((dmask >> t) & 1u);
+                    pbr[g][t] = on ? pb_raw[t * C + c] : 0u;
+                    pbi[g][t] = on ? pb_ids[t * C + c] : 0u;
+                }
+            }
+        };
+        // loads that do not depend on pass 1, issued with its loads: the first
+        // group's pass-2 inputs and where each distal tuft's previous states are
+        // (uniform per item: bit 0 virtual, else the state buffer; bit 2 its best
+        // dendrites' fired bits kept, else the snapshot)
+        uint32_t fl0[F3_G], ain0[F3_G], pbr0[F3_G][BSM_MAX_TUFTS], pbi0[F3_G][BSM_MAX_TUFTS];
+        if constexpr (PRE) load2(0, fl0, ain0, pbr0, pbi0);
+        uint32_t pv[BSM_MAX_TUFTS];
+#pragma unroll
+        for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++)
+            pv[t] = (learn && t < n_tufts && ((dmask >> t) & 1u))
+                        ? *BSM_G(const uint32_t, J.t_vbits[t]) >> pq : 0u;
+
         // pass 1: pyr_cycle, running intra-column best (Q3 reset quirk)
         uint32_t cs_r[F3_G] = {0, 0, 0, 0};
         uint32_t best = 0;
@@ -1272,19 +1307,24 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
             if (valid) {
                 uint32_t cs[F3_G], fl[F3_G], ain[F3_G], pbr[F3_G][BSM_MAX_TUFTS],
                     pbi[F3_G][BSM_MAX_TUFTS];
+                if (PRE && s0 == 0) {
 #pragma unroll
-                for (uint32_t g = 0; g < F3_G; g++) {
-                    const uint32_t c = (s0 + g) * VU + col;
-                    const bool in = s0 + g < depth;
-                    cs[g] = s0 == 0 ? cs_r[g] : (in ? (uint32_t)soma[c] : 0u);
-                    fl[g] = (in && learn) ? flags[c] : 0u;
-                    ain[g] = (in && !out_ctl) ? axn[J.slices[J.base_slc].idz + c] : 0u;
+                    for (uint32_t g = 0; g < F3_G; g++) {
+                        cs[g] = cs_r[g];
+                        fl[g] = fl0[g];
+                        ain[g] = ain0[g];
 #pragma unroll
-                    for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
-                        const bool on = in && learn && t < n_tufts && ((dmask >> t) & 1u);
-                        pbr[g][t] = on ? pb_raw[t * C + c] : 0u;
-                        pbi[g][t] = on ? pb_ids[t * C + c] : 0u;
+                        for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
+                            pbr[g][t] = pbr0[g][t];
+                            pbi[g][t] = pbi0[g][t];
+                        }
                     }
+                } else {
+                    load2(s0, fl, ain, pbr, pbi);
+#pragma unroll
+                    for (uint32_t g = 0; g < F3_G; g++)
+                        cs[g] = s0 == 0 ? cs_r[g]
+                                        : (s0 + g < depth ? (uint32_t)soma[(s0 + g) * VU + col] : 0u);
                 }
 #pragma unroll
                 for (uint32_t g = 0; g < F3_G; g++) {
@@ -1324,7 +1364,6 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
             __syncthreads();
             if (learn) {
                 const uint32_t nl = n_l;
-                const uint32_t pq = (p & 1) ^ 1;   // the previous cycle's states
                 const uint8_t* __restrict__ prev_st = J.st[pq];
                 const uint32_t maxch = J.max_chunks;
                 const int32_t r0 = J.rnd[rnd_ofs + J.slot_mtp0];
@@ -1335,14 +1374,6 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                 // at once (a chunk past a dendrite or the queue loads the queue's
                 // first chunk and is not applied)
                 const uint32_t total = nl * maxch;
-                // where each distal tuft's previous states are, uniform per item:
-                // bit 0 virtual (else the state buffer), bit 2 its best dendrites'
-                // fired bits kept (else the snapshot)
-                uint32_t pv[BSM_MAX_TUFTS];
-#pragma unroll
-                for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++)
-                    pv[t] = (t < n_tufts && ((dmask >> t) & 1u))
-                                ? *BSM_G(const uint32_t, J.t_vbits[t]) >> pq : 0u;
                 const Quot q_mc = Quot::of(maxch);
                 constexpr int LU = ltp_u<W>();
                 for (uint32_t y0 = tid; y0 < total; y0 += LU * blockDim.x) {
@@ -2023,11 +2054,20 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
         default: kf_ssc_post<1><<<st.grid, 256, st.lds_bytes, s>>>(sj, it, st.n_items, par, rnd_ofs, st.tv, st.tu); break;
         }
     } else {
+        // preload when the launch fits one round of four workgroups per CU (the
+        // preloading form's occupancy) on MI355X's 256 CUs
+        const bool pre = st.n_items <= 4 * 256;
         switch (st.W) {
-        case 16: kf_pyr_post<16><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
-        case 8: kf_pyr_post<8><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
-        case 4: kf_pyr_post<4><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
-        default: kf_pyr_post<1><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); break;
+#define F3_CASE(w)                                                                           \
+    case w:                                                                                  \
+        if (pre) kf_pyr_post<w, true><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); \
+        else kf_pyr_post<w, false><<<st.grid, F3_COLS, 0, s>>>(pj, it, st.n_items, par, rnd_ofs); \
+        break;
+        F3_CASE(16)
+        F3_CASE(8)
+        F3_CASE(4)
+        default: F3_CASE(1)
+#undef F3_CASE
         }
     }
     BSM_HIP(hipGetLastError());
