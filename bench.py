@@ -62,21 +62,34 @@ def baseline_threads():
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "kf_tuft_config2_pmc.md")
 
 
-def syn_bytes(stage, tuft, kind=2):
-    """Per-synapse HBM bytes of one tuft in an F1 launch.  Check-free gather: a
+def writes_states(stage, kind, t):
+    """Whether a check-free launch's tuft t of a layer of this kind writes its
+    states: only tuft 0 of a spiny stellate layer (kind 1), and only where its
+    fired bits are not stored (chunks below 8 synapses, or a launch staging
+    several slices); every other tuft's states are virtual."""
+    return kind == 1 and t == 0 and (stage["chunk"] < 8 or stage["staged"] > 1)
+
+
+def syn_bytes(stage, tuft, kind=2, t=0):
+    """Per-synapse HBM bytes of tuft t in an F1 launch.  Check-free gather: a
     2-bit staged-slice code when it stages several slices and nothing for one,
-    the offsets (1 B packed, else 2 B) and one strength-sign bit; pyramidal
-    tufts (kind 2) keep their states virtual (none written), spiny stellate
-    ones write them (1 B).  Checked gather: the 1 B slice id, 2 B offsets,
-    strength in and state out."""
+    the offsets (1 B packed, else 2 B) and one strength-sign bit; states are
+    virtual (none written), except that tuft 0 of a spiny stellate layer stores
+    each synapse's fired bit for its layer's learning (or its 1 B state, see
+    writes_states).  Checked gather: the 1 B slice id, 2 B offsets, strength in
+    and state out."""
     if stage["fast"]:
         slc = 0 if len(set(tuft.src_slcs)) <= 1 else 0.25
-        return slc + (1 if stage["packed"] else 2) + 0.125 + (0 if kind == 2 else 1)
+        out = 0
+        if kind == 1 and t == 0:
+            out = 1 if writes_states(stage, kind, t) else 0.125
+        return slc + (1 if stage["packed"] else 2) + 0.125 + out
     return 1 + 2 + 2
 
 
 def snap_bytes(scheme, tuft):
-    """Bytes of a check-free pyramidal tuft's snapshot: its staged source slices."""
+    """Bytes of a check-free tuft's snapshot (virtual states): its staged source
+    slices."""
     sl = scheme.slices[min(tuft.src_slcs)]
     return len(set(tuft.src_slcs)) * sl.v_size * sl.u_size
 
@@ -90,11 +103,11 @@ def stage_bytes(scheme, li, kind, stage):
     C = scheme.cells(li)
     if kind == 1:
         b = 0
-        for t in lay.tufts:
-            b += int(C * t.syns_per_tft * syn_bytes(stage, t, lay.kind)) + C * t.dens_per_tft * 4
+        for ti, t in enumerate(lay.tufts):
+            b += int(C * t.syns_per_tft * syn_bytes(stage, t, lay.kind, ti)) + C * t.dens_per_tft * 4
             if lay.kind == 2:
                 b += C * 12
-            if stage["fast"] and lay.kind == 2:
+            if stage["fast"] and not writes_states(stage, lay.kind, ti):
                 b += 2 * snap_bytes(scheme, t)
         if lay.kind == 1:
             b += C
@@ -456,7 +469,8 @@ def main():
             "how": "host frame per cycle through bsm_area_intake (synchronous copy), then "
                    "bsm_area_cycle with host-drawn rnd; Python host loop"}
     (dk, dl), (dt, alg_tot, n_launch, dstage) = max(groups.items(), key=lambda kv: kv[1][0])
-    syn_bs = [syn_bytes(dstage, t, scheme.layers[dl].kind) for t in scheme.layers[dl].tufts]
+    syn_bs = [syn_bytes(dstage, t, scheme.layers[dl].kind, ti)
+              for ti, t in enumerate(scheme.layers[dl].tufts)]
     # the engine's count equals the recipe's
     assert alg_tot == stage_bytes(scheme, dl, dk, dstage), (alg_tot, syn_bs)
     avg_s = dt / n_prof / n_launch / 1000.0

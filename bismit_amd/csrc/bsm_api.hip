@@ -159,6 +159,9 @@ static void plan_area(bsm_area* a, ArenaPlan& ap) {
         for (uint32_t t = 0; t < L.d.n_tufts; t++) ap.add(&L.snap[t], 2 * (size_t)L.snap_cap[t]);
         for (uint32_t t = 0; t < L.d.n_tufts; t++)
             ap.add(&L.bmask[t], 2 * sizeof(uint32_t) * (size_t)L.cells * L.bm_wpd[t]);
+        // spiny stellate layers: tuft 0's fired bits (one bit per synapse)
+        ap.add(&L.fbits, (L.d.kind == BSM_LAYER_SSC && L.d.n_tufts)
+                             ? ((size_t)L.cells * L.t[0].spt + 31) / 32 * 4 : 0);
         ap.add(&L.vbits, sizeof(uint32_t) * BSM_MAX_TUFTS);
         ap.add(&L.vt_d, sizeof(VirtTuft) * BSM_MAX_TUFTS);
         ap.add(&L.den_states, nd);

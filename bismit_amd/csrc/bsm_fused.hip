@@ -615,6 +615,13 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         // a job whose states are not virtual writes them (its fired window bytes)
         const Rsrc r_out = make_rsrc(J.st[pu], J.virt ? 0u : nrec);
         const bool wr = !J.virt;
+        // a virtual spiny stellate tuft stores each chunk's fired bits (W / 8
+        // bytes at byte so / 8) for its layer's learning (launches of one staged
+        // slice and chunks of 8 or more synapses; the planner keeps every other
+        // spiny stellate tuft writing its states)
+        constexpr bool FB = NST == 1 && W >= 8;
+        const bool fbw = FB && J.fb != nullptr;
+        const Rsrc r_fb = make_rsrc(J.fb, fbw ? (nrec + 7) / 8 : 0u);
         // packed offsets carry v + 8 and u + 8: their bias folds into the base
         const int32_t pbias = PACKED ? 8 * (int32_t)nc + 8 : 0;
         const uint32_t coefA = (nc & 0xFFu) | (1u << 8);
@@ -714,7 +721,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                                              __builtin_amdgcn_perm(bb[w][1], bb[w][0], 0x0c0c0400u),
                                              0x05040100u);
             if (wr && valid) Chunk<W>::store_b(r_out, m.so, x);
-            if (lmask && valid) {
+            if ((lmask || (FB && fbw)) && valid) {
                 // fired = bit 7 of each state byte; bit 4w + k for byte k of word w
                 uint32_t bt[(NW + 1) / 2];
 #pragma unroll
@@ -724,12 +731,21 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                         bt[h] = __builtin_amdgcn_udot4((x[2 * h + 1] >> 7) & 0x01010101u, 0x80402010u,
                                                        bt[h], false);
                 }
-                if constexpr (W == 32)
-                    ((uint32_t*)lmask)[m.q] = bt[0] | (bt[1] << 8) | (bt[2] << 16) | (bt[3] << 24);
-                else if constexpr (NW > 2)
-                    lmask[m.q] = (uint16_t)(bt[0] | (bt[1] << 8));
-                else
-                    lmask[m.q] = (uint16_t)bt[0];
+                if constexpr (W == 32) {
+                    const uint32_t fv = bt[0] | (bt[1] << 8) | (bt[2] << 16) | (bt[3] << 24);
+                    if (lmask) ((uint32_t*)lmask)[m.q] = fv;
+                    if constexpr (FB)
+                        if (fbw) __builtin_amdgcn_raw_buffer_store_b32(fv, r_fb, (int)(m.so >> 3), 0, 0);
+                } else if constexpr (NW > 2) {
+                    const uint32_t fv = bt[0] | (bt[1] << 8);
+                    if (lmask) lmask[m.q] = (uint16_t)fv;
+                    if constexpr (FB)
+                        if (fbw) __builtin_amdgcn_raw_buffer_store_b16((uint16_t)fv, r_fb, (int)(m.so >> 3), 0, 0);
+                } else {
+                    if (lmask) lmask[m.q] = (uint16_t)bt[0];
+                    if constexpr (FB && W == 8)
+                        if (fbw) __builtin_amdgcn_raw_buffer_store_b8((uint8_t)bt[0], r_fb, (int)(m.so >> 3), 0, 0);
+                }
             }
             if (whole) {
                 uint32_t ps = 0, rs = 0;
@@ -1077,6 +1093,8 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         // rnd values, the own slice entry, this thread's first cell's rating
         const int32_t rnd_c = J.mode == 0 ? J.rnd[rnd_ofs + J.slot_ctrl] : 0;
         const int32_t rnd_m = J.learn ? J.rnd[rnd_ofs + J.slot_mtp] : 0;
+        // where this cycle's states of the learned tuft are (bit p: virtual)
+        const uint32_t vb0 = J.learn ? *BSM_G(const uint32_t, J.vbits) : 0u;
         const SliceRt own = J.slices[J.base_slc + slc];
         uint32_t actp[F2_MAXC];  // activity ratings of this thread's cells
 #pragma unroll
@@ -1175,7 +1193,9 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             // else decrement (0x1FF); ltp_u chunks per thread in flight at once (a
             // chunk past the list loads chunk 0 again and is not applied)
             const uint32_t na = n_act, spt = J.spt, CPC = spt / W, total = na * CPC;
-            // this cycle's states (spiny stellate tufts always write theirs)
+            // this cycle's states: virtual -> the fired bits F1 stored, else the
+            // state buffer
+            const bool virt = (vb0 >> (p & 1)) & 1u;
             const uint8_t* __restrict__ st = J.st[p & 1];
             constexpr int LU = ltp_u<W>();
             for (uint32_t y0 = tid; y0 < total; y0 += LU * nth) {
@@ -1187,7 +1207,8 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     const uint32_t ci = y / CPC, k = y - ci * CPC;
                     zz[j] = list[ci] * spt;
                     ii[j] = zz[j] + k * W;
-                    ch[j].load(st, J.str, ii[j]);
+                    if (virt) ch[j].load_bits(J.fb, ii[j], J.str, ii[j]);   // tuft 0 starts the layer
+                    else ch[j].load(st, J.str, ii[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < LU; j++)
@@ -1375,6 +1396,21 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                 // first chunk and is not applied)
                 const uint32_t total = nl * maxch;
                 const Quot q_mc = Quot::of(maxch);
+                // the distal tufts' geometry, uniform per item, picked per queued
+                // dendrite by its tuft (no per-lane loads of job fields)
+                uint32_t f_spd[BSM_MAX_TUFTS], f_spt[BSM_MAX_TUFTS], f_idz[BSM_MAX_TUFTS],
+                    f_wpd[BSM_MAX_TUFTS];
+                const uint32_t* f_bm[BSM_MAX_TUFTS];
+#pragma unroll
+                for (uint32_t t = 0; t < BSM_MAX_TUFTS; t++) {
+                    f_spd[t] = J.t_spd[t];
+                    f_spt[t] = J.t_spt[t];
+                    f_idz[t] = J.t_syn_idz[t];
+                    f_wpd[t] = J.t_wpd[t];
+                    f_bm[t] = J.t_bmask[t] + (size_t)pq * J.t_bmlen[t];
+                }
+                auto pick = [](const auto* a, uint32_t t) { return t == 0 ? a[0] : (t == 1 ? a[1] : a[2]); };
+                static_assert(BSM_MAX_TUFTS == 3, "pick() selects among three tufts");
                 constexpr int LU = ltp_u<W>();
                 for (uint32_t y0 = tid; y0 < total; y0 += LU * blockDim.x) {
                     LtpChunk<W> ch[LU];
@@ -1387,17 +1423,17 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
                         const uint32_t k = y < total ? y - li * maxch : 0u;
                         const uint32_t t = lt[li], e = lz[li];
                         const uint32_t cel = e & 0xFFFFFFu, best = e >> 24;
+                        const uint32_t spd = pick(f_spd, t);
                         tt[j] = t;
-                        ok[j] = y < total && k * W < J.t_spd[t];
-                        zz[j] = cel * J.t_spt[t] + J.t_syn_idz[t] + best * J.t_spd[t];
+                        ok[j] = y < total && k * W < spd;
+                        zz[j] = cel * pick(f_spt, t) + pick(f_idz, t) + best * spd;
                         ii[j] = zz[j] + (ok[j] ? k * W : 0u);
-                        const uint32_t vb = t == 0 ? pv[0] : (t == 1 ? pv[1] : pv[2]);
+                        const uint32_t vb = pick(pv, t);
                         if (vb & 1u) {
                             // virtual previous states: the kept fired bits of the
                             // best dendrite, else the snapshot
                             if (vb & 4u)
-                                ch[j].load_bits(J.t_bmask[t] + (size_t)pq * J.t_bmlen[t] +
-                                                    (size_t)cel * J.t_wpd[t],
+                                ch[j].load_bits(pick(f_bm, t) + (size_t)cel * pick(f_wpd, t),
                                                 ii[j] - zz[j], J.str, ii[j]);
                             else
                                 ch[j].load_virt(J.vt[t], J.vt[t].snap + (size_t)pq * J.vt[t].snap_len,
@@ -1540,10 +1576,11 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     // geometry, so contiguous in the axon space) are its snapshot
     j.snap = L.snap[t];
     j.snap_len = fast ? n * (uint32_t)j.st_vsize[0] * j.st_usize[0] : 0u;
-    // Pyramidal tufts' states are virtual.  A spiny stellate tuft writes its
-    // states: its launch is bound by latency, not bytes, and its layer's
-    // learning reads the states of every active cell's synapses.
-    j.virt = L.d.kind == BSM_LAYER_PYR ? 1u : 0u;
+    // Every fast tuft's states are virtual.  Spiny stellate learning needs only
+    // whether each synapse of an active cell fired: tuft 0 of a spiny stellate
+    // layer stores those bits instead of the states (1 bit per synapse, not 8).
+    j.virt = 1u;
+    j.fb = (L.d.kind == BSM_LAYER_SSC && t == 0) ? L.fbits : nullptr;
     j.bmask = L.bm_wpd[t] ? L.bmask[t] : nullptr;
     j.wpd = L.bm_wpd[t];
     j.bm_len = L.cells * L.bm_wpd[t];
@@ -1581,6 +1618,8 @@ SscJob make_ssc_job(bsm_area* a, uint32_t l, const int32_t* rnd) {
     j.learn = !(a->settings & BSM_SET_DISABLE_LEARNING);
     j.par_xor = 0;
     j.sg = L.syn_sg;
+    j.vbits = L.vbits;
+    j.fb = L.fbits;
     return j;
 }
 
@@ -1684,6 +1723,14 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
                    ? (W == 32 ? 4u : 2u) * j.depth * tiles * (j.spt / W) + 8 : 0u;
     };
     packed &= fast;
+    // fired bits are stored whole bytes per chunk, in launches of one staged
+    // slice: otherwise (or in the checked kernel) a spiny stellate tuft writes
+    // its states instead
+    for (uint32_t id : job_ids)
+        if (jobs[id].fb && (!fast || W < 8 || nst > 1)) {
+            jobs[id].fb = nullptr;
+            jobs[id].virt = 0;
+        }
     uint32_t TV = tn.tile_v, TU = tn.tile_u;
     // several staged slices with packed offsets: window slots of 32 rows, so a
     // window (tile rows + reach) must fit in 32 rows
@@ -1778,14 +1825,16 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
     st.alg_bytes = 0;
     // per synapse, in eighth bytes: fast (virtual states): the 2-bit staged-slice
     // code when several slices are staged (nothing for one), the offsets (1 B
-    // packed, 2 B otherwise) and the strength's sign bit, no state written;
-    // checked: the 1 B slice id, 2 B offsets, the strength in and the state out
+    // packed, 2 B otherwise) and the strength's sign bit, no state written (a
+    // spiny stellate tuft's fired bit out; a fast tuft that is not virtual, its
+    // state out); checked: the 1 B slice id, 2 B offsets, the strength in and
+    // the state out
     for (uint32_t id : job_ids) {
         // per dendrite raw/state out + activity in/out; per cell-tuft 4 B in + 8 B
         // out (pyramidal); per SSC cell its energy; fast: the snapshot copy
         const TuftJob& j = jobs[id];
         const uint32_t syn_e = fast ? (j.n_staged <= 1 ? 0u : 2u) + (packed ? 8u : 16u) + 1u +
-                                          (j.virt ? 0u : 8u)
+                                          (j.virt ? 0u : 8u) + (j.fb ? 1u : 0u)
                                     : 8u + 16u + 8u + 8u;
         st.alg_bytes += (uint64_t)j.C * j.spt * syn_e / 8 + (uint64_t)j.C * j.dens * 4 +
                         (j.tft[0] ? (uint64_t)j.C * 12 : 0) + (j.ssc_energy ? (uint64_t)j.C : 0) +

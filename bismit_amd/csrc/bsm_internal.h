@@ -107,6 +107,10 @@ struct TuftJob {
     uint32_t virt;
     uint32_t* bmask;       // distal tufts: best-dendrite fired bits (see VirtTuft), or null
     uint32_t bm_len, wpd;
+    // spiny stellate tufts with virtual states: the fired bit of every synapse of
+    // the cycle (bit i = tuft synapse i), which the layer's learning reads; null
+    // for other tufts
+    uint32_t* fb;
     uint8_t* snap;         // parity q at snap + q * snap_len
     uint32_t snap_len;
     uint32_t* vbits;
@@ -134,6 +138,10 @@ struct SscJob {
     uint32_t slot_ctrl, slot_mtp;
     uint32_t V, U, depth, C;
     uint32_t spt, base_slc, radius;
+    // where this cycle's states of tuft 0 are: bit p of *vbits set -> virtual,
+    // fired bits in fb (see TuftJob::fb); else the state buffer st[p]
+    const uint32_t* vbits;
+    const uint32_t* fb;
     uint32_t mode;         // 0 inhib_simple, 1 passthrough(rnd 0), 2 none
     uint32_t learn;
     uint32_t par_xor;
@@ -219,6 +227,7 @@ struct LayerRt {
     uint32_t snap_cap[BSM_MAX_TUFTS];  // bytes per parity
     uint32_t* bmask[BSM_MAX_TUFTS];    // distal pyramidal tufts: best-dendrite fired bits
     uint32_t bm_wpd[BSM_MAX_TUFTS];    // words per dendrite (0: none kept)
+    uint32_t* fbits;                   // spiny stellate layers: tuft 0's fired bits
     uint32_t* vbits;
     VirtTuft* vt_d;
     uint8_t *den_states, *den_raw, *den_energies, *den_act, *den_thr;

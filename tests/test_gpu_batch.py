@@ -164,16 +164,17 @@ def test_config3_plan_uses_fast_wide_variant(ctx):
 
     def launch_bytes(p, pick):
         # per synapse, in eighth bytes: a 2-bit staged-slice code unless the tuft
-        # stages one slice, packed offsets, the strength's sign bit; spiny
-        # stellate tufts write their states (1 B), pyramidal ones keep them
-        # virtual (the snapshot of the staged slices read and written instead)
-        def syn_e(lay, t):
-            return (0 if len(set(t.src_slcs)) <= 1 else 2) + 8 + 1 + (8 if lay.kind == 1 else 0)
+        # stages one slice, packed offsets, the strength's sign bit; states are
+        # virtual (the snapshot of the staged slices read and written instead),
+        # and a spiny stellate layer's tuft 0 stores each synapse's fired bit
+        def syn_e(lay, t, ti):
+            return (0 if len(set(t.src_slcs)) <= 1 else 2) + 8 + 1 + (
+                1 if lay.kind == 1 and ti == 0 else 0)
 
         def snap(s, lay, t):
             sl = s.slices[min(t.src_slcs)]
-            return len(set(t.src_slcs)) * sl.v_size * sl.u_size if lay.kind == 2 else 0
-        return sum(s.cells(li) * t.syns_per_tft * syn_e(lay, t) // 8 +
+            return len(set(t.src_slcs)) * sl.v_size * sl.u_size
+        return sum(s.cells(li) * t.syns_per_tft * syn_e(lay, t, ti) // 8 +
                    s.cells(li) * (t.dens_per_tft * 4 + (12 if lay.kind == 2 else 0)) +
                    (s.cells(li) if lay.kind == 1 else 0) + 2 * snap(s, lay, t)
                    for s in schemes for li, lay in enumerate(s.layers)

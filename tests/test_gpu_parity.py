@@ -249,6 +249,10 @@ def test_cycle_config2_full_size(ctx):
     assert [p["alg_bytes"] for p in plan if p["kind"] == 1][1:] == \
         [cells * 512 * e // 8 + cells * 8 * 4 + cells * 12 + 2 * n * 256 * 256
          for e, n in ((11, 2), (9, 1))]
+    # the spiny stellate launch: packed offsets, sign bit and the stored fired
+    # bit per synapse (10 eighths), 4 B/dendrite, 1 B/cell (energy), the
+    # snapshot of its one staged slice read and written
+    assert plan[0]["alg_bytes"] == 256 * 256 * (128 * 10 // 8 + 4 * 4 + 1) + 2 * 256 * 256
     g.close()
     o.close()
 
