@@ -2124,8 +2124,9 @@ int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob
 }
 
 // Device-to-device frame copy into the axon space (AxonSpace::intake,
-// src/cortex/axon_space.rs:509-629): one block per INTAKE_SEG bytes.
-constexpr uint32_t INTAKE_SEG = 16384;
+// src/cortex/axon_space.rs:509-629): one block per INTAKE_SEG bytes (16 B per
+// thread for a 64 KiB frame: 16 blocks, one load and one store each).
+constexpr uint32_t INTAKE_SEG = 4096;
 __global__ __launch_bounds__(256) void k_intake(uint8_t* __restrict__ dst,
                                                 const uint8_t* __restrict__ src, uint32_t len) {
     const uint32_t lo = blockIdx.x * INTAKE_SEG;
