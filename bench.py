@@ -356,11 +356,11 @@ def main():
                     help="plain form: tile chosen by the autotuner instead of the fixed default")
     ap.add_argument("--tile", type=int, nargs=2, metavar=("V", "U"), default=[4, 32],
                     help="column tile of the plain form (default 4 32: with the one-stream "
-                         "schedule and 32-synapse chunks 76.4 us per cycle against 78.4 us at "
-                         "2x32, 79.5 us at 4x16 and 84.1 us at 1x32)")
+                         "schedule and 32-synapse chunks the fastest of 1x32, 2x32, 4x16, 4x32 "
+                         "and 8x8 on MI355X)")
     ap.add_argument("--chunk", type=int, default=32,
                     help="synapses per lane per chunk of the plain form's synapse stage "
-                         "(default 32: 76.4 us per cycle at 4x32 tiles against 79.9 us at 16)")
+                         "(default 32: about 4% faster per cycle than 16 at 4x32 tiles)")
     ap.add_argument("--inhib-tile", type=int, default=0,
                     help="side of the inhibition tile (8, 16, 32; 0: the planner's choice)")
     ap.add_argument("--schedule", choices=("one", "side"), default="one",
