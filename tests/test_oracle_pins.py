@@ -896,3 +896,156 @@ def test_activity_rating_decrement_tests_the_updated_rating(lib):
     # (no increment can happen): 2^16 trials at 40000 expect 40000 decrements
     dec = sum(1 - lib.orc_update_activity_rating(1, 0, 4242, s, 40000) for s in range(n))
     assert abs(dec - 40000) < 600, dec
+
+
+# ------------------------------------------------- area cycle: enqueue order
+
+
+def _three_layer_scheme(settings):
+    """Input slice, a spiny stellate layer, a pyramidal layer with intra-column
+    inhibition (proximal + basal distal tuft) and a pyramidal layer whose
+    control is pyr_output (proximal + apical distal tuft)."""
+    S = recipes.SliceScheme
+    T = recipes.TuftScheme
+    side = 16
+    slices = [S.spatial_input(16, 16, side, side), S(side, side), S(side, side), S(side, side),
+              S(side, side)]
+    layers = [
+        recipes.LayerScheme("iv", L.LAYER_SSC, 1, 1, [T.basal_proximal(2, 8, 40, [0])],
+                            L.CTRL_INHIB_SIMPLE, 2),
+        recipes.LayerScheme("iii", L.LAYER_PYR, 2, 2,
+                            [T.basal_proximal(2, 8, 0, [1]), T.basal_distal(2, 8, 30, [2, 3])],
+                            L.CTRL_INTRA_COLUMN),
+        recipes.LayerScheme("v", L.LAYER_PYR, 1, 4,
+                            [T.basal_proximal(2, 8, 0, [2, 3]), T.apical_distal(2, 8, 30, [4])],
+                            L.CTRL_PYR_OUTPUT),
+    ]
+    return recipes.AreaScheme("order", side, side, slices, layers, settings, input_slcs=[0])
+
+
+def _cycle_by_kernels(o, sch, r):
+    """One CorticalArea::cycle driven kernel by kernel in the reference's
+    enqueue order (src/cortex/cortical_area.rs:716-789): spatial layers cycle
+    (spiny_stellates.rs:220-252: flags, gather and dendrites per tuft
+    (dendrites.rs:191-209, synapses.rs:363-387), soma, control post), then
+    spatial layers learn (:256-268), then every other layer cycles
+    (pyramidals.rs:297-335: dendrites, tft_cycle per tuft, pyr_cycle, control
+    post, tft_dst_mtp per distal tuft, tufts.rs:304-358).  The draws are
+    taken per layer in descriptor order as den[t], control, learning[t]."""
+    s = sch.settings
+    learn = not s & L.SET_DISABLE_LEARNING
+    slot0 = np.cumsum([0] + [2 * len(l.tufts) + 1 for l in sch.layers])
+
+    def dens(li, lay):
+        o.k("syn_flags", li)
+        for t in range(len(lay.tufts)):
+            o.k("syn_cycle", li, t)
+        for t in range(len(lay.tufts)):
+            o.k("den_cycle", li, t, int(r[slot0[li] + t]))
+
+    def control(li, lay):
+        rc = int(r[slot0[li] + len(lay.tufts)])
+        if lay.control == L.CTRL_INHIB_SIMPLE:
+            if s & L.SET_BYPASS_INHIB:
+                o.k("inhib_passthrough", li, 0)
+            else:
+                o.k("inhib_simple", li, rc)
+        elif lay.control == L.CTRL_INTRA_COLUMN:
+            o.k("inhib_intra_column", li)
+        elif lay.control == L.CTRL_PYR_OUTPUT:
+            o.k("pyr_output", li)
+
+    spatial = [(li, l) for li, l in enumerate(sch.layers) if l.spatial]
+    other = [(li, l) for li, l in enumerate(sch.layers) if not l.spatial]
+    if not s & L.SET_DISABLE_SSCS:
+        for li, lay in spatial:
+            dens(li, lay)
+            o.k("ssc_cycle", li)
+            control(li, lay)
+        if learn:
+            for li, lay in spatial:
+                o.k("ssc_mtp", li, int(r[slot0[li] + len(lay.tufts) + 1]) & M32)
+    if not s & L.SET_DISABLE_PYRS:
+        for li, lay in other:
+            dens(li, lay)
+            for t in range(len(lay.tufts)):
+                o.k("tft_cycle", li, t)
+            o.k("pyr_cycle", li)
+            control(li, lay)
+            if learn:
+                for t, ts in enumerate(lay.tufts):
+                    if ts.den_kind == L.DEN_DISTAL:
+                        o.k("tft_dst_mtp", li, t, int(r[slot0[li] + len(lay.tufts) + 1 + t]))
+
+
+@pytest.mark.parametrize("settings", [0, L.SET_DISABLE_LEARNING, L.SET_BYPASS_INHIB,
+                                      L.SET_DISABLE_SSCS, L.SET_DISABLE_PYRS])
+def test_area_cycle_is_the_reference_enqueue_order(settings):
+    """orc_area_cycle equals the single-kernel entry points (each pinned above)
+    run in the reference's enqueue order, over 12 cycles with learning on real
+    activity: a swapped stage (learning before inhibition, tft_dst_mtp before
+    pyr_cycle, spatial learning inside the spatial cycle loop) or a draw taken
+    from the wrong slot changes some buffer."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from helpers import host_layout, load_layout, layer_bufs
+    sch = _three_layer_scheme(settings)
+    a, b = P.OracleArea(sch), P.OracleArea(sch)
+    lay = host_layout(sch, seed=3, reach=3)
+    load_layout(a, sch, lay)
+    load_layout(b, sch, lay)
+    rng = np.random.default_rng(50)
+    st0 = []
+    for li in range(len(sch.layers)):
+        st = rng.integers(-20, 30, sch.layer_synapses(li)).astype(np.int8)
+        a.write("SYN_STRENGTHS", li, st)
+        b.write("SYN_STRENGTHS", li, st)
+        st0.append(st)
+    assert a.n_slots == sch.rnd_slots() == 3 + 5 + 5
+    n_cyc = 12
+    rnd = rng.integers(-2**31, 2**31, (n_cyc, a.n_slots)).astype(np.int32)
+    for c in range(n_cyc):
+        frame = ((rng.random(256) < 0.3) * rng.integers(1, 256, 256)).astype(np.uint8)
+        a.intake(frame)
+        b.intake(frame)
+        a.cycle(rnd[c], 1)
+        _cycle_by_kernels(b, sch, rnd[c])
+        assert np.array_equal(a.read("AXN_STATES"), b.read("AXN_STATES")), c
+        for li in range(len(sch.layers)):
+            for name in layer_bufs(sch, li):
+                assert np.array_equal(a.read(name, li), b.read(name, li)), (c, li, name)
+    # not vacuous: layers that cycle on input fired, and with learning on
+    # their strengths moved; a disabled kind stays silent and unlearned (with
+    # SSCs off the pyramidal layers get no input, so only the flags change)
+    for li, lay in enumerate(sch.layers):
+        off = settings & (L.SET_DISABLE_SSCS if lay.spatial else L.SET_DISABLE_PYRS)
+        fired = a.read("SOMA_STATES", li).any()
+        moved = not np.array_equal(a.read("SYN_STRENGTHS", li), st0[li])
+        if off or settings & L.SET_DISABLE_SSCS:
+            assert not fired, (li, settings)
+        else:
+            assert fired, (li, settings)
+            assert moved == (not settings & L.SET_DISABLE_LEARNING), (li, settings)
+
+
+def test_slice_offsets_and_rnd_slots_by_hand():
+    """orc_slice_idz and orc_rnd_slots on a scheme worked by hand: slices of
+    24x24 (scaled input), then 16x16 for the spiny stellate slice and each of
+    the two pyramidal slices, then 230x230 (non-spatial) start at 0, 576, 832,
+    1088, 1344; slots are 2 * tufts + 1 per layer (den[t], control, learning[t])."""
+    sch = recipes.cortical_area("h", 16, input_side=24, pyr_depth=2, ssc_dens=1, ssc_spd=4,
+                                pyr_dens=1, pyr_spd=4)
+    sch.slices.append(recipes.SliceScheme.nonspatial(230, 230))
+    o = P.OracleArea(sch)
+    lib = o.lib
+    import ctypes as C
+    got = []
+    for s in range(5):
+        z = C.c_uint32()
+        assert lib.orc_slice_idz(o.h, s, C.byref(z)) == 0
+        got.append(z.value)
+    assert got == [0, 576, 832, 1088, 1344]
+    z = C.c_uint32()
+    assert lib.orc_slice_idz(o.h, 5, C.byref(z)) != 0
+    assert o.read("AXN_STATES").size == 1344 + 230 * 230
+    assert o.n_slots == (2 * 1 + 1) + (2 * 2 + 1) == 8
