@@ -1852,6 +1852,18 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             for (uint32_t b = 0; b < ntu; b++)
                 items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)a, (uint16_t)b});
     }
+    // Largest items first: workgroups are dispatched in item order, so the
+    // launch ends on the smallest items instead of on a late large one (a
+    // batch mixes items of up to 12x different synapse counts; one item's
+    // stream is latency-bound once few workgroups are left).
+    auto item_syns = [&](const WorkItem& w) {
+        const TuftJob& j = jobs[w.job];
+        const uint64_t tvn = std::min<uint32_t>(j.tile_v, j.V - (uint32_t)w.tv * j.tile_v);
+        const uint64_t tun = std::min<uint32_t>(j.tile_u, j.U - (uint32_t)w.tu * j.tile_u);
+        return (uint64_t)j.depth * tvn * tun * j.spt;
+    };
+    std::stable_sort(items.begin() + st.item0, items.end(),
+                     [&](const WorkItem& x, const WorkItem& y) { return item_syns(x) > item_syns(y); });
     st.n_items = (uint32_t)items.size() - st.item0;
     st.lds_bytes = lds;
     st.W = W;
@@ -1896,6 +1908,15 @@ int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_id
                 for (uint32_t b = 0; b < ntu; b++)
                     items.push_back(WorkItem{(uint16_t)id, (uint16_t)s, (uint16_t)a, (uint16_t)b});
     }
+    // largest items first (cells x synapses per cell, which bounds the learning)
+    auto item_w = [&](const WorkItem& w) {
+        const SscJob& j = jobs[w.job];
+        const uint64_t tvn = std::min<uint32_t>(TV, j.V - (uint32_t)w.tv * TV);
+        const uint64_t tun = std::min<uint32_t>(TU, j.U - (uint32_t)w.tu * TU);
+        return tvn * tun * (j.spt + 16);
+    };
+    std::stable_sort(items.begin() + st.item0, items.end(),
+                     [&](const WorkItem& x, const WorkItem& y) { return item_w(x) > item_w(y); });
     st.n_items = (uint32_t)items.size() - st.item0;
     uint32_t win = (TV + 2 * rmax) * (TU + 2 * rmax);
     st.lds_bytes = ((win + 15) & ~15u) + 7 * TV * TU;  // window | learn list | hex list | flags
@@ -1918,6 +1939,17 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
         if (nb > 65535) return fail("too many column blocks");
         for (uint32_t b = 0; b < nb; b++) items.push_back(WorkItem{(uint16_t)id, 0, (uint16_t)b, 0});
     }
+    // largest items first (columns x slices x distal synapses per cell)
+    auto item_w = [&](const WorkItem& w) {
+        const PyrJob& j = jobs[w.job];
+        const uint64_t cols = std::min<uint32_t>(F3_COLS, j.V * j.U - (uint32_t)w.tv * F3_COLS);
+        uint64_t dst = 16;
+        for (uint32_t t = 0; t < j.n_tufts; t++)
+            if (j.distal_mask >> t & 1u) dst += j.t_spt[t];
+        return cols * j.depth * dst;
+    };
+    std::stable_sort(items.begin() + st.item0, items.end(),
+                     [&](const WorkItem& x, const WorkItem& y) { return item_w(x) > item_w(y); });
     st.n_items = (uint32_t)items.size() - st.item0;
     st.lds_bytes = 0;
     st.grid = std::min<uint32_t>(st.n_items, 65535);
