@@ -1460,6 +1460,9 @@ __global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jo
 namespace bsm {
 
 static constexpr uint32_t LDS_LIMIT = 64 * 1024;
+// LDS per F1 workgroup that lets five share a CU's 160 KiB (the kernel's
+// launch bounds allow five per CU)
+static constexpr uint32_t OCC_LDS = 160 * 1024 / 5;
 
 static uint32_t pick_w(uint64_t a, uint64_t b, uint64_t c, uint32_t max_w) {
     const uint32_t ws[5] = {32, 16, 8, 4, 1};
@@ -1803,6 +1806,44 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
                 j.tile_v = ntv;
                 j.tile_u = ntu;
             }
+        }
+    }
+    // Occupancy: the launch's LDS is the largest window plus the largest sums
+    // and tuft bytes over its jobs, so while that exceeds a fifth of a CU's LDS
+    // (five workgroups per CU, the kernel's launch bounds) the job needing the
+    // most shrinks its tile (rows first), as long as it keeps two chunks per lane.
+    if (fast) {
+        auto launch_lds = [&]() {
+            uint32_t c = 0, a = 0, d = 0;
+            for (uint32_t id : job_ids) {
+                const TuftJob& j = jobs[id];
+                const uint32_t tiles = j.tile_v * j.tile_u;
+                c = std::max(c, stage_bytes(j, j.tile_v, j.tile_u, fast));
+                a = std::max(a, 4 * j.depth * tiles * j.dens);
+                if (j.tft[0]) d = std::max(d, 2 * j.depth * tiles * j.dens + mask_bytes(j, tiles));
+            }
+            return ((c + 15) & ~15u) + a + d;
+        };
+        std::vector<uint8_t> fixed(jobs.size(), 0);
+        while (launch_lds() > OCC_LDS) {
+            uint32_t pick = UINT32_MAX, most = 0;
+            for (uint32_t id : job_ids) {
+                const TuftJob& j = jobs[id];
+                const uint32_t need = job_lds(j, j.tile_v, j.tile_u);
+                if (!fixed[id] && need > most) { most = need; pick = id; }
+            }
+            if (pick == UINT32_MAX) break;
+            TuftJob& j = jobs[pick];
+            const uint32_t per_col = j.depth * (j.spt / std::max(W, 1u));
+            uint32_t ntv = j.tile_v, ntu = j.tile_u;
+            if (ntv > 1) ntv = (ntv + 1) / 2;
+            else if (ntu > 1) ntu = (ntu + 1) / 2;
+            if ((ntv == j.tile_v && ntu == j.tile_u) || per_col * ntv * ntu < 2 * 256) {
+                fixed[pick] = 1;
+                continue;
+            }
+            j.tile_v = ntv;
+            j.tile_u = ntu;
         }
     }
     // the launch's LDS: the largest window (+ row table), sums and tuft bytes
