@@ -314,7 +314,7 @@ def batched_leg(ctx, n_cycles=200, runs=3, tune=True, serial=False):
             "ref_8B_equivalent_frac": batched * REF_BYTES_PER_SYNAPSE / (HBM_PEAK_GBS * 1e9)}
 
 
-def serving_leg(ctx, wall_s=20.0, rate=200.0, capacity=1.0e12):
+def serving_leg(ctx, wall_s=20.0, rate=200.0, capacity=1.0e12, **server_kw):
     """Config 5: Poisson session arrivals, admission by HBM footprint and compute
     budget, continuous batching of frames into shared cycles; p50/p99 per-frame
     latency from frame production at the client."""
@@ -329,7 +329,7 @@ def serving_leg(ctx, wall_s=20.0, rate=200.0, capacity=1.0e12):
             offered += syn[side] * sp.n_frames
     # the estimate starts at 0.6 of the static batch's rate (masked cycles and the
     # loop's host work sustained 0.53-0.56 of it in earlier runs) and may rise to it
-    srv = Server(ctx, capacity=capacity, initial_capacity=0.6 * capacity)
+    srv = Server(ctx, capacity=capacity, initial_capacity=0.6 * capacity, **server_kw)
     st = srv.run(specs, wall_limit_s=wall_s)
     out = st.summary()
     out.update({"arrival_rate_sessions_per_s": rate,

@@ -131,6 +131,7 @@ SIGNATURES = [
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_int)]),
     ("bsm_sampler_set_backpressure", C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    ("bsm_sampler_set_buffers", C.c_int, [_P, C.c_uint32]),
     ("bsm_sampler_info", C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]),
     ("bsm_sampler_destroy", C.c_int, [_P]),

@@ -135,6 +135,18 @@ struct bsm_batch {
     bsm::SscJob* sjobs_d = nullptr;
     bsm::PyrJob* pjobs_d = nullptr;
     bsm::WorkItem* items_d = nullptr;
+    // host copies for masked cycles: every stage's work items and, per job of
+    // each kind (F1 tufts, F2, F3), its area
+    std::vector<bsm::WorkItem> items_h;
+    std::vector<uint32_t> job_area[3];
+    // per stage, its items grouped by area: the areas in the order their first
+    // (largest) item comes in the stage, each area's items in stage order
+    struct StageAreas {
+        std::vector<uint32_t> order;          // areas with items in the stage
+        std::vector<uint32_t> first, count;   // per area: its run in `idx`
+        std::vector<uint32_t> idx;            // item indices, grouped by area
+    };
+    std::vector<StageAreas> by_area;
     bsm::AreaIo* io_d = nullptr;
     uint32_t* rng_d = nullptr;      // 4 words per area
     uint32_t* counter_d = nullptr;  // frame counters, one per cycle parity
@@ -191,6 +203,7 @@ static int batch_plan(bsm_batch* b) {
     b->pjobs.clear();
     b->stages.clear();
     b->epochs.clear();
+    for (auto& v : b->job_area) v.clear();
     for (bsm_area* a : b->areas) b->epochs.push_back(a->layout_epoch);
     std::vector<WorkItem> items;
     const uint32_t na = (uint32_t)b->areas.size();
@@ -240,6 +253,7 @@ static int batch_plan(bsm_batch* b) {
                         (side ? side_ids : t_ids).push_back((uint32_t)b->tjobs.size());
                         (side ? side_w : t_w).push_back(tuft_chunk(a, l, t, b->tune.chunk));
                         b->tjobs.push_back(j);
+                        b->job_area[0].push_back(ai);
                     }
                     if (spatial) {
                         SscJob j = make_ssc_job(a, l, a->rnd_d);
@@ -247,6 +261,7 @@ static int batch_plan(bsm_batch* b) {
                         j.dyn = dw;
                         s_ids.push_back((uint32_t)b->sjobs.size());
                         b->sjobs.push_back(j);
+                        b->job_area[1].push_back(ai);
                     } else {
                         uint32_t w;
                         PyrJob j = make_pyr_job(a, l, a->rnd_d, &w);
@@ -255,6 +270,7 @@ static int batch_plan(bsm_batch* b) {
                         w3 = std::min(w3, w);
                         p_ids.push_back((uint32_t)b->pjobs.size());
                         b->pjobs.push_back(j);
+                        b->job_area[2].push_back(ai);
                     }
                 }
             }
@@ -317,13 +333,42 @@ static int batch_plan(bsm_batch* b) {
         up((void**)&b->pjobs_d, b->cap_p, b->pjobs.data(), b->pjobs.size() * sizeof(PyrJob)) ||
         up((void**)&b->items_d, b->cap_i, items.data(), items.size() * sizeof(WorkItem)))
         return 1;
+    b->items_h.swap(items);
+    // masked cycles gather the active areas' items per stage from these groups
+    b->by_area.assign(b->stages.size(), {});
+    for (size_t si = 0; si < b->stages.size(); si++) {
+        const Stage& st = b->stages[si];
+        const std::vector<uint32_t>& ja = b->job_area[st.kind == 1 ? 0 : (st.kind == 2 ? 1 : 2)];
+        auto& g = b->by_area[si];
+        g.first.assign(na, 0);
+        g.count.assign(na, 0);
+        for (uint32_t i = st.item0; i < st.item0 + st.n_items; i++) {
+            const uint32_t ai = ja[b->items_h[i].job];
+            if (g.count[ai]++ == 0) g.order.push_back(ai);
+        }
+        uint32_t at = 0;
+        for (uint32_t ai : g.order) {
+            g.first[ai] = at;
+            at += g.count[ai];
+        }
+        g.idx.resize(at);
+        std::vector<uint32_t> fill(na, 0);
+        for (uint32_t i = st.item0; i < st.item0 + st.n_items; i++) {
+            const uint32_t ai = ja[b->items_h[i].job];
+            g.idx[g.first[ai] + fill[ai]++] = i;
+        }
+    }
     b->dirty = false;
     return 0;
 }
 
 // One batched cycle on the batch stream.  With ev, events ev[2i], ev[2i+1]
 // bracket stage i (never while capturing).
-static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
+static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr,
+                         const std::vector<Stage>* stg = nullptr,
+                         const WorkItem* items_d = nullptr) {
+    const std::vector<Stage>& stages = stg ? *stg : b->stages;
+    const WorkItem* items = items_d ? items_d : b->items_d;
     const uint32_t na = (uint32_t)b->areas.size();
     if (b->n_tracts) {
         kb_tracts<<<dim3(b->n_tracts, b->tract_parts), 256, 0, b->stream>>>(b->tracts_d);
@@ -333,17 +378,17 @@ static int enqueue_cycle(bsm_batch* b, uint32_t par, hipEvent_t* ev = nullptr) {
     BSM_HIP(hipGetLastError());
     if (ev) {
         // profiling: one stream, stages in list order, each launch timed alone
-        for (size_t i = 0; i < b->stages.size(); i++) {
+        for (size_t i = 0; i < stages.size(); i++) {
             BSM_HIP(hipEventRecord(ev[2 * i], b->stream));
-            if (launch_stage(b->stream, b->stages[i], b->tjobs_d, b->sjobs_d, b->pjobs_d,
-                             b->items_d, par, 0))
+            if (launch_stage(b->stream, stages[i], b->tjobs_d, b->sjobs_d, b->pjobs_d,
+                             items, par, 0))
                 return 1;
             BSM_HIP(hipEventRecord(ev[2 * i + 1], b->stream));
         }
-    } else if (enqueue_stages(b->stream, b->side, b->stages, false,
+    } else if (enqueue_stages(b->stream, b->side, stages, false,
                               [&](const Stage& st, hipStream_t s) {
                                   return launch_stage(s, st, b->tjobs_d, b->sjobs_d, b->pjobs_d,
-                                                      b->items_d, par, 0);
+                                                      items, par, 0);
                               })) {
         return 1;
     }
@@ -374,7 +419,8 @@ static bool layouts_changed(const bsm_batch* b) {
     return false;
 }
 
-static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = nullptr) {
+static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = nullptr,
+                      const std::vector<Stage>* stg = nullptr, const WorkItem* items_d = nullptr) {
     if (b->areas.empty()) return 0;   // a dynamic batch between its last and next member
     if (!b->dirty && layouts_changed(b)) {
         // a layout write may have changed which gather path is exact: re-plan
@@ -389,7 +435,8 @@ static int run_cycles(bsm_batch* b, uint32_t n, int use_graph, hipEvent_t* ev = 
         b->bpar ^= 1u;
         if (use_graph) {
             BSM_HIP(hipGraphLaunch(b->exec[b->bpar], b->stream));
-        } else if (enqueue_cycle(b, b->bpar, ev ? ev + (size_t)2 * c * b->stages.size() : nullptr)) {
+        } else if (enqueue_cycle(b, b->bpar, ev ? ev + (size_t)2 * c * b->stages.size() : nullptr,
+                                 stg, items_d)) {
             return 1;
         }
         // dynamic batches keep parities on the device (read back at sync) and
@@ -850,10 +897,37 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
         }
     }
     if (n == 0) return 0;
-    if (grow_xfer(b, frame_bytes)) return 1;
+    if (!b->dirty && layouts_changed(b)) {
+        // as in run_cycles: a layout write may have changed which gather path is exact
+        BSM_HIP(hipStreamSynchronize(b->stream));
+        if (bsm_batch_sync(b)) return 1;
+        b->dirty = true;
+    }
     if (b->dirty && batch_plan(b)) return 1;
     if (use_graph && !b->exec[0] && capture(b)) return 1;
-    const size_t hdr = xfer_hdr(n), total = hdr + frame_bytes;
+    // Without a graph the cycle launches only the active areas' work items: the
+    // transfer carries each stage's items of active areas (in the plan's order,
+    // largest first), so a cycle's launches scale with the areas it steps.
+    const bool compact = !use_graph;
+    std::vector<Stage> cst;
+    size_t n_items = 0;
+    if (compact) {
+        cst = b->stages;
+        for (size_t si = 0; si < cst.size(); si++) {
+            const auto& g = b->by_area[si];
+            uint32_t m = 0;
+            for (uint32_t ai : g.order) m += active[ai] ? g.count[ai] : 0u;
+            Stage& st = cst[si];
+            st.item0 = (uint32_t)n_items;
+            st.n_items = m;
+            st.grid = std::min<uint32_t>(m, 65535);
+            n_items += m;
+        }
+    }
+    const size_t hdr = xfer_hdr(n);
+    const size_t items_at = (hdr + frame_bytes + 15) & ~(size_t)15;
+    const size_t total = compact ? items_at + n_items * sizeof(WorkItem) : hdr + frame_bytes;
+    if (grow_xfer(b, total - hdr)) return 1;
     const int k = b->xfer_next;
     b->xfer_next ^= 1;
     BSM_HIP(hipEventSynchronize(b->xfer_ev[k]));
@@ -870,10 +944,25 @@ int bsm_batch_cycle_masked(bsm_batch* b, const uint8_t* active, const uint8_t* f
     uint64_t* fo = (uint64_t*)(h + ((n + 15) & ~(size_t)15));
     for (size_t i = 0; i < n; i++) fo[i] = frame_ofs ? frame_ofs[i] : ~0ull;
     if (frame_bytes) std::memcpy(h + hdr, frames, frame_bytes);
+    if (compact) {
+        WorkItem* out = (WorkItem*)(h + items_at);
+        size_t q = 0;
+        for (size_t si = 0; si < cst.size(); si++) {
+            const auto& g = b->by_area[si];
+            for (uint32_t ai : g.order) {
+                if (!active[ai]) continue;
+                const uint32_t* ix = g.idx.data() + g.first[ai];
+                for (uint32_t j = 0; j < g.count[ai]; j++) out[q++] = b->items_h[ix[j]];
+            }
+        }
+        if (q != n_items) return fail("masked cycle: work item count changed");
+    }
     BSM_HIP(hipEventRecord(b->busy_ev[k][0], b->stream));
     BSM_HIP(hipMemcpyAsync(b->xfer_d, h, total, hipMemcpyHostToDevice, b->stream));
     BSM_HIP(hipEventRecord(b->xfer_ev[k], b->stream));
-    if (run_cycles(b, 1, use_graph)) return 1;
+    if (run_cycles(b, 1, use_graph, nullptr, compact ? &cst : nullptr,
+                   compact ? (const WorkItem*)(b->xfer_d + items_at) : nullptr))
+        return 1;
     if (sample_after_cycle(b->sample_plan, b->stream, b->areas.data(), n, active, b->dyn_d))
         return 1;
     BSM_HIP(hipEventRecord(b->busy_ev[k][1], b->stream));

@@ -1705,6 +1705,31 @@ static uint32_t stage_bytes(const TuftJob& j, uint32_t TV, uint32_t TU, bool fas
     return tot;
 }
 
+// Reorders items[from, end) by decreasing weight (weight(item)), keeping the
+// order of equal weights: a counting sort over the distinct weights, linear in
+// the items (a batch plans ~10^5 items each time its membership changes).
+template <class Weight>
+static void largest_first(std::vector<WorkItem>& items, size_t from, Weight&& weight) {
+    const size_t n = items.size() - from;
+    if (n < 2) return;
+    std::vector<uint64_t> w(n);
+    for (size_t i = 0; i < n; i++) w[i] = weight(items[from + i]);
+    std::vector<uint64_t> keys(w);
+    std::sort(keys.begin(), keys.end(), std::greater<uint64_t>());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    if (keys.size() == 1) return;
+    std::vector<uint32_t> rank(n), start(keys.size() + 1, 0);
+    for (size_t i = 0; i < n; i++) {
+        rank[i] = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), w[i],
+                                              std::greater<uint64_t>()) - keys.begin());
+        start[rank[i] + 1]++;
+    }
+    for (size_t k = 0; k < keys.size(); k++) start[k + 1] += start[k];
+    std::vector<WorkItem> out(n);
+    for (size_t i = 0; i < n; i++) out[start[rank[i]]++] = items[from + i];
+    std::copy(out.begin(), out.end(), items.begin() + from);
+}
+
 int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
             const std::vector<uint32_t>& job_w, const Tuning& tn, std::vector<WorkItem>& items,
             Stage& st) {
@@ -1903,8 +1928,7 @@ int plan_f1(std::vector<TuftJob>& jobs, const std::vector<uint32_t>& job_ids,
         const uint64_t tun = std::min<uint32_t>(j.tile_u, j.U - (uint32_t)w.tu * j.tile_u);
         return (uint64_t)j.depth * tvn * tun * j.spt;
     };
-    std::stable_sort(items.begin() + st.item0, items.end(),
-                     [&](const WorkItem& x, const WorkItem& y) { return item_syns(x) > item_syns(y); });
+    largest_first(items, st.item0, item_syns);
     st.n_items = (uint32_t)items.size() - st.item0;
     st.lds_bytes = lds;
     st.W = W;
@@ -1956,8 +1980,7 @@ int plan_f2(const std::vector<SscJob>& jobs, const std::vector<uint32_t>& job_id
         const uint64_t tun = std::min<uint32_t>(TU, j.U - (uint32_t)w.tu * TU);
         return tvn * tun * (j.spt + 16);
     };
-    std::stable_sort(items.begin() + st.item0, items.end(),
-                     [&](const WorkItem& x, const WorkItem& y) { return item_w(x) > item_w(y); });
+    largest_first(items, st.item0, item_w);
     st.n_items = (uint32_t)items.size() - st.item0;
     uint32_t win = (TV + 2 * rmax) * (TU + 2 * rmax);
     st.lds_bytes = ((win + 15) & ~15u) + 7 * TV * TU;  // window | learn list | hex list | flags
@@ -1989,8 +2012,7 @@ int plan_f3(const std::vector<PyrJob>& jobs, const std::vector<uint32_t>& job_id
             if (j.distal_mask >> t & 1u) dst += j.t_spt[t];
         return cols * j.depth * dst;
     };
-    std::stable_sort(items.begin() + st.item0, items.end(),
-                     [&](const WorkItem& x, const WorkItem& y) { return item_w(x) > item_w(y); });
+    largest_first(items, st.item0, item_w);
     st.n_items = (uint32_t)items.size() - st.item0;
     st.lds_bytes = 0;
     st.grid = std::min<uint32_t>(st.n_items, 65535);

@@ -11,6 +11,7 @@
 // frames of every session that steps in a cycle with one call.
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "bsm_internal.h"
@@ -52,6 +53,24 @@ static void sdr_advance(bsm_sdr* s) {
         s->active[i] = c;
     }
     for (uint32_t c : s->drop) s->member[c] = 0;
+}
+
+// Runs f(i) for i in [0, n), split over up to 8 threads when n is large (the
+// streams are independent, so the frames do not depend on the split).
+template <class F>
+static void for_streams(uint32_t n, F&& f) {
+    const uint32_t nt = n >= 64 ? std::min<uint32_t>(8, n / 16) : 1;
+    if (nt <= 1) {
+        for (uint32_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (uint32_t w = 1; w < nt; w++)
+        pool.emplace_back([&, w] {
+            for (uint32_t i = n * w / nt; i < n * (w + 1) / nt; i++) f(i);
+        });
+    for (uint32_t i = 0; i < n / nt; i++) f(i);
+    for (std::thread& th : pool) th.join();
 }
 
 }  // namespace bsm
@@ -119,7 +138,13 @@ int bsm_sdr_frames(bsm_sdr* s, uint32_t n_frames, uint8_t* frames) {
 int bsm_sdr_next_packed(bsm_sdr* const* s, uint32_t n, uint8_t* buf, const uint64_t* ofs) {
     if ((!s || !buf || !ofs) && n) return fail("null arg");
     for (uint32_t i = 0; i < n; i++)
-        if (bsm_sdr_next(s[i], buf + ofs[i])) return 1;
+        if (!s[i]) return fail("null stream");
+    for_streams(n, [&](uint32_t i) {
+        uint8_t* frame = buf + ofs[i];
+        std::memset(frame, 0, s[i]->n);
+        for (uint32_t c : s[i]->active) frame[c] = 255;
+        sdr_advance(s[i]);
+    });
     return 0;
 }
 
@@ -133,11 +158,15 @@ int bsm_sdr_next_indices(bsm_sdr* s, uint32_t* idx) {
 int bsm_sdr_next_packed_sdr(bsm_sdr* const* s, uint32_t n, uint8_t* buf, const uint64_t* ofs) {
     if ((!s || !buf || !ofs) && n) return fail("null arg");
     for (uint32_t i = 0; i < n; i++) {
+        if (!s[i]) return fail("null stream");
         if (ofs[i] & 3) return fail("SDR entry offsets must be 4-byte aligned");
+    }
+    for_streams(n, [&](uint32_t i) {
         uint32_t* e = (uint32_t*)(buf + ofs[i]);
         e[0] = s[i]->k;
-        if (bsm_sdr_next_indices(s[i], e + 1)) return 1;
-    }
+        std::memcpy(e + 1, s[i]->active.data(), (size_t)s[i]->k * sizeof(uint32_t));
+        sdr_advance(s[i]);
+    });
     return 0;
 }
 

@@ -9,7 +9,11 @@
 // composing the previous-active flag bit on the device -- and one copy moves the
 // packed frames into a pinned host block.  The block, with the event that marks
 // it complete, is shared by the frames it holds and returns to a per-context
-// pool when the last of them has been received or dropped.
+// pool when the last of them has been received or dropped.  A sampler holds up
+// to 1, 2 or 3 frames in order (the reference's SamplerBufferKind Single,
+// Double, Triple, src/cortex/cortical_area.rs:52-58, of which it implements
+// Single): a frame is sent when a slot is free; otherwise it is skipped, or,
+// with backpressure, the cycle waits for the receiver.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
@@ -99,6 +103,15 @@ struct SamplePlan {
 };
 
 enum SampleState { SMP_EMPTY = 0, SMP_FILLING = 1, SMP_PENDING = 2 };
+constexpr uint32_t SMP_SLOTS = 3;           // frames a sampler holds at most
+
+// One held frame: being filled (its gather and copy enqueued) or pending.
+struct SampleSlot {
+    int state = SMP_EMPTY;
+    uint64_t cycle = 0;
+    std::shared_ptr<FrameBlock> blk;        // the block holding the frame
+    size_t off = 0;
+};
 
 }  // namespace bsm
 
@@ -109,12 +122,22 @@ struct bsm_sampler {
     uint64_t offset, len, bytes;          // bytes: len rounded up to the staging alignment
     std::mutex m;
     std::condition_variable cv;
-    int state = bsm::SMP_EMPTY;
     bool backpressure = false;
     bool closed = false;
-    uint64_t frame_cycle = 0;
-    std::shared_ptr<bsm::FrameBlock> blk;   // the block holding the pending frame
-    size_t blk_off = 0;
+    uint32_t depth = 1;                     // frames held at most (1..SMP_SLOTS)
+    uint32_t head = 0, count = 0;           // oldest held frame, frames held
+    bsm::SampleSlot slot[bsm::SMP_SLOTS];
+    bsm::SampleSlot& oldest() { return slot[head]; }
+    bool has_frame() { return count > 0 && slot[head].state == bsm::SMP_PENDING; }
+    void pop() {
+        slot[head] = bsm::SampleSlot();
+        head = (head + 1) % bsm::SMP_SLOTS;
+        count--;
+    }
+    void clear() {
+        for (auto& x : slot) x = bsm::SampleSlot();
+        head = count = 0;
+    }
 };
 
 namespace bsm {
@@ -262,13 +285,15 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
         bsm_sampler* sp = P->smp[k];
         std::unique_lock<std::mutex> lk(sp->m);
         if (sp->closed) continue;
-        if (sp->state != SMP_EMPTY) {
-            if (!sp->backpressure) continue;   // frame skipped: the receiver keeps the older one
-            sp->cv.wait(lk, [&] { return sp->state == SMP_EMPTY || sp->closed || !sp->backpressure; });
-            if (sp->closed || sp->state != SMP_EMPTY) continue;
+        if (sp->count >= sp->depth) {
+            if (!sp->backpressure) continue;   // frame skipped: the receiver keeps the older ones
+            sp->cv.wait(lk, [&] { return sp->count < sp->depth || sp->closed || !sp->backpressure; });
+            if (sp->closed || sp->count >= sp->depth) continue;
         }
-        sp->state = SMP_FILLING;
-        sp->frame_cycle = areas[i]->cycles_done;
+        SampleSlot& x = sp->slot[(sp->head + sp->count) % SMP_SLOTS];
+        sp->count++;
+        x.state = SMP_FILLING;
+        x.cycle = areas[i]->cycles_done;
         inc.push_back(k);
         at.push_back(total);
         total += sp->bytes;
@@ -278,7 +303,11 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
         for (uint32_t k : inc) {
             bsm_sampler* sp = P->smp[k];
             std::lock_guard<std::mutex> g(sp->m);
-            sp->state = SMP_EMPTY;
+            if (!sp->closed && sp->count) {
+                // the frame reserved last is the one that will not come
+                sp->slot[(sp->head + sp->count - 1) % SMP_SLOTS] = SampleSlot();
+                sp->count--;
+            }
             sp->cv.notify_all();
         }
     };
@@ -327,9 +356,11 @@ int sample_after_cycle(SamplePlan*& P, hipStream_t s, bsm_area* const* areas, si
     for (size_t q = 0; q < inc.size(); q++) {
         bsm_sampler* sp = P->smp[inc[q]];
         std::lock_guard<std::mutex> g(sp->m);
-        sp->blk = blk;
-        sp->blk_off = at[q];
-        sp->state = SMP_PENDING;
+        if (sp->closed || !sp->count) continue;
+        SampleSlot& x = sp->slot[(sp->head + sp->count - 1) % SMP_SLOTS];
+        x.blk = blk;
+        x.off = at[q];
+        x.state = SMP_PENDING;
         sp->cv.notify_all();
     }
     return 0;
@@ -349,8 +380,7 @@ static void sampler_close(bsm_sampler* sp) {
         std::lock_guard<std::mutex> g(sp->m);
         if (sp->closed) return;
         sp->closed = true;
-        sp->state = SMP_EMPTY;   // a pending frame ends unreceived
-        sp->blk.reset();
+        sp->clear();             // pending frames end unreceived
         sp->cv.notify_all();
     }
     bsm_area* a = sp->area;
@@ -413,24 +443,22 @@ int bsm_sampler_recv(bsm_sampler* sp, int wait_for_frame, void* dst, uint64_t ds
     {
         std::unique_lock<std::mutex> lk(sp->m);
         if (sp->closed) return 0;
-        if (sp->state != SMP_PENDING) {
+        if (!sp->has_frame()) {
             if (!wait_for_frame) return 0;
-            sp->cv.wait(lk, [&] { return sp->state == SMP_PENDING || sp->closed; });
-            if (sp->closed || sp->state != SMP_PENDING) return 0;
+            sp->cv.wait(lk, [&] { return sp->has_frame() || sp->closed; });
+            if (sp->closed || !sp->has_frame()) return 0;
         }
-        blk = sp->blk;
-        off = sp->blk_off;
-        fc = sp->frame_cycle;
+        blk = sp->oldest().blk;
+        off = sp->oldest().off;
+        fc = sp->oldest().cycle;
     }
     hipError_t e = hipEventSynchronize(blk->ev);
     if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize (sampler frame)");
     std::memcpy(dst, blk->host + off, sp->len);
     {
         std::lock_guard<std::mutex> g(sp->m);
-        if (!sp->closed) {
-            sp->state = SMP_EMPTY;
-            sp->blk.reset();
-        }
+        if (!sp->closed && sp->count && sp->oldest().blk == blk && sp->oldest().off == off)
+            sp->pop();
         sp->cv.notify_all();
     }
     if (cycle) *cycle = fc;
@@ -458,12 +486,12 @@ int bsm_sampler_recv_many(bsm_sampler* const* s, uint32_t n, int wait_for_frame,
         if (dst_len[i] < sp->len || !dst[i]) return fail("destination shorter than the sampler range");
         std::unique_lock<std::mutex> lk(sp->m);
         if (sp->closed) continue;
-        if (sp->state != SMP_PENDING) {
+        if (!sp->has_frame()) {
             if (!wait_for_frame) continue;
-            sp->cv.wait(lk, [&] { return sp->state == SMP_PENDING || sp->closed; });
-            if (sp->closed || sp->state != SMP_PENDING) continue;
+            sp->cv.wait(lk, [&] { return sp->has_frame() || sp->closed; });
+            if (sp->closed || !sp->has_frame()) continue;
         }
-        takes.push_back(Take{i, sp->blk, sp->blk_off, sp->frame_cycle});
+        takes.push_back(Take{i, sp->oldest().blk, sp->oldest().off, sp->oldest().cycle});
         bytes += sp->len;
     }
     const FrameBlock* last = nullptr;
@@ -490,10 +518,8 @@ int bsm_sampler_recv_many(bsm_sampler* const* s, uint32_t n, int wait_for_frame,
         bsm_sampler* sp = s[t.i];
         {
             std::lock_guard<std::mutex> g(sp->m);
-            if (!sp->closed) {
-                sp->state = SMP_EMPTY;
-                sp->blk.reset();
-            }
+            if (!sp->closed && sp->count && sp->oldest().blk == t.blk && sp->oldest().off == t.off)
+                sp->pop();
             sp->cv.notify_all();
         }
         if (cycles) cycles[t.i] = t.fc;
@@ -511,12 +537,21 @@ int bsm_sampler_set_backpressure(bsm_sampler* sp, int backpressure, int* prior) 
     return 0;
 }
 
+int bsm_sampler_set_buffers(bsm_sampler* sp, uint32_t n_frames) {
+    if (!sp) return fail("null sampler");
+    if (n_frames < 1 || n_frames > SMP_SLOTS) return fail("a sampler holds 1 to 3 frames");
+    std::lock_guard<std::mutex> g(sp->m);
+    sp->depth = n_frames;
+    sp->cv.notify_all();
+    return 0;
+}
+
 int bsm_sampler_info(bsm_sampler* sp, uint64_t* len, int* backpressure, int* pending) {
     if (!sp) return fail("null sampler");
     std::lock_guard<std::mutex> g(sp->m);
     if (len) *len = sp->len;
     if (backpressure) *backpressure = sp->backpressure ? 1 : 0;
-    if (pending) *pending = sp->state == SMP_PENDING ? 1 : 0;
+    if (pending) *pending = sp->has_frame() ? 1 : 0;
     return 0;
 }
 

@@ -21,8 +21,10 @@ from . import _lib as L
 
 
 class SamplerBufferKind(Enum):
-    """Only `SINGLE` is implemented, as upstream (src/cortex/cortical_area.rs:
-    1038-1048 leaves the other kinds unimplemented)."""
+    """Frames a sampler holds: `SINGLE` (the reference's only implemented kind,
+    src/cortex/cortical_area.rs:1038-1048), `DOUBLE` and `TRIPLE` (declared
+    upstream, :52-58, unimplemented there): a cycle's frame is sent while fewer
+    are held and received oldest first.  `NONE` and `MAP` are not implemented."""
     NONE = 0
     MAP = 1
     SINGLE = 2
@@ -177,12 +179,34 @@ class ReceiverSet:
         self.cycles = np.zeros(n, dtype=np.uint64)
         self.got = np.zeros(n, dtype=np.int32)
 
-    def recv(self, wait_for_frame=False):
+    def recv(self, wait_for_frame=False, idx=None):
         """Receives into self.outs; self.got[i] / self.cycles[i] say which got a
         frame and its cycle.  Without wait_for_frame, receivers with nothing
-        pending or in flight get none; frames in flight are waited for."""
+        pending or in flight get none; frames in flight are waited for.  idx
+        (an integer array) limits the call to those receivers; the others keep
+        their got / cycles entries."""
         if not self.rxs:
             return self.got
+        lib = self.rxs[0].lib
+        if idx is None:
+            hs, ds, ls, cyc, got = self._hs, self._ds, self._ls, self.cycles, self.got
+        else:
+            idx = np.asarray(idx, dtype=np.int64)
+            if idx.size == 0:
+                return self.got
+            hs, ds, ls = self._hs[idx], self._ds[idx], self._ls[idx]
+            cyc = np.zeros(idx.size, dtype=np.uint64)
+            got = np.zeros(idx.size, dtype=np.int32)
+        L.check(lib.bsm_sampler_recv_many(
+            C.cast(L.vptr(hs), C.POINTER(C.c_void_p)), len(hs),
+            1 if wait_for_frame else 0, C.cast(L.vptr(ds), C.POINTER(C.c_void_p)),
+            C.cast(L.vptr(ls), C.POINTER(C.c_uint64)),
+            C.cast(L.vptr(cyc), C.POINTER(C.c_uint64)),
+            C.cast(L.vptr(got), C.POINTER(C.c_int))))
+        if idx is not None:
+            self.cycles[idx] = cyc
+            self.got[idx] = got
+        return self.got
         lib = self.rxs[0].lib
         L.check(lib.bsm_sampler_recv_many(
             C.cast(L.vptr(self._hs), C.POINTER(C.c_void_p)), len(self.rxs),
@@ -195,8 +219,9 @@ class ReceiverSet:
 
 def create(area, kind: SamplerKind, buffer_kind=SamplerBufferKind.SINGLE, backpressure=False):
     """A sampler on `area` (CorticalArea::sampler, src/cortex/cortical_area.rs:1017-1284)."""
-    if buffer_kind != SamplerBufferKind.SINGLE:
-        raise NotImplementedError(f"sampler buffer kind {buffer_kind} (only SINGLE)")
+    held = {SamplerBufferKind.SINGLE: 1, SamplerBufferKind.DOUBLE: 2, SamplerBufferKind.TRIPLE: 3}
+    if buffer_kind not in held:
+        raise NotImplementedError(f"sampler buffer kind {buffer_kind} (SINGLE, DOUBLE or TRIPLE)")
     buf = kind.buf
     if buf == "AXN_STATES":
         if kind.layer is None:
@@ -210,5 +235,10 @@ def create(area, kind: SamplerKind, buffer_kind=SamplerBufferKind.SINGLE, backpr
     h = C.c_void_p()
     L.check(area.lib.bsm_sampler_create(area.h, L.BUF[buf], li, off, n, 1 if backpressure else 0,
                                         C.byref(h)))
+    if held[buffer_kind] > 1:
+        rc = area.lib.bsm_sampler_set_buffers(h, held[buffer_kind])
+        if rc:
+            area.lib.bsm_sampler_destroy(h)
+            L.check(rc)
     dt = np.int8 if buf in L.SIGNED_BUFS else np.uint8
     return TractReceiver(area, h, n, dt, (off, off + n))

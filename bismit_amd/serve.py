@@ -46,7 +46,7 @@ from . import _lib as L
 from . import recipes
 from .cortex import Batch, CorticalArea, footprint
 from .inputs import SdrStream, next_packed
-from .sampler import Frame, ReceiverSet, SamplerKind
+from .sampler import Frame, ReceiverSet, SamplerBufferKind, SamplerKind
 
 NO_FRAME = np.uint64(2**64 - 1)
 
@@ -112,7 +112,7 @@ class Session:
         self.done = 0                      # frames completed
         self.synapses = sum(sc.n_synapses() for sc in self.schemes)
         # per-cycle output readback (no backpressure: a slow consumer skips frames)
-        self.outputs = [a.sampler(SamplerKind.Axons(output_layer(sc)))
+        self.outputs = [a.sampler(SamplerKind.Axons(output_layer(sc)), SamplerBufferKind.DOUBLE)
                         for a, sc in zip(self.areas, self.schemes)] if sample_outputs else []
         self.last_outputs = [None] * len(self.areas)
         self.pos = np.zeros(0, np.int64)   # the areas' positions in the batch
@@ -215,7 +215,8 @@ class Server:
     def __init__(self, ctx, hbm_budget_bytes=None, max_sessions=None, epoch_s=0.05,
                  use_graph=False, on_retire=None, setup_workers=4, regrow_every=201,
                  regrow_reach=6, sample_outputs=True, on_output=None, max_wait_s=0.25,
-                 capacity=1.0e12, target_util=0.8, max_lag_s=0.1, initial_capacity=None):
+                 capacity=1.0e12, target_util=0.8, max_lag_s=0.1, initial_capacity=None,
+                 cycle_gap_s=0.002):
         self.ctx = ctx
         # Masked cycles are enqueued directly by default: the batch changes with
         # most admission epochs, and capturing plus instantiating its two graphs
@@ -233,6 +234,10 @@ class Server:
         self.budget = hbm_budget_bytes
         self.max_sessions = max_sessions      # None: HBM and compute decide
         self.epoch_s = epoch_s
+        # shared cycles start at least this far apart: frames gather into larger
+        # cycles (each costs the host a fixed ~0.5-1 ms to build and launch), at
+        # a latency cost of at most the gap
+        self.cycle_gap_s = cycle_gap_s
         self.max_wait_s = max_wait_s          # None: sessions wait as long as it takes
         self.capacity = capacity              # synapse-updates/s the GPU sustains at most
         self.initial_capacity = initial_capacity or capacity
@@ -393,6 +398,9 @@ class Server:
         return c
 
     def _enqueue(self, c):
+        """Enqueues the cycle, then the regrowth it makes due: a session regrows
+        after every `regrow_every` of its cycles, so the regrowth follows this
+        cycle on the batch stream, ahead of any later cycle."""
         n = len(self.batch.areas)
         mask = np.zeros(n, np.uint8)
         fo = np.full(n, NO_FRAME, np.uint64)
@@ -400,35 +408,9 @@ class Server:
         mask[pos] = 1
         fo[pos] = c.ofs | np.uint64(L.FRAME_SDR)
         self.batch.cycle_masked(mask, self._buf[:c.total], fo, graph=self.use_graph)
-
-    def _receive(self, c, clock):
-        """Waits for the cycle in flight and receives its output frames; counts
-        the cycle for its sessions and enqueues the regrowth it makes due (on the
-        batch stream, ahead of the next cycle)."""
-        rs = self.recvset
-        t0 = clock()
-        while not self.batch.query():      # the cycle's completion, polled
-            time.sleep(0)
-        self._stats.phase("receive_gpu_wait", clock() - t0)
-        got = rs.recv(wait_for_frame=False)
-        if not self.sample_outputs:
-            self.batch.sync()
-        t = clock()
-        # the frames, looked up before a membership change rebuilds the receiver set
-        frames = None
-        if self.sample_outputs:
-            for s in c.sessions:
-                if not got[s.rx].all():
-                    raise RuntimeError(f"session {s.spec.sid}: output frame missing")
-            if self.on_output:
-                frames = [[Frame(rs.outs[i], int(rs.cycles[i])) for i in s.rx]
-                          for s in c.sessions]
-        for s in c.sessions:
-            s.done += 1
         if self.regrow_every:
-            due = [s for s in c.sessions if s.done % self.regrow_every == 0]
+            due = [s for s in c.sessions if s.next % self.regrow_every == 0]
             if due:
-                n = len(self.batch.areas)
                 which = np.zeros(n, np.uint8)
                 seeds = [0] * n
                 epochs = [0] * n
@@ -436,9 +418,52 @@ class Server:
                     for k, p in enumerate(s.pos):
                         which[p] = 1
                         seeds[p] = s.regrow_seed(k)
-                        epochs[p] = s.done // self.regrow_every - 1
+                        epochs[p] = s.next // self.regrow_every - 1
                 self.batch.regrow(seeds, epochs, self.regrow_reach, which)
+
+    def _receive(self, c, clock):
+        """Waits for the cycle and receives its sessions' output frames (the
+        oldest each sampler holds; a later cycle may already be running), and
+        counts the cycle for its sessions."""
+        rs = self.recvset
+        t0 = clock()
+        frames = None
+        if self.sample_outputs:
+            idx = np.concatenate([s.rx for s in c.sessions])
+            got = rs.recv(wait_for_frame=True, idx=idx)
+            t = clock()
+            self._stats.phase("receive_frames", t - t0)
+            for s in c.sessions:
+                if not got[s.rx].all() or (rs.cycles[s.rx] != s.done + 1).any():
+                    raise RuntimeError(f"session {s.spec.sid}: output frame missing")
+            if self.on_output:
+                frames = [[Frame(rs.outs[i], int(rs.cycles[i])) for i in s.rx]
+                          for s in c.sessions]
+        else:
+            self.batch.sync()
+            t = clock()
+            self._stats.phase("receive_frames", t - t0)
+        for s in c.sessions:
+            s.done += 1
         return frames, t
+
+    def _membership_due(self, setting_up):
+        """A session's set-up ended (it waits to join, or raised), or a resident
+        session has received its last frame."""
+        with self._ready_lock:
+            if self._ready:
+                return True
+        return any(f.done() for f in setting_up) or any(s.finished() for s in self.resident)
+
+    def _finish(self, c, clock, st):
+        """Receives a cycle in flight and accounts it."""
+        t0 = clock()
+        frames, t = self._receive(c, clock)
+        st.phase("receive", clock() - t0)
+        t1 = clock()
+        self._account(c, frames, t, st)
+        self._observe(st, clock())
+        st.phase("account", clock() - t1)
 
     def _account(self, c, frames, t, st):
         """Hands the received frames out and records the cycle's latencies (the
@@ -480,7 +505,8 @@ class Server:
         clock = lambda: time.perf_counter() - t_start
         last_epoch = -1e9
         t_last = 0.0
-        inflight = None
+        inflight = deque()
+        t_enq = -1.0
         while True:
             now = clock()
             st.queue_area += len(waiting) * (now - t_last)
@@ -500,32 +526,40 @@ class Server:
             if now - last_epoch >= self.epoch_s:
                 last_epoch = now
                 self._admit(waiting, setting_up, pool, st, now)
-            # the next cycle's frames are made while the GPU runs the current one;
-            # the current one's frames are handed out while the next one runs
+            # The next cycle's frames are made while the GPU runs the current one;
+            # the next cycle is enqueued as soon as the current one ends, before
+            # the host receives the current one's frames (the output samplers hold
+            # two frames), so the GPU waits only for the enqueue.  A membership
+            # change (a set-up session joins, a finished one leaves) first drains
+            # what is in flight.
             t0 = clock()
-            nxt = self._prepare(now)
+            if self._membership_due(setting_up) or not self.sample_outputs:
+                while inflight:
+                    self._finish(inflight.popleft(), clock, st)
+                self._membership(setting_up, st, clock())
             t1 = clock()
-            st.phase("prepare", t1 - t0)
-            done = None
-            if inflight is not None:
-                got, t_got = self._receive(inflight, clock)
-                done, inflight = (inflight, got, t_got), None
-                t2 = clock()
-                st.phase("receive", t2 - t1)
-                t1 = t2
-            self._membership(setting_up, st, clock())
+            st.phase("membership", t1 - t0)
+            if inflight and t1 - t_enq < self.cycle_gap_s:
+                time.sleep(self.cycle_gap_s - (t1 - t_enq))
+                now = t1 = clock()
+            nxt = self._prepare(now)
             t2 = clock()
-            st.phase("membership", t2 - t1)
-            if nxt is not None:
-                self._enqueue(nxt)
-                inflight = nxt
+            st.phase("prepare", t2 - t1)
+            if inflight:
+                while not self.batch.query():      # the current cycle's end, polled
+                    time.sleep(0)
             t3 = clock()
-            st.phase("enqueue", t3 - t2)
-            if done is not None:
-                self._account(*done, st)
-                self._observe(st, clock())
-            st.phase("account", clock() - t3)
-            if inflight is not None:
+            st.phase("gpu_wait", t3 - t2)
+            if nxt is not None:
+                t_enq = clock()
+                self._enqueue(nxt)
+            t4 = clock()
+            st.phase("enqueue", t4 - t3)
+            if inflight:
+                self._finish(inflight.popleft(), clock, st)
+            if nxt is not None:
+                inflight.append(nxt)
+            if inflight:
                 continue
             if not self.resident and not pending and not waiting and not setting_up:
                 with self._ready_lock:
@@ -539,9 +573,8 @@ class Server:
             t_sl = clock()
             time.sleep(max(0.0, min(nxt_t - t_sl, 0.002)))
             self._idle_s += clock() - t_sl
-        if inflight is not None:
-            got, t_got = self._receive(inflight, clock)
-            self._account(inflight, got, t_got, st)
+        while inflight:
+            self._finish(inflight.popleft(), clock, st)
         st.wall_s = clock()
         st.config["capacity_estimate_end"] = self.capacity_est
         st.config["loop_idle_frac"] = self._idle_s / max(st.wall_s, 1e-9)

@@ -231,8 +231,8 @@ int bsm_smoother_centers(int32_t radius, uint32_t v_size, uint32_t u_size, int32
  *      tract channel of src/subcortex/tract_channel.rs:369-480) ----
  * A sampler reads bytes [offset, offset + len) of one buffer kind of a layer
  * (layer ignored for BSM_AXN_STATES) at the end of every cycle the area steps,
- * plain or batched.  Channel semantics: a cycle sends a frame when the previous
- * one has been received; otherwise it waits for the receiver when backpressure
+ * plain or batched.  Channel semantics (one held frame, the default): a cycle
+ * sends a frame when the previous one has been received; otherwise it waits for the receiver when backpressure
  * is on and skips the frame when it is off, so a receiver always gets the frame
  * of the first cycle after its last receive.  Frames are gathered on the device
  * into pinned host slots (one kernel and one copy per cycle for all samplers of
@@ -254,6 +254,13 @@ int bsm_sampler_recv_many(bsm_sampler* const* s, uint32_t n, int wait_for_frame,
                           void* const* dst, const uint64_t* dst_len, uint64_t* cycles, int* got);
 /* Sets backpressure and returns the prior state in *prior (may be null). */
 int bsm_sampler_set_backpressure(bsm_sampler* s, int backpressure, int* prior);
+/* Frames the sampler holds at most, 1 to 3: SamplerBufferKind Single, Double and
+ * Triple (src/cortex/cortical_area.rs:52-58; the reference implements Single
+ * only, :1041-1048, and its channel buffer kinds Double and Triple are
+ * unimplemented, src/subcortex/tract_channel.rs:259-290).  A cycle sends a
+ * frame while fewer are held, else skips it or, with backpressure, waits; recv
+ * takes the oldest held frame, so frames arrive in cycle order.  Default 1. */
+int bsm_sampler_set_buffers(bsm_sampler* s, uint32_t n_frames);
 int bsm_sampler_info(bsm_sampler* s, uint64_t* len, int* backpressure, int* pending);
 int bsm_sampler_destroy(bsm_sampler* s);
 

@@ -63,6 +63,41 @@ def test_plain_sampler_frames_every_kind(ctx):
     a.close()
 
 
+@pytest.mark.parametrize("held", [2, 3])
+def test_multi_buffer_sampler_holds_frames_in_order(ctx, held):
+    """Double and Triple buffer kinds (declared upstream, src/cortex/
+    cortical_area.rs:52-58): up to `held` frames wait for the receiver, in
+    cycle order, each equal to a read after its cycle; later ones are skipped
+    until a frame is received, then sending resumes."""
+    from bismit_amd.sampler import SamplerBufferKind
+    bk = {2: SamplerBufferKind.DOUBLE, 3: SamplerBufferKind.TRIPLE}[held]
+    scheme = recipes.config1_area()
+    a = CorticalArea(scheme, ctx, rnd_seed=17, layout_seed=18)
+    kinds = [SamplerKind.Axons(None), SamplerKind.SynStates(1), SamplerKind.SomaStates(1)]
+    rxs = [a.sampler(k, bk) for k in kinds]
+    fr = glyphs(scheme, 12, 19)
+    want = []
+    for c in range(held + 2):              # cycles held+1, held+2 are skipped
+        a.intake(fr[c])
+        a.cycle(1)
+        want.append([expected(a, k) for k in kinds])
+    for j in range(held):
+        for k, rx in zip(kinds, rxs):
+            f = rx.recv(False)
+            assert f is not None and f.cycle == j + 1, (k, j)
+            assert np.array_equal(f.data, want[j][kinds.index(k)]), (k, j)
+    assert all(rx.recv(False) is None for rx in rxs)
+    a.intake(fr[held + 2])
+    a.cycle(1)
+    after = [expected(a, k) for k in kinds]
+    for k, rx in zip(kinds, rxs):
+        f = rx.recv(False)
+        assert f is not None and f.cycle == held + 3 and np.array_equal(f.data, after[kinds.index(k)])
+    for rx in rxs:
+        rx.close()
+    a.close()
+
+
 def test_sampler_skips_without_backpressure(ctx):
     scheme = recipes.config1_area()
     a = CorticalArea(scheme, ctx, rnd_seed=7, layout_seed=8)

@@ -182,3 +182,28 @@ def test_symbol_stream():
     syms = inputs.SymbolStream(seed=2).symbols(20000)
     counts = np.bincount(syms, minlength=96)
     assert counts.argmax() < 20  # Zipf head dominates
+
+
+def test_sdr_packed_many_streams_equal_each_stream_alone():
+    """next_packed over many streams (split over threads in the library) gives
+    every stream the frame it gives alone, in both the dense and the SDR form."""
+    from bismit_amd.inputs import SdrStream, next_packed
+    rng = np.random.default_rng(3)
+    sides = [8 * int(rng.integers(8, 33)) for _ in range(130)]
+    a = [SdrStream(s * s, 0.02, 0.5, seed=i) for i, s in enumerate(sides)]
+    b = [SdrStream(s * s, 0.02, 0.5, seed=i) for i, s in enumerate(sides)]
+    for sdr in (True, False, True):
+        sizes = [4 * (st.k + 1) if sdr else st.n for st in a]
+        ofs = np.zeros(len(sizes), np.uint64)
+        ofs[1:] = np.cumsum(sizes[:-1], dtype=np.uint64)
+        buf = np.zeros(int(ofs[-1]) + sizes[-1], np.uint8)
+        next_packed(a, buf, ofs, sdr=sdr)
+        for i, st in enumerate(b):
+            got = buf[int(ofs[i]):int(ofs[i]) + sizes[i]]
+            if sdr:
+                e = got.view(np.uint32)
+                assert e[0] == st.k and np.array_equal(e[1:], st.next_indices())
+            else:
+                assert np.array_equal(got, st.next_frame())
+    for st in a + b:
+        st.close()
