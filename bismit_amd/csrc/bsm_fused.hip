@@ -152,11 +152,25 @@ __device__ __forceinline__ uint32_t sg_pos(uint32_t i) { return 8u * (i & 3u) + 
 // writes it meanwhile.
 constexpr uint32_t TAIL_PD = 4;
 constexpr uint32_t TAIL_PC = 1;
+// Quad path (4 | dendrites per cell): thread t takes the 4-dendrite groups
+// t, t + nth, ... (4 consecutive dendrites of one cell, one 32-bit word of each
+// dendrite array); the first TAIL_PQ groups' words are loaded at tile start as
+// whole words, so no byte has to be unpacked before the synapse loop.
+constexpr uint32_t TAIL_PQ = 2;
 struct TailPre {
-    uint32_t act[TAIL_PD / 4];   // dendrite activity ratings
-    uint32_t en[TAIL_PD / 4];    // SSC soma energies (dendrite index < cells)
+    uint32_t act[TAIL_PQ];       // dendrite activity ratings (4 per word)
+    uint32_t en[TAIL_PQ];        // SSC soma energies (dendrite index < cells)
     uint32_t tft[TAIL_PC];       // best_id, best_raw, best, state of a cell-tuft
 };
+
+// Whether a job's epilogue takes the quad path: dendrites per cell a multiple
+// of 4 (a group never spans two cells, and its words are 4-byte aligned) and,
+// with SSC energies, a multiple-of-4 cell count (a group is wholly below it or
+// wholly above).
+__device__ inline bool tail_quad(const TuftJob& J) {
+    return (J.dens & 3u) == 0 && (J.den_idz & 3u) == 0 &&
+           (J.ssc_energy == nullptr || (J.C & 3u) == 0);
+}
 
 __device__ inline uint32_t tile_cell(uint32_t ct, uint32_t tvn, uint32_t tun, uint32_t v0,
                                      uint32_t u0, uint32_t VU, uint32_t U) {
@@ -215,6 +229,23 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
     // so that they share one round trip
     const uint32_t dens = J.dens;
     auto den_act = BSM_G(const uint8_t, J.den_act);
+    if (tail_quad(J)) {
+        // whole words, kept as loaded (the epilogue picks their bytes); indices
+        // past the tile are clamped to group 0 and the words go unused
+        const bool has_en = J.ssc_energy != nullptr;
+#pragma unroll
+        for (uint32_t k = 0; k < TAIL_PQ; k++) {
+            const uint32_t x0 = 4u * (tid + k * nth);
+            uint32_t dj;
+            const uint32_t ct = map.den_cell(x0 < nd_t ? x0 : 0u, dj);
+            const uint32_t dl = map.cell(ct) * dens + dj;
+            pre.act[k] = *BSM_G(const uint32_t, J.den_act + dl);
+            const uint32_t dg = J.den_idz + dl;
+            pre.en[k] = has_en ? *BSM_G(const uint32_t, J.ssc_energy + (dg < J.C ? dg : 0u)) : 0u;
+        }
+        pre.tft[0] = 0u;   // loaded at the epilogue's start
+        return;
+    }
     // no branch around a load: a layer without energies or tuft arrays reads
     // its dendrite activities in their place (masked out below)
     const bool has_en = J.ssc_energy != nullptr, has_tft = J.tft[0] != nullptr;
@@ -242,7 +273,7 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
             tv[k][b] = BSM_G(const uint8_t, has_tft ? (const uint8_t*)J.tft[b] : J.den_act)[has_tft ? cel : 0u];
     }
 #pragma unroll
-    for (uint32_t k = 0; k < TAIL_PD / 4; k++) {
+    for (uint32_t k = 0; k < TAIL_PQ; k++) {
         pre.act[k] = 0;
         pre.en[k] = 0;
     }
@@ -315,11 +346,73 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             denc[nd_t + x] = (uint8_t)st_o;
         }
     };
+    const bool quad = tail_quad(J);
+    // quad path: the first cell's current tuft bytes, loaded now and used by
+    // the tuft pass after the dendrites (their latency hides behind them)
+    uint32_t tb[4] = {0, 0, 0, 0};
+    if (quad && do_tft) {
+        const uint32_t cel = map.cell(tid < cells_t ? tid : 0u);
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) tb[b] = BSM_G(const uint8_t, J.tft[b])[cel];
+    }
+    // 4 dendrites of one cell at a time: words of every dendrite array
+    auto den_quad = [&](uint32_t q, uint32_t actw, uint32_t enw, bool have) {
+        const uint32_t x0 = 4u * q;
+        uint32_t dj0;
+        const uint32_t ct = map.den_cell(x0, dj0);
+        const uint32_t dl0 = map.cell(ct) * dens + dj0;
+        const uint32_t dg0 = J.den_idz + dl0;
+        const bool en_on = J.ssc_energy != nullptr && dg0 < J.C;
+        if (!have) {
+            actw = *BSM_G(const uint32_t, J.den_act + dl0);
+            enw = en_on ? *BSM_G(const uint32_t, J.ssc_energy + dg0) : 0u;
+        }
+        uint32_t ao = 0, ro = 0, so = 0, eo = enw;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; b++) {
+            const uint32_t pv = acc[x0 + b];
+            uint32_t sum = pv & 0xFFFFu;
+            const uint32_t raw = pv >> 16;
+            const int32_t active = sum > J.thresh;
+            sum = active ? sum : 0u;
+            const uint32_t raw_o = min(raw >> 4, 255u);
+            uint32_t st_o = min(sum >> 4, 255u);
+            const int32_t seed = (int32_t)((raw + J.thresh) | (dg0 + b));
+            ao |= activity((actw >> (8 * b)) & 0xFFu, active, rnd, seed, DEN_ACT_DECAY) << (8 * b);
+            if (en_on) {
+                // ssc_cycle (cl/bismit.cl:849-884) on soma index dg == dendrite dg
+                const uint32_t e = (enw >> (8 * b)) & 0xFFu;
+                const uint32_t restless = (e > 191) & (st_o != 0);
+                const uint32_t dark = e == 255;
+                if (dark | restless) eo = (eo & ~(0xFFu << (8 * b))) | ((e - 64) << (8 * b));
+                st_o = min(st_o + 255u * restless + 255u * dark, 255u);
+            }
+            ro |= raw_o << (8 * b);
+            so |= st_o << (8 * b);
+        }
+        *BSM_G(uint32_t, J.den_act + dl0) = ao;
+        *BSM_G(uint32_t, J.den_raw + dl0) = ro;
+        *BSM_G(uint32_t, J.den_st + dl0) = so;
+        if (en_on) *BSM_G(uint32_t, J.ssc_energy + dg0) = eo;
+        if (do_tft) {
+            *(uint32_t*)(denc + x0) = ro;
+            *(uint32_t*)(denc + nd_t + x0) = so;
+        }
+    };
+    if (quad) {
+        const uint32_t ng = nd_t / 4;
+#pragma unroll
+        for (uint32_t k = 0; k < TAIL_PQ; k++) {
+            const uint32_t q = tid + k * nth;
+            if (q < ng) den_quad(q, pre.act[k], pre.en[k], true);
+        }
+        for (uint32_t q = tid + TAIL_PQ * nth; q < ng; q += nth) den_quad(q, 0u, 0u, false);
+    }
     // the inputs of the next TAIL_PE dendrites of each thread are loaded
     // together before any dendrite is finished (indices clamped, unused loads
     // discarded), so that they share one round trip
     constexpr uint32_t TAIL_PE = 2;
-    const bool more = nd_t > TAIL_PD * nth;
+    const bool more = !quad && nd_t > TAIL_PD * nth;
     uint32_t a2[TAIL_PE], e2[TAIL_PE];
     if (more) {
         const bool has_en = J.ssc_energy != nullptr;
@@ -337,7 +430,7 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
 #pragma unroll
     for (uint32_t k = 0; k < TAIL_PD; k++) {
         const uint32_t x = tid + k * nth;
-        if (x < nd_t)
+        if (!quad && x < nd_t)
             den_one(x, (pre.act[k / 4] >> (8 * (k % 4))) & 0xFFu,
                     (pre.en[k / 4] >> (8 * (k % 4))) & 0xFFu, true);
     }
@@ -348,7 +441,8 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             if (x < nd_t) den_one(x, a2[k], e2[k], true);
         }
     }
-    for (uint32_t x = tid + (TAIL_PD + TAIL_PE) * nth; x < nd_t; x += nth) den_one(x, 0u, 0u, false);
+    if (!quad)
+        for (uint32_t x = tid + (TAIL_PD + TAIL_PE) * nth; x < nd_t; x += nth) den_one(x, 0u, 0u, false);
 
     // ---- tufts (tft_cycle)
     if (do_tft) {
@@ -370,7 +464,7 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             uint32_t cur = 0;
 #pragma unroll
             for (uint32_t q = 0; q < TAIL_PC; q++)
-                if (k == q) cur = pre.tft[q];
+                if (k == q) cur = quad ? tb[0] | (tb[1] << 8) | (tb[2] << 16) | (tb[3] << 24) : pre.tft[q];
             if (k >= TAIL_PC)
                 cur = (uint32_t)t0[cel] | ((uint32_t)t1[cel] << 8) | ((uint32_t)t2[cel] << 16) |
                       ((uint32_t)t3[cel] << 24);
