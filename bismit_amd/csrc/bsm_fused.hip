@@ -163,13 +163,15 @@ struct TailPre {
     uint32_t tft[TAIL_PC];       // best_id, best_raw, best, state of a cell-tuft
 };
 
-// Whether a job's epilogue takes the quad path: dendrites per cell a multiple
-// of 4 (a group never spans two cells, and its words are 4-byte aligned) and,
-// with SSC energies, a multiple-of-4 cell count (a group is wholly below it or
-// wholly above).
-__device__ inline bool tail_quad(const TuftJob& J) {
+// Whether a tile's epilogue takes the quad path: dendrites per cell a multiple
+// of 4 (a group never spans two cells, and its words are 4-byte aligned), with
+// SSC energies a multiple-of-4 cell count (a group is wholly below it or wholly
+// above), and at least one group per thread (smaller tiles keep every thread
+// busy on single dendrites: Config 2's spiny stellate tile, 512 dendrites, took
+// 0.7 us longer with groups).
+__device__ inline bool tail_quad(const TuftJob& J, uint32_t nd_t, uint32_t nth) {
     return (J.dens & 3u) == 0 && (J.den_idz & 3u) == 0 &&
-           (J.ssc_energy == nullptr || (J.C & 3u) == 0);
+           (J.ssc_energy == nullptr || (J.C & 3u) == 0) && nd_t >= 4u * nth;
 }
 
 __device__ inline uint32_t tile_cell(uint32_t ct, uint32_t tvn, uint32_t tun, uint32_t v0,
@@ -229,7 +231,7 @@ __device__ inline void tail_prefetch(const TuftJob& J, const Map& map, uint32_t 
     // so that they share one round trip
     const uint32_t dens = J.dens;
     auto den_act = BSM_G(const uint8_t, J.den_act);
-    if (tail_quad(J)) {
+    if (tail_quad(J, nd_t, nth)) {
         // whole words, kept as loaded (the epilogue picks their bytes); indices
         // past the tile are clamped to group 0 and the words go unused
         const bool has_en = J.ssc_energy != nullptr;
@@ -346,7 +348,7 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
             denc[nd_t + x] = (uint8_t)st_o;
         }
     };
-    const bool quad = tail_quad(J);
+    const bool quad = tail_quad(J, nd_t, nth);
     // quad path: the first cell's current tuft bytes, loaded now and used by
     // the tuft pass after the dendrites (their latency hides behind them)
     uint32_t tb[4] = {0, 0, 0, 0};
@@ -450,11 +452,26 @@ __device__ inline void tuft_tail(const TuftJob& J, uint8_t* lds, const uint32_t*
         const uint32_t l2 = J.l2;
         for (uint32_t ct = tid, k = 0; ct < cells_t; ct += nth, k++) {
             uint32_t best_id = 0, best_raw = 0, best = 0, asum = 0;
-            for (uint32_t j = 0; j < dens; j++) {
-                const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
-                if (sr > best_raw) { best_id = j; best_raw = sr; }
-                best = max(best, s);
-                asum += s;
+            if (quad) {
+                // the cell's dendrite bytes read four at a time
+                for (uint32_t j4 = 0; j4 < dens; j4 += 4) {
+                    const uint32_t wr = *(const uint32_t*)(denc + ct * dens + j4);
+                    const uint32_t ws = *(const uint32_t*)(denc + nd_t + ct * dens + j4);
+                    asum = __builtin_amdgcn_sad_u8(ws, 0u, asum);
+#pragma unroll
+                    for (uint32_t b = 0; b < 4; b++) {
+                        const uint32_t sr = (wr >> (8 * b)) & 0xFFu;
+                        if (sr > best_raw) { best_id = j4 + b; best_raw = sr; }
+                        best = max(best, (ws >> (8 * b)) & 0xFFu);
+                    }
+                }
+            } else {
+                for (uint32_t j = 0; j < dens; j++) {
+                    const uint32_t sr = denc[ct * dens + j], s = denc[nd_t + ct * dens + j];
+                    if (sr > best_raw) { best_id = j; best_raw = sr; }
+                    best = max(best, s);
+                    asum += s;
+                }
             }
             const uint32_t tstate = asum >= (1u << (l2 + 8)) ? 255u : (asum >> l2);
             const uint32_t cel = map.cell(ct);
