@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         const uint32_t pu = __builtin_amdgcn_readfirstlane(p) & 1u;
         const uint32_t rows = depth * tvn, cells_t = rows * tun, nd_t = cells_t * dens;
         for (uint32_t x = tid; x < nd_t; x += NTH) acc[x] = 0u;
-        const SliceRt s0 = J.slices[J.staged[0]];
+        const SliceRt s0 = J.st0;
         const uint32_t idx = (uint32_t)wi.tv * J.ntu + wi.tu;   // the item among the job's
         if (idx == 0 && tid == 0) {
             // bit pu: states virtual; bit 2 + pu: best-dendrite fired bits kept
@@ -1648,6 +1648,7 @@ TuftJob make_tuft_job(bsm_area* a, uint32_t l, uint32_t t, const Tuning& tn, con
     j.par_xor = 0;
     const uint32_t n = stage_slices(a, T, j.staged);
     j.n_staged = n;
+    j.st0 = n ? a->slices_h[j.staged[0]] : SliceRt{};
     for (uint32_t k = 0; k < n; k++) {
         const SliceRt& s = a->slices_h[j.staged[k]];
         j.st_vsize[k] = s.v_size;

@@ -96,6 +96,8 @@ struct TuftJob {
     uint32_t l2;
     uint32_t n_staged;
     uint32_t staged[4];
+    SliceRt st0;           // geometry of staged[0] (the check-free gather's windows),
+                           // copied here so that a tile reads it with the job
     uint32_t scalar_quirk; // syns_per_tft % 4 != 0: char-wrapped mids (scalar kernel)
     uint32_t halo;         // staged reach around the tile, in source-slice columns
     uint32_t par_xor;      // state buffer written = st[(par ^ par_xor) & 1]
