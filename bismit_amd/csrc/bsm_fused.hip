@@ -154,9 +154,9 @@ constexpr uint32_t TAIL_PD = 4;
 constexpr uint32_t TAIL_PC = 1;
 // Quad path (4 | dendrites per cell): thread t takes the 4-dendrite groups
 // t, t + nth, ... (4 consecutive dendrites of one cell, one 32-bit word of each
-// dendrite array); the first TAIL_PQ groups' words are loaded at tile start as
+// dendrite array); the first group's words (TAIL_PQ) are loaded at tile start as
 // whole words, so no byte has to be unpacked before the synapse loop.
-constexpr uint32_t TAIL_PQ = 2;
+constexpr uint32_t TAIL_PQ = 1;
 struct TailPre {
     uint32_t act[TAIL_PQ];       // dendrite activity ratings (4 per word)
     uint32_t en[TAIL_PQ];        // SSC soma energies (dendrite index < cells)
