@@ -216,7 +216,7 @@ class Server:
                  use_graph=False, on_retire=None, setup_workers=4, regrow_every=201,
                  regrow_reach=6, sample_outputs=True, on_output=None, max_wait_s=0.25,
                  capacity=1.0e12, target_util=0.8, max_lag_s=0.1, initial_capacity=None,
-                 cycle_gap_s=0.002):
+                 cycle_gap_s=0.002, membership_s=0.1):
         self.ctx = ctx
         # Masked cycles are enqueued directly by default: the batch changes with
         # most admission epochs, and capturing plus instantiating its two graphs
@@ -238,6 +238,11 @@ class Server:
         # cycles (each costs the host a fixed ~0.5-1 ms to build and launch), at
         # a latency cost of at most the gap
         self.cycle_gap_s = cycle_gap_s
+        # joins and retirements are applied together, at most every membership_s
+        # (each change re-plans the batch, ~7 ms for 300 resident areas); a
+        # finished session waits there unstepped, a set-up one starts late and
+        # catches up
+        self.membership_s = membership_s
         self.max_wait_s = max_wait_s          # None: sessions wait as long as it takes
         self.capacity = capacity              # synapse-updates/s the GPU sustains at most
         self.initial_capacity = initial_capacity or capacity
@@ -507,6 +512,7 @@ class Server:
         t_last = 0.0
         inflight = deque()
         t_enq = -1.0
+        t_memb = -1e9
         while True:
             now = clock()
             st.queue_area += len(waiting) * (now - t_last)
@@ -533,10 +539,13 @@ class Server:
             # change (a set-up session joins, a finished one leaves) first drains
             # what is in flight.
             t0 = clock()
-            if self._membership_due(setting_up) or not self.sample_outputs:
+            if (self._membership_due(setting_up) and
+                    (now - t_memb >= self.membership_s or not self.resident)) or \
+                    not self.sample_outputs:
                 while inflight:
                     self._finish(inflight.popleft(), clock, st)
                 self._membership(setting_up, st, clock())
+                t_memb = now
             t1 = clock()
             st.phase("membership", t1 - t0)
             if inflight and t1 - t_enq < self.cycle_gap_s:
