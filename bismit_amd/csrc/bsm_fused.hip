@@ -1173,6 +1173,7 @@ __device__ inline uint32_t hex_uninhibited(const uint8_t* win, uint32_t ctr, uin
 }
 
 constexpr uint32_t F2_MAXC = 4;  // tile cells per thread (tiles up to 4 x 256 cells)
+constexpr uint32_t F2_WK = 7;    // soma window bytes per thread loaded ahead of their stores
 
 template <int W>
 __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jobs,
@@ -1217,7 +1218,26 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                 actp[k] = BSM_G(const uint8_t, J.cel_act)[slc * VU + (v0 + tr) * U + (u0 + tc)];
             }
         }
-        for (uint32_t x = tid; x < WV * WU; x += nth) {
+        // the soma window: every thread issues its first F2_WK bytes' loads
+        // before it stores any (one round trip for windows up to F2_WK * 256
+        // bytes, a 32 x 32 tile at radius 4), bytes off the layer read 0
+        const uint32_t WN = WV * WU;
+        uint32_t wv[F2_WK];
+#pragma unroll
+        for (uint32_t k = 0; k < F2_WK; k++) {
+            const uint32_t x = tid + k * nth;
+            const uint32_t wr = x / WU, wc = x - wr * WU;
+            const int32_t nv = (int32_t)v0 - r + (int32_t)wr, nu = (int32_t)u0 - r + (int32_t)wc;
+            wv[k] = 0;
+            if (x < WN && nv >= 0 && nv < (int32_t)V && nu >= 0 && nu < (int32_t)U)
+                wv[k] = soma[nv * (int32_t)U + nu];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < F2_WK; k++) {
+            const uint32_t x = tid + k * nth;
+            if (x < WN) lds[x] = (uint8_t)wv[k];
+        }
+        for (uint32_t x = tid + F2_WK * nth; x < WN; x += nth) {
             const uint32_t wr = x / WU, wc = x - wr * WU;
             const int32_t nv = (int32_t)v0 - r + (int32_t)wr, nu = (int32_t)u0 - r + (int32_t)wc;
             uint8_t val = 0;
@@ -1341,9 +1361,11 @@ constexpr uint32_t F3_G = 4;  // slices whose per-column inputs are loaded ahead
 // whole block.  Every load of a group is issued before its stores.
 // PRE: the first group's pass-2 inputs are loaded with pass 1's (one memory
 // round trip fewer per item, more registers: for launches of about one
-// workgroup per CU, where occupancy does not matter).
+// workgroup per CU, where occupancy does not matter).  Without PRE the kernel
+// is held to five waves per SIMD (at most 96 VGPRs; without the bound the
+// compiler takes about 103, four waves): many-item launches are latency-bound.
 template <int W, bool PRE>
-__global__ __launch_bounds__(256) void kf_pyr_post(const PyrJob* __restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PRE ? 1 : 5))) void kf_pyr_post(const PyrJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
                                                    uint32_t n_items, uint32_t par, uint32_t rnd_ofs) {
     __shared__ uint32_t lz[F3_COLS * F3_G * BSM_MAX_TUFTS];  // queued dendrite: cell | best << 24
