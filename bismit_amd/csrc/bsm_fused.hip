@@ -1251,11 +1251,15 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
         __syncthreads();
 
         const int32_t imax = (r + INHIB_HORIZ_OFS) * (r + INHIB_HORIZ_OFS);
-        if (J.mode == 0 && r != 4) {
-            // Radii other than the reference's: the generic hex test decides only
-            // cells with a non-zero soma, packed densely so that lanes of silent
-            // cells do not run its loop.  (At radius 4 each cell's thread runs the
-            // unrolled test itself below.)
+        // The hex test decides only cells with a non-zero soma (a few percent of a
+        // spiny stellate layer's), packed densely so that lanes of silent cells do
+        // not run it: for radii other than the reference's (the generic loop), and
+        // at radius 4 for tiles of more than one cell per thread (the unrolled
+        // test, about once per tile instead of once per thread's cell).  Tiles of
+        // one cell per thread run the unrolled test in the cell's own thread
+        // below, with no packing barrier.
+        const bool pack = J.mode == 0 && (r != 4 || tvn * tun > nth);
+        if (pack) {
             for (uint32_t x = tid; x < tvn * tun; x += nth) {
                 const uint32_t tr = x / tun, tc = x - tr * tun;
                 int safe;
@@ -1267,16 +1271,20 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
             for (uint32_t y = tid; y < nh; y += nth) {
                 const uint32_t x = hexl[y];
                 const uint32_t tr = x / tun, tc = x - tr * tun;
-                const int32_t ci = (int32_t)lds[(tr + r) * WU + tc + r] * imax;
                 uint32_t unin = 1;
-                for (int32_t vo = -r; vo <= r && unin; vo++) {
-                    const int32_t uz = max(-r, -vo - r), um = min(r, -vo + r);
-                    const uint8_t* row = lds + (uint32_t)((int32_t)tr + r + vo) * WU + tc + r;
-                    for (int32_t uo = uz; uo <= um; uo++) {
-                        const int32_t w = -vo - uo;
-                        const int32_t dist = (abs(vo) + abs(uo) + abs(w)) >> 1;
-                        const int32_t dc = dist - INHIB_CENTER_OFS;
-                        unin &= (uint32_t)(ci >= (int32_t)row[uo] * (imax - dc * dc));
+                if (r == 4) {
+                    unin = hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU);
+                } else {
+                    const int32_t ci = (int32_t)lds[(tr + r) * WU + tc + r] * imax;
+                    for (int32_t vo = -r; vo <= r && unin; vo++) {
+                        const int32_t uz = max(-r, -vo - r), um = min(r, -vo + r);
+                        const uint8_t* row = lds + (uint32_t)((int32_t)tr + r + vo) * WU + tc + r;
+                        for (int32_t uo = uz; uo <= um; uo++) {
+                            const int32_t w = -vo - uo;
+                            const int32_t dist = (abs(vo) + abs(uo) + abs(w)) >> 1;
+                            const int32_t dc = dist - INHIB_CENTER_OFS;
+                            unin &= (uint32_t)(ci >= (int32_t)row[uo] * (imax - dc * dc));
+                        }
                     }
                 }
                 hexo[x] = (uint8_t)unin;
@@ -1302,8 +1310,8 @@ __global__ __launch_bounds__(256) void kf_ssc_post(const SscJob* __restrict__ jo
                     cs = safe ? cs : 0u;
                     // radius 4, the reference's (cl/control.cl:9): the unrolled test
                     const uint32_t unin =
-                        cs ? (r == 4 ? hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU)
-                                     : (uint32_t)hexo[x])
+                        cs ? (pack ? (uint32_t)hexo[x]
+                                   : hex_uninhibited<4>(lds, (tr + r) * WU + tc + r, WU))
                            : 1u;
                     out = unin ? cs : 0u;
                     is_act = unin & (cs != 0);

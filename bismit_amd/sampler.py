@@ -207,14 +207,6 @@ class ReceiverSet:
             self.cycles[idx] = cyc
             self.got[idx] = got
         return self.got
-        lib = self.rxs[0].lib
-        L.check(lib.bsm_sampler_recv_many(
-            C.cast(L.vptr(self._hs), C.POINTER(C.c_void_p)), len(self.rxs),
-            1 if wait_for_frame else 0, C.cast(L.vptr(self._ds), C.POINTER(C.c_void_p)),
-            C.cast(L.vptr(self._ls), C.POINTER(C.c_uint64)),
-            C.cast(L.vptr(self.cycles), C.POINTER(C.c_uint64)),
-            C.cast(L.vptr(self.got), C.POINTER(C.c_int))))
-        return self.got
 
 
 def create(area, kind: SamplerKind, buffer_kind=SamplerBufferKind.SINGLE, backpressure=False):
