@@ -1370,8 +1370,9 @@ constexpr uint32_t F3_G = 4;  // slices whose per-column inputs are loaded ahead
 // PRE: the first group's pass-2 inputs are loaded with pass 1's (one memory
 // round trip fewer per item, more registers: for launches of about one
 // workgroup per CU, where occupancy does not matter).  Without PRE the kernel
-// is held to five waves per SIMD (at most 96 VGPRs; without the bound the
-// compiler takes about 103, four waves): many-item launches are latency-bound.
+// asks for at least five waves per SIMD (at most 96 VGPRs): many-item launches
+// are latency-bound, and the compiler's own register choice left them at four
+// to six waves (73-103 VGPRs), with the bound five to seven (70-89).
 template <int W, bool PRE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PRE ? 1 : 5))) void kf_pyr_post(const PyrJob* __restrict__ jobs,
                                                    const WorkItem* __restrict__ items,
