@@ -876,23 +876,29 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
                 }
             }
         };
-        // S chunks per iteration (4 when the tile has more than two chunks per
-        // lane, else 2), all loaded before the first is gathered, in order, so
-        // that each gather waits for its own loads only while the later chunks'
-        // loads stay in flight; the loads are unconditional, their indices
-        // clamped to the last chunk (a chunk past the end re-gathers the last
-        // one and is not used)
+        // S chunk slots per lane (4 when the tile has more than two chunks per
+        // lane, else 2), filled in order before the first is gathered; each slot
+        // is refilled with its chunk of the next round as soon as it has been
+        // gathered, so that a lane keeps S - 1 to S chunks' loads in flight
+        // throughout.  The loads are unconditional, their indices clamped to the
+        // last chunk: a chunk past the end re-gathers the last one and is not
+        // used, and the last round's refills are never gathered (a branch around
+        // them, or offsets past the streams' ends for them, measured slower).
         auto run = [&](auto S_, auto MS_) {
             constexpr int S = decltype(S_)::value;
             ChunkMeta<NW> M[S];
+#pragma unroll
+            for (int k = 0; k < S; k++) {
+                fetch(MS_, min(tid + (uint32_t)k * NTH, last), M[k]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
             for (uint32_t q0 = tid; q0 < n_chunks; q0 += S * NTH) {
 #pragma unroll
                 for (int k = 0; k < S; k++) {
-                    fetch(MS_, min(q0 + (uint32_t)k * NTH, last), M[k]);
+                    work(MS_, M[k], q0 + (uint32_t)k * NTH < n_chunks);
+                    fetch(MS_, min(q0 + (uint32_t)(S + k) * NTH, last), M[k]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
-#pragma unroll
-                for (int k = 0; k < S; k++) work(MS_, M[k], q0 + (uint32_t)k * NTH < n_chunks);
             }
         };
         auto run_s = [&](auto MS_) {
