@@ -1863,15 +1863,23 @@ static void largest_first(std::vector<WorkItem>& items, size_t from, Weight&& we
     if (n < 2) return;
     std::vector<uint64_t> w(n);
     for (size_t i = 0; i < n; i++) w[i] = weight(items[from + i]);
-    std::vector<uint64_t> keys(w);
+    // the distinct weights, largest first; a job's items are adjacent and
+    // mostly of one weight, so only a change from the previous item adds a key
+    // and looks its rank up (a serving batch re-plans items of hundreds of areas)
+    std::vector<uint64_t> keys;
+    for (size_t i = 0; i < n; i++)
+        if (i == 0 || w[i] != w[i - 1]) keys.push_back(w[i]);
     std::sort(keys.begin(), keys.end(), std::greater<uint64_t>());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     if (keys.size() == 1) return;
     std::vector<uint32_t> rank(n), start(keys.size() + 1, 0);
+    uint32_t r = 0;
     for (size_t i = 0; i < n; i++) {
-        rank[i] = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), w[i],
-                                              std::greater<uint64_t>()) - keys.begin());
-        start[rank[i] + 1]++;
+        if (i == 0 || w[i] != w[i - 1])
+            r = (uint32_t)(std::lower_bound(keys.begin(), keys.end(), w[i],
+                                            std::greater<uint64_t>()) - keys.begin());
+        rank[i] = r;
+        start[r + 1]++;
     }
     for (size_t k = 0; k < keys.size(); k++) start[k + 1] += start[k];
     std::vector<WorkItem> out(n);
