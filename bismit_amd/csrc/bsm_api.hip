@@ -858,6 +858,8 @@ int bsm_area_create(bsm_ctx* ctx, const bsm_area_desc* desc, uint64_t rnd_seed, 
     a->pjobs_d = nullptr;
     a->items_d = nullptr;
     a->plan_dirty = true;
+    a->fuse_job = -1;
+    a->fuse_lo = a->fuse_len = 0;
     a->batch_stream = nullptr;
     a->batch = nullptr;
     a->layout_epoch = 0;

@@ -207,8 +207,13 @@ int launch_smoother(bsm_area* a, uint32_t layer, hipStream_t s) {
     return 0;
 }
 
-int launch_area_stage(bsm_area* a, hipStream_t s, const Stage& st, uint32_t rnd_ofs) {
+int launch_area_stage(bsm_area* a, hipStream_t s, const Stage& st, uint32_t rnd_ofs,
+                      const uint8_t* frame) {
     if (st.kind == 4) return launch_smoother(a, st.layer, s);
+    if (frame && a->fuse_job >= 0)
+        return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
+                            (uint32_t)a->L[st.layer].cur, rnd_ofs, frame, a->axn + a->fuse_lo,
+                            (uint32_t)a->fuse_job);
     return launch_stage(s, st, a->tjobs_d, a->sjobs_d, a->pjobs_d, a->items_d,
                         (uint32_t)a->L[st.layer].cur, rnd_ofs);
 }

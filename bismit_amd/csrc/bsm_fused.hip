@@ -579,7 +579,10 @@ template <int W, int NST, bool PACKED>
 __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
                                                     const WorkItem* __restrict__ items,
                                                     uint32_t n_items, uint32_t par,
-                                                    uint32_t rnd_ofs, uint32_t stage_cap) {
+                                                    uint32_t rnd_ofs, uint32_t stage_cap,
+                                                    const uint8_t* __restrict__ frame,
+                                                    uint8_t* __restrict__ intake_dst,
+                                                    uint32_t intake_job) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr uint32_t NTH = 256;   // launched with 256 threads
     // the LDS byte address of the dynamic LDS array
@@ -637,7 +640,10 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         // the window bytes (syn_fire of each axon of the tile + halo of each
         // staged slice, zeros off the slice), the item's share of the snapshot
         // of the staged slices (virtual states), the epilogue's inputs
-        auto axs = BSM_G(const uint8_t, J.axn + s0.idz);
+        // fused intake (one staged slice): the window and the snapshot read the
+        // frame, which the item's snapshot share also writes into the slice
+        const bool fin = frame != nullptr && wi.job == intake_job;
+        auto axs = BSM_G(const uint8_t, fin ? frame : J.axn + s0.idz);
         const uint32_t wtot = n_st * WS;
         constexpr int WK = 6;   // window bytes per thread in the first pass
         uint32_t wv[WK], wok = 0;
@@ -693,8 +699,16 @@ __global__ __launch_bounds__(256, 5) void kf_tuft_fast(const TuftJob* __restrict
         }
         {
             auto dst = BSM_G(uint8_t, J.snap + (size_t)pu * J.snap_len);
-            if (snap_lo + tid < snap_hi) dst[snap_lo + tid] = (uint8_t)sv;
-            for (uint32_t x = snap_lo + tid + NTH; x < snap_hi; x += NTH) dst[x] = axs[x];
+            auto ain = BSM_G(uint8_t, intake_dst);
+            if (snap_lo + tid < snap_hi) {
+                dst[snap_lo + tid] = (uint8_t)sv;
+                if (fin) ain[snap_lo + tid] = (uint8_t)sv;
+            }
+            for (uint32_t x = snap_lo + tid + NTH; x < snap_hi; x += NTH) {
+                const uint8_t b = axs[x];
+                dst[x] = b;
+                if (fin) ain[x] = b;
+            }
         }
         // per tile row (slice, v): synapse offset and window offset of its first
         // cell, and the cell itself
@@ -2283,6 +2297,25 @@ int build_plan(bsm_area* a) {
     if (up((void**)&a->sjobs_d, a->sjobs.data(), a->sjobs.size() * sizeof(SscJob))) return 1;
     if (up((void**)&a->pjobs_d, a->pjobs.data(), a->pjobs.size() * sizeof(PyrJob))) return 1;
     if (up((void**)&a->items_d, items.data(), items.size() * sizeof(WorkItem))) return 1;
+    // fused intake: the first stage is the check-free F1 of a layer of one tuft
+    // (job 0, alone in its launch) staging one slice with virtual states, whose
+    // snapshot covers the slice (so the launch's items copy every byte of it)
+    a->fuse_job = -1;
+    a->fuse_lo = a->fuse_len = 0;
+    if (!a->stages.empty() && !a->tjobs.empty()) {
+        const Stage& s0 = a->stages[0];
+        const TuftJob& j = a->tjobs[0];
+        if (s0.kind == 1 && s0.fast && s0.stream == 0 && a->L[s0.layer].d.n_tufts == 1 &&
+            j.fast && j.virt && j.n_staged == 1) {
+            const SliceRt& sl = a->slices_h[j.staged[0]];
+            const uint32_t bytes = (uint32_t)sl.v_size * sl.u_size;
+            if (bytes && j.snap_len == bytes) {
+                a->fuse_job = 0;
+                a->fuse_lo = (uint32_t)sl.idz;
+                a->fuse_len = bytes;
+            }
+        }
+    }
     a->plan_dirty = false;
     return 0;
 }
@@ -2294,9 +2327,8 @@ int build_plan(bsm_area* a) {
         break;
 #define F1_FAST_CASE(w, n, pk)                                                               \
     case ((w * 8 + n) * 2 + pk):                                                             \
-        kf_tuft_fast<w, n, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(tj, it, st.n_items,  \
-                                                                       par, rnd_ofs,         \
-                                                                       st.stage_cap);        \
+        kf_tuft_fast<w, n, pk != 0><<<st.grid, 256, st.lds_bytes, s>>>(                     \
+            tj, it, st.n_items, par, rnd_ofs, st.stage_cap, frame, intake_dst, intake_job);  \
         break;
 
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
@@ -2324,7 +2356,9 @@ int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_
 }
 
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
-                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs) {
+                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs,
+                 const uint8_t* frame, uint8_t* intake_dst, uint32_t intake_job) {
+    if (frame && !(st.kind == 1 && st.fast)) return fail("fused intake needs a check-free F1 stage");
     if (st.n_items == 0) return 0;
     const WorkItem* it = items + st.item0;
     if (st.kind == 1 && st.fast) {
@@ -2395,12 +2429,14 @@ int launch_intake(hipStream_t s, uint8_t* dst, const uint8_t* src, uint32_t len)
 // One plain cycle: every cycled layer's state parity flips once, then the
 // stages run on the area stream (and its side stream).
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial,
-                const std::function<int(hipStream_t)>* intake, bool intake_first) {
+                const std::function<int(hipStream_t)>* intake, bool intake_first,
+                const uint8_t* fused_frame) {
     for (uint32_t l : a->flip_layers) a->L[l].cur ^= 1;
     if (intake && intake_first && (*intake)(a->stream)) return 1;
     if (enqueue_stages(a->stream, a->side, a->stages, serial,
                        [&](const Stage& st, hipStream_t s) {
-                           return launch_area_stage(a, s, st, rnd_ofs);
+                           return launch_area_stage(a, s, st, rnd_ofs,
+                                                    &st == &a->stages[0] ? fused_frame : nullptr);
                        },
                        [&](hipStream_t s) {
                            return intake && !intake_first ? (*intake)(s) : 0;
@@ -2441,11 +2477,18 @@ static int run_cycles(bsm_area* a, uint32_t axn_offset, const uint8_t* frames, u
                       uint32_t len, uint32_t n_cycles) {
     const bool feed = frames && len;
     const bool first = !feed || !side_before_intake(a, axn_offset, len);
+    // A frame of exactly the first stage's staged slice, which no pyramidal
+    // tuft reads, goes in with that launch (no copy launch of its own): the
+    // launch is the first to read the slice and the only one to write it.
+    const bool fuse = feed && !first && a->fuse_job >= 0 && axn_offset == a->fuse_lo &&
+                      len == a->fuse_len;
     for (uint32_t c = 0; c < n_cycles; c++) {
         const std::function<int(hipStream_t)> intake = [&](hipStream_t s) {
             return launch_intake(s, a->axn + axn_offset, frames + (size_t)c * stride, len);
         };
-        if (cycle_fused(a, c * a->n_slots, false, feed ? &intake : nullptr, first)) return 1;
+        if (cycle_fused(a, c * a->n_slots, false, feed && !fuse ? &intake : nullptr, first,
+                        fuse ? frames + (size_t)c * stride : nullptr))
+            return 1;
     }
     return 0;
 }

@@ -385,6 +385,12 @@ struct bsm_area {
     std::vector<uint32_t> flip_layers;  // layers whose state parity flips each cycle
     bsm::Side side;
     bool plan_dirty;
+    // Fused intake: the plan's first stage is the F1 of one tuft (job 0) that
+    // stages one slice, virtual states, whose snapshot covers the whole slice;
+    // a frame of exactly that slice's bytes [fuse_lo, fuse_lo + fuse_len) is
+    // then taken by that launch (fuse_job 0), else fuse_job is -1.
+    int32_t fuse_job;
+    uint32_t fuse_lo, fuse_len;
     hipEvent_t ev0, ev1;
     hipStream_t batch_stream;  // non-null while the area belongs to a batch
     bsm_batch* batch;          // that batch (buffer I/O syncs it first, which also brings
@@ -448,14 +454,22 @@ int regrow_launch(bsm_area* a, hipStream_t s, uint64_t seed, uint64_t epoch, int
                   bool* widened);
 int stage_info(const std::vector<Stage>& stages, uint32_t max_stages, bsm_stage_info* out,
                uint32_t* n_stages);
+// An F1 launch may take a frame for one of its jobs (intake_job): that job's one
+// staged slice is read from the frame, and the launch writes the frame into
+// the slice's axon bytes (intake_dst) beside the job's snapshot copy, in place
+// of a separate intake copy before it.
 int launch_stage(hipStream_t s, const Stage& st, const TuftJob* tj, const SscJob* sj,
-                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs);
+                 const PyrJob* pj, const WorkItem* items, uint32_t par, uint32_t rnd_ofs,
+                 const uint8_t* frame = nullptr, uint8_t* intake_dst = nullptr,
+                 uint32_t intake_job = 0xFFFFFFFFu);
 // One plain cycle on the area stream (and its side stream).  intake, when
 // given, is launched on the area stream before the main stages; the side
 // stages start before it when intake_first is false.
+// fused_frame: this cycle's frame, taken by the area's first stage (its
+// fused intake, bsm_area::fuse_job) instead of an intake copy.
 int cycle_fused(bsm_area* a, uint32_t rnd_ofs, bool serial,
                 const std::function<int(hipStream_t)>* intake = nullptr,
-                bool intake_first = true);
+                bool intake_first = true, const uint8_t* fused_frame = nullptr);
 // Device pointer and length of one buffer kind of the area (the current state
 // buffer for BSM_SYN_STATES; the static flag bits for BSM_SYN_FLAG_SETS).
 int resolve_buf(bsm_area* a, int buf, uint32_t layer, uint8_t** ptr, size_t* len);
@@ -481,7 +495,9 @@ int filter_scratch_setup(int device);
 int launch_smoother(bsm_area* a, uint32_t layer, hipStream_t s);
 void smoother_free(LayerRt& L);
 // A stage of the area's plain plan on stream s (smoother stages included).
-int launch_area_stage(bsm_area* a, hipStream_t s, const Stage& st, uint32_t rnd_ofs);
+// frame: the cycle's frame for the area's fused intake (first stage only).
+int launch_area_stage(bsm_area* a, hipStream_t s, const Stage& st, uint32_t rnd_ofs,
+                      const uint8_t* frame = nullptr);
 int ensure_rnd(bsm_area* a, size_t n_ints);
 void layout_stats_reset(bsm_area* a, uint32_t l, uint32_t t, uint32_t reach, const uint8_t* ids,
                         uint32_t n_ids);

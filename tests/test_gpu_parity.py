@@ -116,6 +116,39 @@ def test_cycle_config1_1000_cycles(ctx):
     g.close()
 
 
+def test_fused_intake_runs_match_oracle(ctx):
+    """The Config 2 recipe at 48x48 columns through multi-cycle calls from
+    device frames: each frame is exactly the spiny stellate tuft's one staged
+    slice, which no pyramidal tuft reads, so the spiny stellate F1 launch takes
+    it (reads its windows from the frame, writes it into the axon slice) in
+    place of an intake copy.  Every buffer equals the oracle's after each call
+    (calls of 1 to 17 cycles), and the input slice holds the call's last frame."""
+    scheme = recipes.cortical_area("c2s", 48, pyr_dens=8, pyr_spd=64)
+    g, o = make_pair(ctx, scheme, layout_seed=3)
+    twin = CorticalArea(scheme, ctx, rnd_seed=0, init_layout=False)
+    off, ln = scheme.input_range()
+    assert ln == 48 * 48
+    frames = inputs.SdrStream(ln, 0.02, 0.5, seed=4).frames(40)
+    dev = ctx.alloc(frames.nbytes)
+    ctx.upload(dev, frames)
+    done = 0
+    for k in (1, 2, 5, 17, 15):
+        print("run", done, k, flush=True)
+        rnd = twin.draw_rnd(k)
+        g.run(C.c_void_p(dev.value + done * ln), ln, ln, k, offset=off)
+        for c in range(k):
+            o.intake(frames[done + c])
+            o.cycle(rnd[c * g.n_slots:(c + 1) * g.n_slots], 1)
+        done += k
+        g.sync()
+        assert np.array_equal(g.axon_states()[off:off + ln], frames[done - 1])
+        compare(g, o, scheme, f"run to cycle {done}")
+    assert np.count_nonzero(g.read("SYN_STRENGTHS", 0)) > 0
+    g.close()
+    twin.close()
+    ctx.free(dev)
+
+
 def test_cycle_config1_multi_cycle_runs(ctx):
     """Config 1 through multi-cycle calls (bsm_area_run: intake + cycle per
     frame from device memory, rnd from the area's own stream; then
