@@ -575,11 +575,12 @@ struct ChunkMeta {
 // item's share of the snapshot of the staged slices is copied instead.  Per
 // dendrite, (sum over non-negative strengths, raw sum) come from a dot product
 // with the strength-sign bits and a sum of absolute differences.
-// Five waves per SIMD (at most 96 VGPRs); launches of one staged slice fit in
-// 80 VGPRs and take six (on Config 3 the two-stream schedule, whose side launch
-// runs beside them, 0.342 -> 0.335 ms per batched cycle).
+// Five waves per SIMD (at most 96 VGPRs); launches of one staged slice, and of
+// chunks under 32 synapses, fit in 80 VGPRs without spills and take six (on
+// Config 3 the two-stream schedule, whose side launch runs beside the others,
+// 0.342 -> 0.332 ms per batched cycle).
 template <int W, int NST, bool PACKED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NST == 1 ? 6 : 5))) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NST == 1 || W < 32) ? 6 : 5))) void kf_tuft_fast(const TuftJob* __restrict__ jobs,
                                                     const WorkItem* __restrict__ items,
                                                     uint32_t n_items, uint32_t par,
                                                     uint32_t rnd_ofs, uint32_t stage_cap,
