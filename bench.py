@@ -360,7 +360,7 @@ def main():
                          "4x32, 4x64, 8x16 and 8x32 on MI355X, 1.5%% ahead of 4x32)")
     ap.add_argument("--chunk", type=int, default=32,
                     help="synapses per lane per chunk of the plain form's synapse stage "
-                         "(default 32: about 4% faster per cycle than 16 at 4x32 tiles)")
+                         "(default 32: about 4%% faster per cycle than 16 at 4x32 tiles)")
     ap.add_argument("--inhib-tile", type=int, default=0,
                     help="side of the inhibition tile (8, 16, 32; 0: the planner's choice)")
     ap.add_argument("--schedule", choices=("one", "side"), default="one",
